@@ -1,0 +1,144 @@
+/*
+ * mhdg — MI355X (gfx950) Hall-MHD step on the signed cell -> face -> edge -> node complex.
+ *
+ * C-ABI of the device implementation of MPCNS_Uns's `mhd_common` hot path.  Plain pointers and sizes
+ * only.  Every function returns 0 on success and a non-zero status otherwise; the message of the last
+ * failure is available through mhdg_last_error().  Arrays follow the reference's layout: entity-numbered,
+ * AoS `id*ncomp + comp` (reference Common/state/FieldSet.h:2), fluxes in the canonical face orientation,
+ * EMFs as line integrals along the stored edge orientation.
+ *
+ * One context holds one mesh (the whole complex of one GPU: the reference's [owned | ghost] numbering
+ * with no ghosts), its device-resident state and one HIP stream.  Calls on one context are not
+ * thread-safe; contexts are independent.
+ */
+#ifndef MHDG_H
+#define MHDG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhdg_ctx mhdg_ctx;
+
+/* Signed CSR incidence table (reference include/uns/complex/Incidence.h:10-28): row r lists
+ * idx[ptr[r] .. ptr[r+1]) with signs sgn (+1/-1). */
+typedef struct mhdg_incidence {
+    const int64_t *ptr;
+    const int32_t *idx;
+    const int8_t *sgn;
+} mhdg_incidence;
+
+/* Mesh, metric and boundary arrays (reference CellComplex.h:21-53, Geometry.h:25-43,
+ * BoundaryInfo.h:21-29).  f2c and e2f must be the column-sorted transposes of c2f and f2e. */
+typedef struct mhdg_mesh_desc {
+    int32_t n_edges, n_faces, n_cells;
+    mhdg_incidence f2e, c2f, e2f, f2c;
+    const int32_t *cell_material;   /* 0 fluid, 1 solid (uns::Material) */
+    const int8_t *face_kind;        /* 0 Interior, 1 Wall, 2 Inflow, 3 Outflow, 4 Skip (mhd::FaceKind) */
+    const int8_t *edge_mode;        /* 0 free, 1 E = 0, 2 imposed free-stream EMF */
+    const double *face_area;        /* 3F */
+    const double *face_center;      /* 3F */
+    const double *face_area_mag;    /* F  */
+    const double *cell_center;      /* 3C */
+    const double *cell_volume;      /* C  */
+    const double *edge_vec;         /* 3E */
+    const double *edge_mid;         /* 3E */
+    const double *edge_len;         /* E  */
+    const int64_t *ec_ptr;          /* E+1: edge -> incident cells, ascending */
+    const int32_t *ec_cell;
+    const double *ec_weight;        /* w_{e,c} */
+    const double *ef_weight;        /* w_{e,f}, aligned with e2f */
+    const double *star1;            /* E */
+    const double *star2;            /* F */
+} mhdg_mesh_desc;
+
+/* Run configuration: the fields of mhd::Config / mhd::Scales the path reads
+ * (reference Common/config/Config.h:19-96, Common/physics/Scales.h:13-41). */
+typedef struct mhdg_config {
+    double gamma, inver_MA2, M_ref, M_H, M_Na, ne_floor, hall_coef, inver_Rem;
+    double cfl, recon_rel_limit, rho_floor, p_floor, rho_floor_dt;
+    int32_t second_order, lorentz_force, hall, resistive;
+    double eta_r_inner, eta_r_outer, eta_width, hall_r_min, hall_r_max;
+    double ind_rtol;
+    int32_t ind_max_it, ind_restart;
+    double cuct;
+    double fs[2][5];                /* free-stream primitive state (rho, u, v, w, p) per species */
+    double u_inf[3], B_imf[3];
+} mhdg_config;
+
+/* Fills `cfg` with the reference defaults (Config.h:19-96 initialisers, Scales::simple(5/3, 1, 1)). */
+void mhdg_config_defaults(mhdg_config *cfg);
+
+/* ---------------------------------------------------------------------------------------- context */
+/* Uploads mesh + geometry and builds the derived tables of the reference constructors:
+ * Stencil (Reconstruction.cpp:26-78), ImplicitInduction (ImplicitInduction.cpp:10-103), active faces
+ * (Faraday.cpp:7-13).  Replaces the construction of mercury::Solver's members (Solver.cpp:36-44). */
+int mhdg_create(const mhdg_mesh_desc *mesh, const mhdg_config *cfg, mhdg_ctx **out);
+void mhdg_destroy(mhdg_ctx *ctx);
+const char *mhdg_last_error(mhdg_ctx *ctx); /* ctx may be NULL: last creation error */
+int mhdg_device_info(char *buf, int buflen);
+
+/* ------------------------------------------------------------- per-call API (host arrays in/out) */
+/* Each replaces the reference function named; inputs are copied to the device, outputs back. */
+
+/* uns::reconstruct_cell_vector (src/uns/geometry/Reconstruct.cpp:8-46): flux F -> b 3C. */
+int mhdg_cell_vector(mhdg_ctx *ctx, const double *flux, double *b);
+/* mhd::cell_current (Common/induction/Current.cpp:9-31): B 3C -> J 3C. */
+int mhdg_cell_current(mhdg_ctx *ctx, const double *B, double *J);
+/* mhd::Stencil::limited_gradients (Common/fluid/Reconstruction.cpp:80-142): W 5C -> grad 15C.
+ * `ghost` (nullable) is a per-face table of 5 primitives used for boundary faces instead of
+ * ghost_state(); it replaces the GhostFn callback (Reconstruction.h:17-21). */
+int mhdg_limited_gradients(mhdg_ctx *ctx, const double *W, int species, const double fs[5], int limit,
+                           const double *ghost, double *grad);
+/* mhd::fluid_flux_rhs (Common/fluid/FluxRHS.cpp:30-74): W 5C, grad 15C -> rhs 5C. */
+int mhdg_flux_rhs(mhdg_ctx *ctx, int species, const double fs[5], const double *W, const double *grad,
+                  const double *ghost, double *rhs);
+/* mhd::add_ideal_emf (Common/induction/IdealEMF.cpp:25-129): adds into E (E entries). */
+int mhdg_ideal_emf(mhdg_ctx *ctx, const double *u_cell, const double *B_cell_tot, const double *B_cell_ind,
+                   const double *phi_tot, const double *phi_ind, double cuct, double *E);
+/* mhd::faraday_update (Common/induction/Faraday.cpp:15-24): Phi -= dt d1 E on active faces. */
+int mhdg_faraday(mhdg_ctx *ctx, const double *E, double *Phi, double dt);
+/* mhd::faraday_update_edge_dt (Faraday.cpp:27-36): Phi -= d1(dt_e E). */
+int mhdg_faraday_edge_dt(mhdg_ctx *ctx, const double *E, const double *dt_edge, double *Phi);
+/* mhd::div_b (Common/diagnostics/DivB.cpp:8-24): max|d2 B|, max relative, worst cell. */
+int mhdg_div_b(mhdg_ctx *ctx, const double *B, const double *extra, double *max_abs, double *max_rel,
+               int64_t *worst_cell);
+/* mercury::cell_time_steps / local_time_step (MercuryUns/fluid/MercurySources.cpp:65-116). */
+int mhdg_local_dt(mhdg_ctx *ctx, const double *W_H, const double *W_Na, const double *B_tot, double *dt);
+/* mercury::add_sources, Lorentz part (MercurySources.cpp:20-63). */
+int mhdg_add_sources(mhdg_ctx *ctx, int species, const double *W_H, const double *W_Na, const double *B_tot,
+                     const double *J, double dt, double *rhs);
+
+/* ImplicitInduction (Common/induction/ImplicitInduction.h:27-94). */
+int mhdg_ind_set_frozen(mhdg_ctx *ctx, const double *B0_cell, const double *ne_eff_cell);
+int mhdg_ind_apply_F(mhdg_ctx *ctx, const double *phi, double *out);
+/* ImplicitInduction::solve (ImplicitInduction.cpp:299-390).  precision: 0 = FP64 GMRES(m) with
+ * Jacobi and modified Gram-Schmidt (the reference iteration); 1 = mixed precision (FP32 multigrid-
+ * preconditioned inner FGMRES inside FP64 iterative refinement).  The warm-start correction is kept
+ * in the context between calls, as the reference keeps corr_. */
+int mhdg_ind_solve(mhdg_ctx *ctx, const double *Phi_n, const double *E_expl, const int8_t *dirichlet,
+                   double dt, int precision, double *E, int32_t *iterations, double *rel_res);
+
+/* --------------------------------------------------------------- device-resident step (fused path) */
+/* State: U_H, U_Na (5C each, AoS), induced flux Phi (F); background PhiAdd (F) and BaddCell (3C). */
+int mhdg_set_state(mhdg_ctx *ctx, const double *U_H, const double *U_Na, const double *Phi);
+int mhdg_get_state(mhdg_ctx *ctx, double *U_H, double *U_Na, double *Phi);
+int mhdg_set_background(mhdg_ctx *ctx, const double *PhiAdd, const double *BaddCell);
+/* mercury::Solver::step_first_order (MercuryUns/driver/Solver.cpp:228-277) on the resident state:
+ * prepare, CFL step (min with dt_max), limited gradients, Rusanov RHS, Lorentz sources, forward Euler
+ * + floors, ideal UCT EMF + boundary EMF, [implicit Hall/resistive solve], CT.  `precision` as in
+ * mhdg_ind_solve.  Outputs the dt used, GMRES iterations and final relative residual. */
+int mhdg_step(mhdg_ctx *ctx, double dt_max, int precision, double *dt_used, int32_t *iterations,
+              double *rel_res);
+/* Derived cell fields of the last prepared state (Solver.h:97-101): which = 0 W_H, 1 W_Na (5C),
+ * 2 B_ind, 3 B_tot, 4 J, 5 u_plus (3C), 6 E of the last step (E). */
+int mhdg_get_derived(mhdg_ctx *ctx, int which, double *out);
+/* Waits for the context's stream. */
+int mhdg_sync(mhdg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,69 @@
+/*
+ * mhdg host-side mesh setup: complex construction, geometry, boundary classification, background field
+ * and seeded synthetic states.  These run once per mesh on the host (the reference does the same work in
+ * its preprocessing and solver constructors); the per-step path is in mhdg.h.
+ */
+#ifndef MHDG_MESH_H
+#define MHDG_MESH_H
+
+#include <stdint.h>
+
+#include "mhdg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhdg_mesh mhdg_mesh;
+
+/* Synthetic shell (no reference counterpart; the reference generates hexahedral boxes and cubed-sphere
+ * shells, src/uns/generate/*.cpp): lat-lon net n_lat x n_lon, n_r geometric radial layers on
+ * [r_in, r_out] with thickness ratio `ratio`, interior nodes jittered by `jitter` of the local spacing,
+ * every hexahedron split into 6 tetrahedra (3 at the poles).  Tags: 1 = inner, 2 = outer. */
+int mhdg_mesh_shell(int n_lat, int n_lon, int n_r, double r_in, double r_out, double ratio, double jitter,
+                    uint64_t seed, mhdg_mesh **out);
+
+/* Generic polyhedral mesh: cell c has face loops [cell_ptr[c], cell_ptr[c+1]), loop l has the nodes
+ * loop_nodes[loop_ptr[l] .. loop_ptr[l+1]) counter-clockwise seen from outside (3 or 4 nodes).
+ * Replaces uns::ComplexBuilder::add_cell/tag_face/finish (src/uns/complex/ComplexBuilder.cpp:45-134)
+ * followed by uns::compute_geometry (src/uns/geometry/Geometry.cpp:17-169). */
+int mhdg_mesh_from_cells(int32_t n_nodes, const double *xyz, int64_t n_cells, const int32_t *cell_ptr,
+                         const int32_t *loop_ptr, const int32_t *loop_nodes, const int32_t *material,
+                         int64_t n_tags, const int32_t *tag_ptr, const int32_t *tag_nodes, const int32_t *tag_id,
+                         mhdg_mesh **out);
+void mhdg_mesh_free(mhdg_mesh *m);
+const char *mhdg_mesh_error(void);
+
+/* counts[0..7] = nodes, edges, faces, cells, nnz(f2e), nnz(c2f), nnz(e2f), nnz(edge->cell) */
+int mhdg_mesh_counts(const mhdg_mesh *m, int64_t counts[8]);
+/* Pointer to an internal array by name (valid until mhdg_mesh_free): "xyz", "tets", "e2n.ptr", "e2n.idx",
+ * "e2n.sgn", likewise f2e/c2f/e2f/f2c, "face_tag", "material", "face_kind", "edge_mode", and every
+ * geometry array of mhdg_mesh_desc ("face_area", ..., "star1", "star2", "star3", "edge_dual_area",
+ * "cell_min_tet").  *count receives the element count; *elem_size the element size in bytes. */
+int mhdg_mesh_array(const mhdg_mesh *m, const char *name, const void **ptr, int64_t *count, int32_t *elem_size);
+
+/* Boundary classification (mhd::BoundaryInfo::build, Common/boundary/BoundaryInfo.cpp:9-49).
+ * tag_type[t] for tag id t (0..n_tag_types-1): 0 wall, 1 inflow, 2 outflow, 3 farfield;
+ * untagged boundary faces use `default_type`. */
+int mhdg_mesh_boundary(mhdg_mesh *m, const int32_t *tag_type, int32_t n_tag_types, int32_t default_type,
+                       const double u_inf[3]);
+/* Fills a descriptor that points into the mesh's arrays (requires mhdg_mesh_boundary). */
+int mhdg_mesh_desc_fill(const mhdg_mesh *m, mhdg_mesh_desc *desc);
+
+/* Static background (mercury::set_background, MercuryUns/initial/Initial.cpp:66-84): PhiAdd = B_imf.A_f
+ * plus d1 of the dipole potential A = m x (r-c)/(|r-c|^2+soft^2)^{3/2} along the edges; BaddCell its
+ * reconstructed cell vectors. */
+int mhdg_mesh_background(const mhdg_mesh *m, const double B_imf[3], int dipole, const double dip_m[3],
+                         const double dip_c[3], double soft, double *PhiAdd, double *BaddCell);
+/* uns::reconstruct_cell_vector on the host (Reconstruct.cpp:8-46). */
+int mhdg_mesh_cell_vector(const mhdg_mesh *m, const double *flux, double *b);
+/* Seeded synthetic state: free stream fs[species] in every cell with relative Gaussian noise `noise` on
+ * rho and p and noise*|u| on each velocity component, and an induced flux Phi = d1(A) from a seeded edge
+ * potential scaled so that rms(Phi/|A_f|) = noise * rms|BaddCell|.  mt19937_64(seed). */
+int mhdg_mesh_seeded_state(const mhdg_mesh *m, const mhdg_config *cfg, double noise, uint64_t seed,
+                           const double *BaddCell, double *U_H, double *U_Na, double *Phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

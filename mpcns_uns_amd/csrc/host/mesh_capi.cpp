@@ -1,0 +1,280 @@
+// Host C-ABI for mesh setup (include/mhdg_mesh.h).
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+
+#include "../../../include/mhdg_mesh.h"
+#include "complex.h"
+#include "geometry.h"
+#include "shell.h"
+#include "vec3.h"
+
+struct mhdg_mesh
+{
+    mhdg::Complex cx;
+    mhdg::Geometry g;
+    std::vector<double> xyz;
+    std::vector<int32_t> tets;
+    std::vector<int8_t> face_kind, edge_mode;
+    bool have_bc = false;
+};
+
+namespace
+{
+thread_local std::string g_err;
+
+mhdg_mesh *finish_mesh(mhdg_mesh *m)
+{
+    m->g = mhdg::compute_geometry(m->cx, m->xyz.data());
+    return m;
+}
+} // namespace
+
+extern "C" {
+const char *mhdg_mesh_error(void) { return g_err.c_str(); }
+
+int mhdg_mesh_shell(int n_lat, int n_lon, int n_r, double r_in, double r_out, double ratio, double jitter, uint64_t seed, mhdg_mesh **out)
+{
+    try
+    {
+        mhdg::ShellSpec s;
+        s.n_lat = n_lat; s.n_lon = n_lon; s.n_r = n_r; s.r_in = r_in; s.r_out = r_out; s.ratio = ratio; s.jitter = jitter; s.seed = seed;
+        mhdg::ShellMesh sm = mhdg::make_shell(s);
+        auto *m = new mhdg_mesh();
+        m->xyz = std::move(sm.xyz);
+        m->tets = std::move(sm.tets);
+        const int32_t nn = static_cast<int32_t>(m->xyz.size() / 3);
+        m->cx = mhdg::build_tet_complex(nn, m->tets.data(), static_cast<int64_t>(m->tets.size() / 4), sm.tags.data(),
+                                        static_cast<int64_t>(sm.tags.size() / 4));
+        *out = finish_mesh(m);
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+
+int mhdg_mesh_from_cells(int32_t n_nodes, const double *xyz, int64_t n_cells, const int32_t *cell_ptr, const int32_t *loop_ptr,
+                         const int32_t *loop_nodes, const int32_t *material, int64_t n_tags, const int32_t *tag_ptr,
+                         const int32_t *tag_nodes, const int32_t *tag_id, mhdg_mesh **out)
+{
+    try
+    {
+        auto *m = new mhdg_mesh();
+        m->xyz.assign(xyz, xyz + 3 * size_t(n_nodes));
+        mhdg::Builder b(n_nodes, n_cells);
+        std::vector<int32_t> lp;
+        for (int64_t c = 0; c < n_cells; ++c)
+        {
+            const int32_t l0 = cell_ptr[c], l1 = cell_ptr[c + 1];
+            lp.assign(size_t(l1 - l0) + 1, 0);
+            for (int32_t l = l0; l <= l1; ++l) lp[l - l0] = loop_ptr[l] - loop_ptr[l0];
+            b.add_cell(lp.data(), loop_nodes + loop_ptr[l0], l1 - l0, material ? material[c] : 0);
+        }
+        for (int64_t t = 0; t < n_tags; ++t) b.tag_face(tag_nodes + tag_ptr[t], tag_ptr[t + 1] - tag_ptr[t], tag_id[t]);
+        m->cx = b.finish();
+        *out = finish_mesh(m);
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+
+void mhdg_mesh_free(mhdg_mesh *m) { delete m; }
+
+int mhdg_mesh_counts(const mhdg_mesh *m, int64_t c[8])
+{
+    c[0] = m->cx.nn; c[1] = m->cx.ne; c[2] = m->cx.nf; c[3] = m->cx.nc;
+    c[4] = m->cx.f2e.nnz(); c[5] = m->cx.c2f.nnz(); c[6] = m->cx.e2f.nnz(); c[7] = static_cast<int64_t>(m->g.ec_cell.size());
+    return 0;
+}
+
+int mhdg_mesh_array(const mhdg_mesh *m, const char *name, const void **ptr, int64_t *count, int32_t *elem)
+{
+    const std::string n(name);
+    auto give = [&](const auto &v) {
+        *ptr = v.data();
+        *count = static_cast<int64_t>(v.size());
+        *elem = static_cast<int32_t>(sizeof(v[0]));
+        return 0;
+    };
+    const mhdg::Complex &cx = m->cx;
+    const mhdg::Geometry &g = m->g;
+    auto csr = [&](const mhdg::Csr &t, const std::string &part) {
+        if (part == "ptr") return give(t.ptr);
+        if (part == "idx") return give(t.idx);
+        return give(t.sgn);
+    };
+    const auto dotp = n.find('.');
+    if (dotp != std::string::npos)
+    {
+        const std::string tb = n.substr(0, dotp), part = n.substr(dotp + 1);
+        if (tb == "e2n") return csr(cx.e2n, part);
+        if (tb == "f2e") return csr(cx.f2e, part);
+        if (tb == "c2f") return csr(cx.c2f, part);
+        if (tb == "e2f") return csr(cx.e2f, part);
+        if (tb == "f2c") return csr(cx.f2c, part);
+        if (tb == "n2e") return csr(cx.n2e, part);
+        return 1;
+    }
+    if (n == "xyz") return give(m->xyz);
+    if (n == "tets") return give(m->tets);
+    if (n == "face_tag") return give(cx.face_tag);
+    if (n == "material") return give(cx.material);
+    if (n == "face_kind") return give(m->face_kind);
+    if (n == "edge_mode") return give(m->edge_mode);
+    if (n == "edge_vec") return give(g.edge_vec);
+    if (n == "edge_mid") return give(g.edge_mid);
+    if (n == "edge_len") return give(g.edge_len);
+    if (n == "face_area") return give(g.face_area);
+    if (n == "face_center") return give(g.face_center);
+    if (n == "face_area_mag") return give(g.face_area_mag);
+    if (n == "cell_center") return give(g.cell_center);
+    if (n == "cell_volume") return give(g.cell_volume);
+    if (n == "cell_min_tet") return give(g.cell_min_tet);
+    if (n == "edge_dual_area") return give(g.edge_dual_area);
+    if (n == "ec_ptr") return give(g.ec_ptr);
+    if (n == "ec_cell") return give(g.ec_cell);
+    if (n == "ec_weight") return give(g.ec_weight);
+    if (n == "ef_weight") return give(g.ef_weight);
+    if (n == "ec_misoriented") return give(g.ec_misoriented);
+    if (n == "star1") return give(g.star1);
+    if (n == "star2") return give(g.star2);
+    if (n == "star3") return give(g.star3);
+    return 1;
+}
+
+// Common/boundary/BoundaryInfo.cpp:9-49
+int mhdg_mesh_boundary(mhdg_mesh *m, const int32_t *tag_type, int32_t n_types, int32_t default_type, const double u_inf[3])
+{
+    using namespace mhdg;
+    const Complex &cx = m->cx;
+    m->face_kind.assign(cx.nf, 0);
+    m->edge_mode.assign(cx.ne, 0);
+    const V3 u(u_inf[0], u_inf[1], u_inf[2]);
+    for (int32_t f = 0; f < cx.nf; ++f)
+    {
+        const int64_t k0 = cx.f2c.ptr[f];
+        const int nc = static_cast<int>(cx.f2c.ptr[f + 1] - k0);
+        if (nc == 2)
+        {
+            const bool s0 = cx.material[cx.f2c.idx[k0]] == 1, s1 = cx.material[cx.f2c.idx[k0 + 1]] == 1;
+            m->face_kind[f] = (s0 && s1) ? 4 : (s0 || s1) ? 1 : 0;
+            continue;
+        }
+        if (cx.material[cx.f2c.idx[k0]] == 1) { m->face_kind[f] = 4; continue; }
+        const int tag = cx.face_tag[f];
+        const int type = (tag > 0 && tag < n_types && tag_type[tag] >= 0) ? tag_type[tag] : default_type;
+        if (type == 0) m->face_kind[f] = 1;
+        else if (type == 1) m->face_kind[f] = 2;
+        else if (type == 2) m->face_kind[f] = 3;
+        else if (type == 3)
+        {
+            const V3 n_out = (cx.f2c.sgn[k0] / m->g.face_area_mag[f]) * ld3(m->g.face_area.data(), f);
+            m->face_kind[f] = dot(u, n_out) < 0 ? 2 : 3;
+        }
+        else { g_err = "unknown boundary type"; return 1; }
+    }
+    for (int32_t f = 0; f < cx.nf; ++f)
+        if (m->face_kind[f] == 2)
+            for (int64_t k = cx.f2e.ptr[f]; k < cx.f2e.ptr[f + 1]; ++k) m->edge_mode[cx.f2e.idx[k]] = 2;
+    for (int32_t f = 0; f < cx.nf; ++f)
+        if (m->face_kind[f] == 1)
+            for (int64_t k = cx.f2e.ptr[f]; k < cx.f2e.ptr[f + 1]; ++k) m->edge_mode[cx.f2e.idx[k]] = 1;
+    m->have_bc = true;
+    return 0;
+}
+
+int mhdg_mesh_desc_fill(const mhdg_mesh *m, mhdg_mesh_desc *d)
+{
+    if (!m->have_bc) { g_err = "mhdg_mesh_boundary has not been called"; return 1; }
+    const mhdg::Complex &cx = m->cx;
+    const mhdg::Geometry &g = m->g;
+    std::memset(d, 0, sizeof(*d));
+    d->n_edges = cx.ne; d->n_faces = cx.nf; d->n_cells = cx.nc;
+    d->f2e = {cx.f2e.ptr.data(), cx.f2e.idx.data(), cx.f2e.sgn.data()};
+    d->c2f = {cx.c2f.ptr.data(), cx.c2f.idx.data(), cx.c2f.sgn.data()};
+    d->e2f = {cx.e2f.ptr.data(), cx.e2f.idx.data(), cx.e2f.sgn.data()};
+    d->f2c = {cx.f2c.ptr.data(), cx.f2c.idx.data(), cx.f2c.sgn.data()};
+    d->cell_material = cx.material.data();
+    d->face_kind = m->face_kind.data();
+    d->edge_mode = m->edge_mode.data();
+    d->face_area = g.face_area.data(); d->face_center = g.face_center.data(); d->face_area_mag = g.face_area_mag.data();
+    d->cell_center = g.cell_center.data(); d->cell_volume = g.cell_volume.data();
+    d->edge_vec = g.edge_vec.data(); d->edge_mid = g.edge_mid.data(); d->edge_len = g.edge_len.data();
+    d->ec_ptr = g.ec_ptr.data(); d->ec_cell = g.ec_cell.data(); d->ec_weight = g.ec_weight.data(); d->ef_weight = g.ef_weight.data();
+    d->star1 = g.star1.data(); d->star2 = g.star2.data();
+    return 0;
+}
+
+// MercuryUns/initial/Initial.cpp:66-84
+int mhdg_mesh_background(const mhdg_mesh *m, const double B_imf[3], int dipole, const double dm[3], const double dc[3], double soft,
+                         double *PhiAdd, double *BaddCell)
+{
+    using namespace mhdg;
+    const Complex &cx = m->cx;
+    const Geometry &g = m->g;
+    const V3 B(B_imf[0], B_imf[1], B_imf[2]);
+    for (int32_t f = 0; f < cx.nf; ++f) PhiAdd[f] = dot(B, ld3(g.face_area.data(), f));
+    if (dipole)
+    {
+        const V3 mm(dm[0], dm[1], dm[2]), cc(dc[0], dc[1], dc[2]);
+        std::vector<double> A(cx.ne, 0.0);
+        for (int32_t e = 0; e < cx.ne; ++e)
+        {
+            const V3 r = ld3(g.edge_mid.data(), e) - cc;
+            const double d2 = dot(r, r) + soft * soft;
+            const V3 a = (1.0 / (d2 * std::sqrt(d2))) * cross(mm, r);
+            A[e] = dot(a, ld3(g.edge_vec.data(), e));
+        }
+        for (int32_t f = 0; f < cx.nf; ++f)
+            for (int64_t k = cx.f2e.ptr[f]; k < cx.f2e.ptr[f + 1]; ++k) PhiAdd[f] += cx.f2e.sgn[k] * A[cx.f2e.idx[k]];
+    }
+    reconstruct_cell_vector(cx, g, PhiAdd, BaddCell);
+    return 0;
+}
+
+int mhdg_mesh_cell_vector(const mhdg_mesh *m, const double *flux, double *b)
+{
+    mhdg::reconstruct_cell_vector(m->cx, m->g, flux, b);
+    return 0;
+}
+
+int mhdg_mesh_seeded_state(const mhdg_mesh *m, const mhdg_config *cfg, double noise, uint64_t seed, const double *Badd, double *UH,
+                           double *UNa, double *Phi)
+{
+    const mhdg::Complex &cx = m->cx;
+    const mhdg::Geometry &g = m->g;
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> N(0.0, 1.0);
+    const double gm1 = cfg->gamma - 1.0;
+    for (int32_t c = 0; c < cx.nc; ++c)
+        for (int s = 0; s < 2; ++s)
+        {
+            const double *fs = cfg->fs[s];
+            const double umag = std::sqrt(fs[1] * fs[1] + fs[2] * fs[2] + fs[3] * fs[3]);
+            const double rho = fs[0] * (1.0 + noise * N(rng));
+            const double u = fs[1] + noise * umag * N(rng), v = fs[2] + noise * umag * N(rng), w = fs[3] + noise * umag * N(rng);
+            const double p = fs[4] * (1.0 + noise * N(rng));
+            double *U = (s == 0 ? UH : UNa) + 5 * size_t(c);
+            U[0] = rho; U[1] = rho * u; U[2] = rho * v; U[3] = rho * w;
+            U[4] = p / gm1 + 0.5 * rho * (u * u + v * v + w * w);
+        }
+    std::vector<double> A(cx.ne);
+    for (int32_t e = 0; e < cx.ne; ++e) A[e] = N(rng);
+    double s_phi = 0, s_b = 0;
+    for (int32_t f = 0; f < cx.nf; ++f)
+    {
+        double v = 0;
+        for (int64_t k = cx.f2e.ptr[f]; k < cx.f2e.ptr[f + 1]; ++k) v += cx.f2e.sgn[k] * A[cx.f2e.idx[k]];
+        Phi[f] = v;
+        const double bn = v / g.face_area_mag[f];
+        s_phi += bn * bn;
+    }
+    for (int32_t c = 0; c < cx.nc; ++c)
+        s_b += Badd[3 * size_t(c)] * Badd[3 * size_t(c)] + Badd[3 * size_t(c) + 1] * Badd[3 * size_t(c) + 1] + Badd[3 * size_t(c) + 2] * Badd[3 * size_t(c) + 2];
+    const double rms_phi = std::sqrt(s_phi / cx.nf), rms_b = std::sqrt(s_b / cx.nc);
+    const double scale = rms_phi > 0 ? noise * rms_b / rms_phi : 0.0;
+    for (int32_t f = 0; f < cx.nf; ++f) Phi[f] *= scale;
+    return 0;
+}
+}

@@ -1,0 +1,57 @@
+#pragma once
+// Device context: mesh tables, resident state, work buffers and the host drivers of the step and of the
+// implicit induction solves.
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../../include/mhdg.h"
+#include "induction.h"
+#include "mixed.h"
+
+struct mhdg_ctx
+{
+    hipStream_t stream = nullptr;
+    mhdg_config cfg{};
+    mhdg::Params P{};
+    mhdg::DevMesh d{};
+    int32_t nc = 0, nf = 0, ne = 0, n_heavy = 0;
+    void *heavy = nullptr;
+    std::vector<double> taper_h;
+    int8_t *dir_mercury = nullptr;
+    // resident state and derived fields
+    double *U[2]{}, *Phi = nullptr, *PhiAdd = nullptr, *Badd = nullptr;
+    double *W[2]{}, *grad[2]{}, *Bind = nullptr, *Btot = nullptr, *uplus = nullptr, *J = nullptr, *alpha = nullptr, *B0 = nullptr;
+    double *FA = nullptr, *Ef = nullptr, *E_expl = nullptr, *E_last = nullptr, *dt_part = nullptr, *dt_dev = nullptr;
+    // implicit induction work
+    double *phi_w = nullptr, *bcell = nullptr, *gcell = nullptr, *corr = nullptr;
+    bool have_corr = false;
+    double *x0 = nullptr, *bvec = nullptr, *rvec = nullptr, *wvec = nullptr, *invdiag = nullptr, *delta = nullptr, *tmp = nullptr;
+    double *scal = nullptr, *hcol = nullptr, *ydev = nullptr;
+    std::vector<double *> V;
+    double **Vptrs = nullptr;
+    int restart = 2;
+    mhdg::RedScratch rs{};
+    mhdg::MixedSolver *mixed = nullptr;
+    void *host_pin = nullptr;
+    std::vector<void *> owned;
+    std::map<int, std::pair<void *, size_t>> staging;
+    std::string err;
+
+    ~mhdg_ctx();
+    void build(const mhdg_mesh_desc &m, const mhdg_config &c);
+    void *alloc(size_t bytes);
+    template <class T> T *upload(const std::vector<T> &v);
+    template <class T> T *upload(const T *v, size_t n);
+    double *stage(int slot, size_t n_doubles);
+    double *stage_in(int slot, const void *host, size_t n_doubles);
+    void d2h(void *host, const void *dev, size_t bytes);
+    void check_launch(const char *what);
+    void launch_edge_emf(const double *Ef, const double *u, const double *phi_ind, double cuct, const double *Ein, double *Eout, int bc);
+    double dot(const double *a, const double *b, int op, int slot);
+    void apply_op(const double *phi, const double *v, const double *inv, const int8_t *dir, double dt, int mode, int mask, const double *b,
+                  double *out);
+    int solve64(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
+    int solve_mixed(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
+    double step(double dt_max, int precision, int *iters, double *res);
+};

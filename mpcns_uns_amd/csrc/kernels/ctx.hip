@@ -1,0 +1,622 @@
+// Context and C-ABI of the device path (include/mhdg.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/mhdg.h"
+#include "../host/tables.h"
+#include "ctx.h"
+
+using namespace mhdg;
+
+namespace
+{
+thread_local std::string g_create_err;
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+inline int nblk(int64_t n, int bs = 256) { return static_cast<int>((n + bs - 1) / bs); }
+} // namespace
+
+void *mhdg_ctx::alloc(size_t bytes)
+{
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, bytes ? bytes : 8), "hipMalloc");
+    owned.push_back(p);
+    return p;
+}
+template <class T> T *mhdg_ctx::upload(const std::vector<T> &v)
+{
+    T *p = static_cast<T *>(alloc(v.size() * sizeof(T)));
+    if (!v.empty()) hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    return p;
+}
+template <class T> T *mhdg_ctx::upload(const T *v, size_t n)
+{
+    T *p = static_cast<T *>(alloc(n * sizeof(T)));
+    if (n) hip_check(hipMemcpy(p, v, n * sizeof(T), hipMemcpyHostToDevice), "upload");
+    return p;
+}
+double *mhdg_ctx::stage(int slot, size_t n)
+{
+    auto &s = staging[slot];
+    if (s.second < n)
+    {
+        if (s.first) hip_check(hipFree(s.first), "hipFree");
+        hip_check(hipMalloc(&s.first, n * sizeof(double)), "hipMalloc");
+        s.second = n;
+    }
+    return static_cast<double *>(s.first);
+}
+double *mhdg_ctx::stage_in(int slot, const void *host, size_t n_doubles)
+{
+    double *p = stage(slot, n_doubles);
+    hip_check(hipMemcpyAsync(p, host, n_doubles * sizeof(double), hipMemcpyHostToDevice, stream), "H2D");
+    return p;
+}
+void mhdg_ctx::d2h(void *host, const void *dev, size_t bytes)
+{
+    hip_check(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream), "D2H");
+    hip_check(hipStreamSynchronize(stream), "sync");
+}
+void mhdg_ctx::check_launch(const char *what)
+{
+    hip_check(hipGetLastError(), what);
+}
+
+mhdg_ctx::~mhdg_ctx()
+{
+    for (void *p : owned) (void)hipFree(p);
+    for (auto &kv : staging) if (kv.second.first) (void)hipFree(kv.second.first);
+    if (host_pin) (void)hipHostFree(host_pin);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ------------------------------------------------------------------------------------ creation
+static void make_params(const mhdg_config &c, Params &P)
+{
+    std::memset(&P, 0, sizeof(P));
+    P.gamma = c.gamma; P.inver_MA2 = c.inver_MA2; P.M_ref = c.M_ref; P.M_H = c.M_H; P.M_Na = c.M_Na; P.ne_floor = c.ne_floor;
+    P.hall_coef = c.hall_coef; P.cfl = c.cfl; P.rel_limit = c.recon_rel_limit; P.rho_floor = c.rho_floor; P.p_floor = c.p_floor;
+    P.rho_floor_dt = c.rho_floor_dt; P.cuct = c.cuct;
+    P.second_order = c.second_order; P.lorentz = c.lorentz_force; P.hall = c.hall && c.hall_coef > 0;
+    P.resistive = c.resistive && c.inver_Rem > 0;
+    for (int s = 0; s < 2; ++s)
+        for (int m = 0; m < 5; ++m) P.fs[s][m] = c.fs[s][m];
+    // E_inf = -1 * (u_inf x B_imf) (MercuryUns/driver/Solver.cpp:162)
+    const V3 u(c.u_inf[0], c.u_inf[1], c.u_inf[2]), B(c.B_imf[0], c.B_imf[1], c.B_imf[2]);
+    const V3 E = -1.0 * cross(u, B);
+    P.E_inf[0] = E.x; P.E_inf[1] = E.y; P.E_inf[2] = E.z;
+}
+
+void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
+{
+    cfg = c;
+    make_params(c, P);
+    hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+    Tables t = build_tables(m, c, kLightMax);
+    if (t.max_valence > kHeavyMax) throw std::runtime_error("edge valence exceeds the supported maximum");
+    nc = t.nc; nf = t.nf; ne = t.ne;
+    n_heavy = static_cast<int32_t>(t.heavy.size());
+    const size_t E = ne, F = nf, C = nc;
+    d.nc = nc; d.nf = nf; d.ne = ne;
+    d.c2f_ptr = upload(t.c2f_ptr); d.c2f = upload(t.c2f); d.cfs = upload(t.cfs);
+    d.f2e_ptr = upload(t.f2e_ptr); d.f2e = upload(t.f2e);
+    d.e2f_ptr = upload(t.e2f_ptr); d.e2f = upload(t.e2f);
+    d.fL = upload(t.fL); d.fR = upload(t.fR);
+    d.ec_ptr = upload(t.ec_ptr); d.ec_cell = upload(t.ec_cell);
+    d.mat = upload(t.mat8);
+    d.kind = upload(m.face_kind, F); d.emode = upload(m.edge_mode, E);
+    d.farea = upload(m.face_area, 3 * F); d.fcen = upload(m.face_center, 3 * F); d.famag = upload(m.face_area_mag, F);
+    d.ccen = upload(m.cell_center, 3 * C); d.cvol = upload(m.cell_volume, C);
+    d.evec = upload(m.edge_vec, 3 * E); d.emid = upload(m.edge_mid, 3 * E); d.elen = upload(m.edge_len, E);
+    d.ecw = upload(m.ec_weight, size_t(m.ec_ptr[E])); d.efw = upload(m.ef_weight, t.e2f.size());
+    d.star2 = upload(m.star2, F);
+    d.snb = upload(t.snb); d.sw = upload(t.sw); d.sd = upload(t.sd); d.sinv = upload(t.sinv); d.svalid = upload(t.svalid);
+    d.recon = upload(t.recon); d.m_e = upload(t.m_e); d.d2 = upload(t.d2); d.taper = upload(t.taper);
+    heavy = upload(t.heavy);
+    taper_h = t.taper;
+    std::vector<int8_t> dir(E);
+    for (size_t e = 0; e < E; ++e) dir[e] = m.edge_mode[e] == 2;
+    dir_mercury = upload(dir);
+
+    auto D = [&](size_t n) { return static_cast<double *>(alloc(n * sizeof(double))); };
+    U[0] = D(5 * C); U[1] = D(5 * C); Phi = D(F); PhiAdd = D(F); Badd = D(3 * C);
+    hip_check(hipMemset(PhiAdd, 0, F * 8), "memset"); hip_check(hipMemset(Badd, 0, 3 * C * 8), "memset");
+    hip_check(hipMemset(U[0], 0, 5 * C * 8), "memset"); hip_check(hipMemset(U[1], 0, 5 * C * 8), "memset");
+    hip_check(hipMemset(Phi, 0, F * 8), "memset");
+    W[0] = D(5 * C); W[1] = D(5 * C); grad[0] = D(15 * C); grad[1] = D(15 * C);
+    Bind = D(3 * C); Btot = D(3 * C); uplus = D(3 * C); J = D(3 * C); alpha = D(C); B0 = D(3 * C);
+    hip_check(hipMemset(alpha, 0, C * 8), "memset"); hip_check(hipMemset(B0, 0, 3 * C * 8), "memset");
+    FA = D(10 * F); Ef = D(3 * F); E_expl = D(E); E_last = D(E);
+    dt_part = D(size_t(nblk(nc)) + 1); dt_dev = D(1);
+    // induction work
+    phi_w = D(F); bcell = D(3 * C); gcell = D(3 * C); corr = D(E); have_corr = false;
+    x0 = D(E); bvec = D(E); rvec = D(E); wvec = D(E); invdiag = D(E); delta = D(E); tmp = D(E);
+    scal = D(64);
+    rs.partial = D(kRedBlocks);
+    rs.ticket = static_cast<unsigned *>(alloc(64));
+    hip_check(hipMemset(rs.ticket, 0, 64), "memset");
+    restart = std::max(2, c.ind_restart);
+    V.resize(restart + 1);
+    for (auto &v : V) v = D(E);
+    Vptrs = static_cast<double **>(alloc(sizeof(double *) * V.size()));
+    hip_check(hipMemcpy(Vptrs, V.data(), sizeof(double *) * V.size(), hipMemcpyHostToDevice), "upload");
+    hcol = D(restart + 2);
+    ydev = D(restart + 1);
+    hip_check(hipHostMalloc(&host_pin, 4096), "hipHostMalloc");
+    hip_check(hipDeviceSynchronize(), "sync");
+}
+
+// ------------------------------------------------------------------------------------ pieces
+void mhdg_ctx::launch_edge_emf(const double *Efp, const double *u, const double *phi_ind, double cuct, const double *Ein, double *Eout,
+                               int bc)
+{
+    hipLaunchKernelGGL(k_edge_emf, dim3(nblk(ne)), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct, kLightMax, Ein, Eout, bc);
+    if (n_heavy)
+        hipLaunchKernelGGL(k_edge_emf_heavy, dim3(n_heavy), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct,
+                           static_cast<const int32_t *>(heavy), Ein, Eout, bc);
+    check_launch("edge emf");
+}
+
+double mhdg_ctx::dot(const double *a, const double *b, int op, int slot)
+{
+    hipLaunchKernelGGL(k_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, a, b, ne, rs, scal + slot, op);
+    check_launch("dot");
+    double v;
+    d2h(&v, scal + slot, 8);
+    return v;
+}
+
+// F(phi) for a face vector phi into out (mode 0), or A x (mode 1) with x = precond(v) / v, optional mask /
+// residual form.  ImplicitInduction::apply_F / apply_A (ImplicitInduction.cpp:127-198).
+void mhdg_ctx::apply_op(const double *phi_in, const double *v, const double *inv, const int8_t *dir, double dt, int mode, int mask,
+                        const double *b, double *out)
+{
+    const double *phi = phi_in;
+    if (!phi)
+    {
+        hipLaunchKernelGGL(k_d1, dim3(nblk(nf)), dim3(256), 0, stream, d, v, inv, dir, phi_w);
+        phi = phi_w;
+    }
+    if (P.hall)
+    {
+        hipLaunchKernelGGL(k_bcell, dim3(nblk(nc)), dim3(256), 0, stream, d, phi, bcell);
+        hipLaunchKernelGGL(k_hall_g, dim3(nblk(nc)), dim3(256), 0, stream, d, bcell, alpha, B0, gcell);
+    }
+    hipLaunchKernelGGL(k_edge_apply, dim3(nblk(ne)), dim3(256), 0, stream, d, P.resistive, P.hall, phi, gcell, v, inv, dir, dt, mode,
+                       mask, b, out);
+    check_launch("apply_op");
+}
+
+// ImplicitInduction::solve (Common/induction/ImplicitInduction.cpp:299-390), FP64, Jacobi, MGS Arnoldi:
+// the reference's control flow with the Hessenberg / Givens arithmetic on the host.
+int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir, double dt, double *Eout, double *rel_res)
+{
+    const int m = restart;
+    const int max_it = cfg.ind_max_it;
+    const double rtol = cfg.ind_rtol;
+    hipLaunchKernelGGL(k_solve_init, dim3(nblk(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt, ne,
+                       x0, invdiag);
+    apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp); // F(Phi^n)
+    hipLaunchKernelGGL(k_add, dim3(nblk(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
+    apply_op(nullptr, x0, nullptr, dir, dt, 1, 0, bvec, rvec); // r = b - A x0
+    hip_check(hipMemsetAsync(delta, 0, size_t(ne) * 8, stream), "memset");
+    const double bb = dot(bvec, bvec, 0, 0);
+    const double bnorm = std::sqrt(std::max(bb, 1e-300));
+    double beta = dot(rvec, rvec, 1, 1);
+    int total = 0;
+    std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
+    std::vector<double> cs(m), sn(m), g(m + 1), yv(m), hc(m + 2);
+    while (beta > rtol * bnorm && total < max_it)
+    {
+        hipLaunchKernelGGL(k_div_scalar, dim3(nblk(ne)), dim3(256), 0, stream, rvec, scal + 1, ne, V[0]);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        for (; j < m && total < max_it; ++j, ++total)
+        {
+            apply_op(nullptr, V[j], invdiag, dir, dt, 1, 1, nullptr, wvec);
+            for (int i = 0; i <= j; ++i)
+                hipLaunchKernelGGL(k_mgs<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, wvec, i > 0 ? V[i - 1] : nullptr, V[i], ne,
+                                   hcol, i, 0, rs);
+            hipLaunchKernelGGL(k_mgs<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, wvec, V[j], nullptr, ne, hcol, j + 1, 1, rs);
+            hipLaunchKernelGGL(k_scale_div<double>, dim3(nblk(ne)), dim3(256), 0, stream, wvec, hcol + j + 1, ne, V[j + 1]);
+            check_launch("arnoldi");
+            d2h(hc.data(), hcol, sizeof(double) * (j + 2));
+            for (int i = 0; i <= j + 1; ++i) H[i][j] = hc[i];
+            for (int i = 0; i < j; ++i)
+            {
+                const double tt = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
+                H[i + 1][j] = -sn[i] * H[i][j] + cs[i] * H[i + 1][j];
+                H[i][j] = tt;
+            }
+            const double den = std::hypot(H[j][j], H[j + 1][j]);
+            cs[j] = den > 0 ? H[j][j] / den : 1.0; sn[j] = den > 0 ? H[j + 1][j] / den : 0.0;
+            H[j][j] = den; H[j + 1][j] = 0.0;
+            g[j + 1] = -sn[j] * g[j]; g[j] = cs[j] * g[j];
+            if (std::abs(g[j + 1]) <= rtol * bnorm) { ++j; ++total; break; }
+        }
+        for (int i = j - 1; i >= 0; --i)
+        {
+            double s = g[i];
+            for (int k = i + 1; k < j; ++k) s -= H[i][k] * yv[k];
+            yv[i] = s / H[i][i];
+        }
+        hip_check(hipMemcpyAsync(ydev, yv.data(), sizeof(double) * std::max(j, 1), hipMemcpyHostToDevice, stream), "H2D");
+        hipLaunchKernelGGL(k_gmres_update, dim3(nblk(ne)), dim3(256), 0, stream, static_cast<const double *const *>(Vptrs), ydev, j,
+                           invdiag, dir, ne, delta);
+        hipLaunchKernelGGL(k_add, dim3(nblk(ne)), dim3(256), 0, stream, x0, delta, ne, tmp);
+        apply_op(nullptr, tmp, nullptr, dir, dt, 1, 0, bvec, rvec);
+        beta = dot(rvec, rvec, 1, 1);
+    }
+    hipLaunchKernelGGL(k_solve_final, dim3(nblk(ne)), dim3(256), 0, stream, x0, delta, Eex, dir, ne, Eout, corr);
+    check_launch("solve final");
+    have_corr = true;
+    if (rel_res) *rel_res = beta / bnorm;
+    return total;
+}
+
+// mercury::Solver::step_first_order (MercuryUns/driver/Solver.cpp:228-277) on the resident state
+double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
+{
+    const int nb = nblk(nc);
+    hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(256), 0, stream, d, P, U[0], U[1], Phi, Badd, W[0], W[1], Bind, Btot, uplus, alpha,
+                       dt_part);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, stream, dt_part, nb, dt_max, dt_dev);
+    double *dth = static_cast<double *>(host_pin);
+    hip_check(hipMemcpyAsync(dth, dt_dev, 8, hipMemcpyDeviceToHost, stream), "D2H");
+    hipEvent_t ev;
+    hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+    hip_check(hipEventRecord(ev, stream), "event");
+    SpC Ws{}, none{}, gradsC{};
+    SpW grads{}, Us{}, noneW{};
+    Ws.p[0] = W[0]; Ws.p[1] = W[1];
+    grads.p[0] = grad[0]; grads.p[1] = grad[1];
+    gradsC.p[0] = grad[0]; gradsC.p[1] = grad[1];
+    Us.p[0] = U[0]; Us.p[1] = U[1];
+    if (P.second_order)
+        hipLaunchKernelGGL(k_grad_current, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, Ws, none, grads, 1, Bind, J);
+    else
+    {
+        hip_check(hipMemsetAsync(grad[0], 0, 15 * size_t(nc) * 8, stream), "memset");
+        hip_check(hipMemsetAsync(grad[1], 0, 15 * size_t(nc) * 8, stream), "memset");
+        hipLaunchKernelGGL(k_grad_current, dim3(nb), dim3(256), 0, stream, d, P, 0, 0, Ws, none, grads, 1, Bind, J);
+    }
+    hipLaunchKernelGGL(k_face, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 2, 0, Ws, gradsC, none, FA, uplus, Btot, Bind, Phi, PhiAdd, Ef);
+    check_launch("explicit stage");
+    hip_check(hipEventSynchronize(ev), "event sync");
+    (void)hipEventDestroy(ev);
+    const double gdt = dth[0];
+    if (!std::isfinite(gdt) || gdt <= 0) throw std::runtime_error("invalid time step");
+    hipLaunchKernelGGL(k_cell_update, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, static_cast<const double *>(FA), none, W[0], W[1], Btot,
+                       J, dt_dev, 0.0, 1, 1, Us, noneW);
+    launch_edge_emf(Ef, uplus, Phi, P.cuct, nullptr, E_expl, 1);
+    *iters = 0;
+    *res = 0.0;
+    const double *Euse = E_expl;
+    if (P.hall || P.resistive)
+    {
+        hip_check(hipMemcpyAsync(B0, Btot, 3 * size_t(nc) * 8, hipMemcpyDeviceToDevice, stream), "copy");
+        if (precision == 0) *iters = solve64(Phi, E_expl, dir_mercury, gdt, E_last, res);
+        else *iters = solve_mixed(Phi, E_expl, dir_mercury, gdt, E_last, res);
+        Euse = E_last;
+    }
+    else
+        hip_check(hipMemcpyAsync(E_last, E_expl, size_t(ne) * 8, hipMemcpyDeviceToDevice, stream), "copy");
+    hipLaunchKernelGGL(k_faraday, dim3(nblk(nf)), dim3(256), 0, stream, d, Euse, nullptr, dt_dev, 0.0, Phi);
+    check_launch("faraday");
+    return gdt;
+}
+
+// ------------------------------------------------------------------------------------ C-ABI
+#define MHDG_TRY(ctx, ...)                                                                                                   \
+    try                                                                                                                      \
+    {                                                                                                                        \
+        __VA_ARGS__;                                                                                                         \
+        return 0;                                                                                                            \
+    }                                                                                                                        \
+    catch (const std::exception &e)                                                                                          \
+    {                                                                                                                        \
+        (ctx)->err = e.what();                                                                                               \
+        return 1;                                                                                                            \
+    }
+
+extern "C" {
+void mhdg_config_defaults(mhdg_config *c)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->gamma = 5.0 / 3.0; c->inver_MA2 = 1.0; c->M_ref = 1.0; c->M_H = 1.0; c->M_Na = 1.0; c->ne_floor = 0.01;
+    c->hall_coef = 0.0; c->inver_Rem = 0.0;
+    c->cfl = 1.0; c->recon_rel_limit = 0.0; c->rho_floor = 1e-10; c->p_floor = 1e-12; c->rho_floor_dt = 1e-2;
+    c->second_order = 1; c->lorentz_force = 1; c->hall = 0; c->resistive = 0;
+    c->eta_r_inner = 0.84; c->eta_r_outer = 1.04; c->eta_width = 0.02; c->hall_r_min = 1.0; c->hall_r_max = 1.012;
+    c->ind_rtol = 1e-4; c->ind_max_it = 100; c->ind_restart = 20; c->cuct = 0.5;
+    const double fs[5] = {1, -1, 0, 0, 1};
+    for (int m = 0; m < 5; ++m) { c->fs[0][m] = fs[m]; c->fs[1][m] = fs[m]; }
+    c->fs[1][0] = 1e-6; c->fs[1][4] = 1e-6;
+    c->u_inf[0] = -1; c->u_inf[1] = 0; c->u_inf[2] = 0;
+}
+
+int mhdg_create(const mhdg_mesh_desc *mesh, const mhdg_config *cfg, mhdg_ctx **out)
+{
+    auto *c = new mhdg_ctx();
+    try
+    {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw std::runtime_error("no HIP device available");
+        c->build(*mesh, *cfg);
+        *out = c;
+        return 0;
+    }
+    catch (const std::exception &e)
+    {
+        g_create_err = e.what();
+        delete c;
+        *out = nullptr;
+        return 1;
+    }
+}
+void mhdg_destroy(mhdg_ctx *c) { delete c; }
+const char *mhdg_last_error(mhdg_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+int mhdg_device_info(char *buf, int len)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return 1;
+    std::snprintf(buf, len, "%s %s CUs=%d L2=%d MB", p.name, p.gcnArchName, p.multiProcessorCount, p.l2CacheSize >> 20);
+    return 0;
+}
+int mhdg_sync(mhdg_ctx *c) { MHDG_TRY(c, hip_check(hipStreamSynchronize(c->stream), "sync")) }
+
+int mhdg_cell_vector(mhdg_ctx *c, const double *flux, double *b)
+{
+    MHDG_TRY(c, {
+        const double *f = c->stage_in(0, flux, c->nf);
+        double *o = c->stage(1, 3 * size_t(c->nc));
+        hipLaunchKernelGGL(k_cell_vector, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, f, o);
+        c->check_launch("cell_vector");
+        c->d2h(b, o, 3 * size_t(c->nc) * 8);
+    })
+}
+int mhdg_cell_current(mhdg_ctx *c, const double *B, double *J)
+{
+    MHDG_TRY(c, {
+        const double *b = c->stage_in(0, B, 3 * size_t(c->nc));
+        double *o = c->stage(1, 3 * size_t(c->nc));
+        SpC none{};
+        SpW nw{};
+        hipLaunchKernelGGL(k_grad_current, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, c->P, 0, 0, none, none, nw, 0, b, o);
+        c->check_launch("cell_current");
+        c->d2h(J, o, 3 * size_t(c->nc) * 8);
+    })
+}
+int mhdg_limited_gradients(mhdg_ctx *c, const double *W, int species, const double fs[5], int limit, const double *ghost, double *grad)
+{
+    MHDG_TRY(c, {
+        if (species < 0 || species >= kMaxSpecies) throw std::runtime_error("species out of range");
+        Params P = c->P;
+        for (int m = 0; m < 5; ++m) P.fs[species][m] = fs[m];
+        SpC Ws{}, gh{};
+        SpW gr{};
+        Ws.p[0] = c->stage_in(0, W, 5 * size_t(c->nc));
+        if (ghost) gh.p[0] = c->stage_in(2, ghost, 5 * size_t(c->nf));
+        gr.p[0] = c->stage(1, 15 * size_t(c->nc));
+        hipLaunchKernelGGL(k_grad_current, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, P, 1, species, Ws, gh, gr, limit,
+                           static_cast<const double *>(nullptr), static_cast<double *>(nullptr));
+        c->check_launch("limited_gradients");
+        c->d2h(grad, gr.p[0], 15 * size_t(c->nc) * 8);
+    })
+}
+int mhdg_flux_rhs(mhdg_ctx *c, int species, const double fs[5], const double *W, const double *grad, const double *ghost, double *rhs)
+{
+    MHDG_TRY(c, {
+        if (species < 0 || species >= kMaxSpecies) throw std::runtime_error("species out of range");
+        Params P = c->P;
+        for (int m = 0; m < 5; ++m) P.fs[species][m] = fs[m];
+        SpC Ws{}, Gs{}, gh{}, none{};
+        SpW ro{}, nw{};
+        Ws.p[0] = c->stage_in(0, W, 5 * size_t(c->nc));
+        Gs.p[0] = c->stage_in(1, grad, 15 * size_t(c->nc));
+        if (ghost) gh.p[0] = c->stage_in(2, ghost, 5 * size_t(c->nf));
+        double *fa = c->stage(3, 5 * size_t(c->nf));
+        ro.p[0] = c->stage(4, 5 * size_t(c->nc));
+        hipLaunchKernelGGL(k_face, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, P, 1, species, Ws, Gs, gh, fa,
+                           static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
+                           static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), static_cast<double *>(nullptr));
+        hipLaunchKernelGGL(k_cell_update, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, P, 1, species, static_cast<const double *>(fa),
+                           none, static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
+                           static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
+                           0.0, 0, 0, nw, ro);
+        c->check_launch("flux_rhs");
+        c->d2h(rhs, ro.p[0], 5 * size_t(c->nc) * 8);
+    })
+}
+int mhdg_ideal_emf(mhdg_ctx *c, const double *u, const double *Bt, const double *Bi, const double *pt, const double *pi, double cuct, double *E)
+{
+    MHDG_TRY(c, {
+        const size_t C3 = 3 * size_t(c->nc);
+        const double *du = c->stage_in(0, u, C3), *dbt = c->stage_in(1, Bt, C3), *dbi = c->stage_in(2, Bi, C3);
+        const double *dpt = c->stage_in(3, pt, c->nf), *dpi = c->stage_in(4, pi, c->nf);
+        double *dE = c->stage_in(5, E, c->ne);
+        double *ef = c->stage(6, 3 * size_t(c->nf));
+        SpC none{};
+        hipLaunchKernelGGL(k_face, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, c->P, 0, 0, none, none, none,
+                           static_cast<double *>(nullptr), du, dbt, dbi, dpt, static_cast<const double *>(nullptr), ef);
+        c->launch_edge_emf(ef, du, dpi, cuct, dE, dE, 0);
+        c->d2h(E, dE, size_t(c->ne) * 8);
+    })
+}
+int mhdg_faraday(mhdg_ctx *c, const double *E, double *Phi, double dt)
+{
+    MHDG_TRY(c, {
+        const double *dE = c->stage_in(0, E, c->ne);
+        double *dP = c->stage_in(1, Phi, c->nf);
+        hipLaunchKernelGGL(k_faraday, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, dE, static_cast<const double *>(nullptr),
+                           static_cast<const double *>(nullptr), dt, dP);
+        c->check_launch("faraday");
+        c->d2h(Phi, dP, size_t(c->nf) * 8);
+    })
+}
+int mhdg_faraday_edge_dt(mhdg_ctx *c, const double *E, const double *dte, double *Phi)
+{
+    MHDG_TRY(c, {
+        const double *dE = c->stage_in(0, E, c->ne), *dT = c->stage_in(2, dte, c->ne);
+        double *dP = c->stage_in(1, Phi, c->nf);
+        hipLaunchKernelGGL(k_faraday, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, dE, dT, static_cast<const double *>(nullptr), 0.0, dP);
+        c->check_launch("faraday_edge_dt");
+        c->d2h(Phi, dP, size_t(c->nf) * 8);
+    })
+}
+int mhdg_div_b(mhdg_ctx *c, const double *B, const double *extra, double *max_abs, double *max_rel, int64_t *worst)
+{
+    MHDG_TRY(c, {
+        const int nb = nblk(c->nc);
+        const double *dB = c->stage_in(0, B, c->nf);
+        const double *dX = extra ? c->stage_in(1, extra, c->nf) : nullptr;
+        double *pa = c->stage(2, nb), *pr = c->stage(3, nb);
+        int32_t *pc = reinterpret_cast<int32_t *>(c->stage(4, nb));
+        hipLaunchKernelGGL(k_div_b, dim3(nb), dim3(256), 0, c->stream, c->d, dB, dX, pa, pc, pr);
+        c->check_launch("div_b");
+        std::vector<double> ha(nb), hr(nb);
+        std::vector<int32_t> hc(nb);
+        c->d2h(ha.data(), pa, nb * 8); c->d2h(hr.data(), pr, nb * 8); c->d2h(hc.data(), pc, nb * 4);
+        double ma = 0, mr = 0;
+        int32_t wc = INT32_MAX;
+        for (int i = 0; i < nb; ++i)
+        {
+            if (ha[i] > ma || (ha[i] == ma && ha[i] > 0 && hc[i] < wc)) { ma = ha[i]; wc = hc[i]; }
+            mr = std::max(mr, hr[i]);
+        }
+        *max_abs = ma; *max_rel = mr; *worst = wc == INT32_MAX ? -1 : wc;
+    })
+}
+int mhdg_local_dt(mhdg_ctx *c, const double *WH, const double *WNa, const double *Bt, double *dt)
+{
+    MHDG_TRY(c, {
+        const int nb = nblk(c->nc);
+        const double *a = c->stage_in(0, WH, 5 * size_t(c->nc)), *b = c->stage_in(1, WNa, 5 * size_t(c->nc));
+        const double *bt = c->stage_in(2, Bt, 3 * size_t(c->nc));
+        double *part = c->stage(3, nb), *o = c->stage(4, 1);
+        hipLaunchKernelGGL(k_local_dt, dim3(nb), dim3(256), 0, c->stream, c->d, c->P, a, b, bt, part);
+        hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, static_cast<const double *>(part), nb, __builtin_inf(), o);
+        c->check_launch("local_dt");
+        c->d2h(dt, o, 8);
+    })
+}
+int mhdg_add_sources(mhdg_ctx *c, int species, const double *WH, const double *WNa, const double *Bt, const double *Jc, double dt, double *rhs)
+{
+    MHDG_TRY(c, {
+        if (species < 0 || species > 1) throw std::runtime_error("sources: species 0 or 1");
+        const size_t C = c->nc;
+        const double *a = c->stage_in(0, WH, 5 * C), *b = c->stage_in(1, WNa, 5 * C), *bt = c->stage_in(2, Bt, 3 * C),
+                     *j = c->stage_in(3, Jc, 3 * C);
+        double *r = c->stage_in(4, rhs, 5 * C);
+        SpC ri{};
+        SpW ro{}, nw{};
+        ri.p[0] = r;
+        ro.p[0] = r;
+        hipLaunchKernelGGL(k_cell_update, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, c->P, 1, species,
+                           static_cast<const double *>(nullptr), ri, a, b, bt, j, static_cast<const double *>(nullptr), dt, 1, 0, nw, ro);
+        c->check_launch("add_sources");
+        c->d2h(rhs, r, 5 * C * 8);
+    })
+}
+int mhdg_ind_set_frozen(mhdg_ctx *c, const double *B0c, const double *ne_eff)
+{
+    MHDG_TRY(c, {
+        const size_t C = c->nc;
+        hip_check(hipMemcpyAsync(c->B0, B0c, 3 * C * 8, hipMemcpyHostToDevice, c->stream), "H2D");
+        std::vector<double> al(C);
+        for (size_t i = 0; i < C; ++i) al[i] = (c->P.hall && ne_eff[i] > 0) ? c->taper_h[i] * c->cfg.hall_coef / ne_eff[i] : 0.0;
+        hip_check(hipMemcpyAsync(c->alpha, al.data(), C * 8, hipMemcpyHostToDevice, c->stream), "H2D");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    })
+}
+int mhdg_ind_apply_F(mhdg_ctx *c, const double *phi, double *out)
+{
+    MHDG_TRY(c, {
+        const double *p = c->stage_in(0, phi, c->nf);
+        c->apply_op(p, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, c->tmp);
+        c->d2h(out, c->tmp, size_t(c->ne) * 8);
+    })
+}
+int mhdg_ind_solve(mhdg_ctx *c, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, int precision, double *E,
+                   int32_t *iterations, double *rel_res)
+{
+    MHDG_TRY(c, {
+        const double *p = c->stage_in(0, Phi_n, c->nf), *ee = c->stage_in(1, E_expl, c->ne);
+        int8_t *dd = reinterpret_cast<int8_t *>(c->stage(2, (size_t(c->ne) + 7) / 8));
+        hip_check(hipMemcpyAsync(dd, dir, size_t(c->ne), hipMemcpyHostToDevice, c->stream), "H2D");
+        double *eo = c->stage(3, c->ne);
+        double rr = 0;
+        const int it = precision == 0 ? c->solve64(p, ee, dd, dt, eo, &rr) : c->solve_mixed(p, ee, dd, dt, eo, &rr);
+        c->d2h(E, eo, size_t(c->ne) * 8);
+        *iterations = it;
+        if (rel_res) *rel_res = rr;
+    })
+}
+int mhdg_set_state(mhdg_ctx *c, const double *UH, const double *UNa, const double *Ph)
+{
+    MHDG_TRY(c, {
+        hip_check(hipMemcpy(c->U[0], UH, 5 * size_t(c->nc) * 8, hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemcpy(c->U[1], UNa, 5 * size_t(c->nc) * 8, hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemcpy(c->Phi, Ph, size_t(c->nf) * 8, hipMemcpyHostToDevice), "H2D");
+        c->have_corr = false;
+    })
+}
+int mhdg_get_state(mhdg_ctx *c, double *UH, double *UNa, double *Ph)
+{
+    MHDG_TRY(c, {
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        hip_check(hipMemcpy(UH, c->U[0], 5 * size_t(c->nc) * 8, hipMemcpyDeviceToHost), "D2H");
+        hip_check(hipMemcpy(UNa, c->U[1], 5 * size_t(c->nc) * 8, hipMemcpyDeviceToHost), "D2H");
+        hip_check(hipMemcpy(Ph, c->Phi, size_t(c->nf) * 8, hipMemcpyDeviceToHost), "D2H");
+    })
+}
+int mhdg_set_background(mhdg_ctx *c, const double *PA, const double *BA)
+{
+    MHDG_TRY(c, {
+        hip_check(hipMemcpy(c->PhiAdd, PA, size_t(c->nf) * 8, hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemcpy(c->Badd, BA, 3 * size_t(c->nc) * 8, hipMemcpyHostToDevice), "H2D");
+    })
+}
+int mhdg_step(mhdg_ctx *c, double dt_max, int precision, double *dt_used, int32_t *iterations, double *rel_res)
+{
+    MHDG_TRY(c, {
+        int it = 0;
+        double rr = 0;
+        const double dt = c->step(dt_max, precision, &it, &rr);
+        if (dt_used) *dt_used = dt;
+        if (iterations) *iterations = it;
+        if (rel_res) *rel_res = rr;
+    })
+}
+int mhdg_get_derived(mhdg_ctx *c, int which, double *out)
+{
+    MHDG_TRY(c, {
+        const size_t C = c->nc;
+        const double *src = nullptr;
+        size_t n = 0;
+        switch (which)
+        {
+        case 0: src = c->W[0]; n = 5 * C; break;
+        case 1: src = c->W[1]; n = 5 * C; break;
+        case 2: src = c->Bind; n = 3 * C; break;
+        case 3: src = c->Btot; n = 3 * C; break;
+        case 4: src = c->J; n = 3 * C; break;
+        case 5: src = c->uplus; n = 3 * C; break;
+        case 6: src = c->E_last; n = c->ne; break;
+        default: throw std::runtime_error("unknown derived field");
+        }
+        c->d2h(out, src, n * 8);
+    })
+}
+}

@@ -1,0 +1,107 @@
+#pragma once
+// Device-side data of one context and the arithmetic helpers shared by the kernels.
+//
+// Incidence on the device is int32 CSR with the sign folded into the entry: v >= 0 is +1 on entity v,
+// v < 0 is -1 on entity ~v (4 bytes per entry instead of 4 + 1).  Per-entity metric arrays are AoS
+// (3 doubles per vector) as in the reference, which keeps a gathered vector in one cache line.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../host/vec3.h"
+
+namespace mhdg
+{
+__device__ __forceinline__ int32_t pid(int32_t v) { return v >= 0 ? v : ~v; }
+__device__ __forceinline__ double psg(int32_t v) { return v >= 0 ? 1.0 : -1.0; }
+// std::min / std::max semantics (second operand wins only on strict comparison)
+__device__ __forceinline__ double mn(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double mx(double a, double b) { return (a < b) ? b : a; }
+
+// Face kinds and edge modes (mhd::FaceKind / BoundaryInfo::edge_mode)
+enum : int8_t { kInterior = 0, kWall = 1, kInflow = 2, kOutflow = 3, kSkip = 4 };
+constexpr int32_t kNoStencil = INT32_MIN;
+
+struct Prim { double rho, u, v, w, p; };
+
+struct DevMesh
+{
+    int32_t nc, nf, ne;
+    const int32_t *c2f_ptr, *c2f;   // cell -> faces, stored order
+    const int32_t *cfs;             // cell -> faces sorted by face id (same ptr as c2f), signed
+    const int32_t *f2e_ptr, *f2e;   // face -> edges, loop order
+    const int32_t *e2f_ptr, *e2f;   // edge -> faces ascending (sign of the f2e entry)
+    const int32_t *fL, *fR;         // cell seeing the face outward (+1) / inward (-1); -1 if none
+    const int32_t *ec_ptr, *ec_cell;
+    const int8_t *mat, *kind, *emode;
+    const double *farea, *fcen, *famag, *ccen, *cvol, *evec, *emid, *elen, *ecw, *efw, *star2;
+    // Stencil (reference Reconstruction.h:34-41), aligned with c2f entries
+    const int32_t *snb;             // neighbour cell, -(f+1) virtual, kNoStencil for a Skip face
+    const double *sw, *sd, *sinv;   // weight, offset (3), symmetric inverse (6 per cell: 0 1 2 4 5 8)
+    const int8_t *svalid;
+    // ImplicitInduction constants (reference ImplicitInduction.h:73-91)
+    const double *recon;            // 3 per c2f entry
+    const double *m_e;              // eta_e / star1_e
+    const double *d2;               // sum of star2 over the faces of e
+    const double *taper;            // per cell
+};
+
+constexpr int kMaxSpecies = 10;
+
+struct Params
+{
+    double gamma, inver_MA2, M_ref, M_H, M_Na, ne_floor, hall_coef;
+    double cfl, rel_limit, rho_floor, p_floor, rho_floor_dt, cuct;
+    int32_t second_order, lorentz, hall, resistive;
+    double fs[kMaxSpecies][5];
+    double E_inf[3];
+};
+
+// Per-species array pointers passed by value (kernel arguments)
+struct SpC { const double *p[kMaxSpecies]; };
+struct SpW { double *p[kMaxSpecies]; };
+
+__device__ __forceinline__ V3 L3(const double *p, long i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+// Common/physics/Euler.h:13-49
+__device__ __forceinline__ Prim to_prim(const double *U, double gamma, double rho_floor, double p_floor)
+{
+    Prim W;
+    W.rho = mx(U[0], rho_floor);
+    W.u = U[1] / W.rho; W.v = U[2] / W.rho; W.w = U[3] / W.rho;
+    const double ke = 0.5 * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w);
+    W.p = mx((gamma - 1.0) * (U[4] - ke), p_floor);
+    return W;
+}
+__device__ __forceinline__ void to_cons(const Prim &W, double gamma, double *U)
+{
+    U[0] = W.rho; U[1] = W.rho * W.u; U[2] = W.rho * W.v; U[3] = W.rho * W.w;
+    U[4] = W.p / (gamma - 1.0) + 0.5 * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w);
+}
+
+// Common/physics/Scales.cpp:50-62
+struct NumInfo { double nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa; };
+__device__ __forceinline__ NumInfo numinfo(const Params &P, double rhoH, double rhoNa)
+{
+    NumInfo n;
+    n.nH = mx(rhoH, 0.0) / P.M_H * P.M_ref;
+    n.nNa = mx(rhoNa, 0.0) / P.M_Na * P.M_ref;
+    n.ne = n.nH + n.nNa;
+    n.ne_eff = sqrt(n.ne * n.ne + P.ne_floor * P.ne_floor);
+    if (n.ne > 1e-300) { n.chiH = n.nH / n.ne; n.chiNa = n.nNa / n.ne; }
+    else { n.chiH = 1.0; n.chiNa = 0.0; }
+    n.wH = n.nH / n.ne_eff;
+    n.wNa = n.nNa / n.ne_eff;
+    return n;
+}
+
+// Logical block index remapped so that the blocks one XCD receives (dispatch is round-robin over the
+// 8 XCDs) cover a contiguous index range: neighbours gathered by adjacent blocks share one L2.
+__device__ __forceinline__ int32_t xcd_block()
+{
+    const uint32_t b = blockIdx.x, n = gridDim.x;
+    const uint32_t per = n / 8u, x = b % 8u, q = b / 8u;
+    if (q >= per) return static_cast<int32_t>(b); // tail blocks (n not a multiple of 8) keep their index
+    return static_cast<int32_t>(x * per + q);
+}
+} // namespace mhdg

@@ -1,0 +1,702 @@
+// Explicit stage of the Hall-MHD step: cell preparation + CFL, limited gradients + cell current, face
+// fluxes + face EMF candidates, deterministic cell owner-gather + sources + forward Euler + floors, edge UCT
+// EMF, constrained transport and div B.  Arithmetic mirrors the reference line by line (cited per kernel)
+// so that results agree bit for bit with the CPU code built without FP contraction.
+#include "kernels.h"
+
+namespace mhdg
+{
+namespace
+{
+// src/uns/geometry/Reconstruct.cpp:11-45 for one cell
+__device__ V3 cell_vector(const DevMesh &d, int32_t c, const double *flux)
+{
+    double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, r[3] = {0, 0, 0};
+    for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+    {
+        const int32_t v = d.c2f[k], f = pid(v);
+        const double A = d.famag[f];
+        if (A <= 0) continue;
+        const V3 n = (psg(v) / A) * L3(d.farea, f);
+        const double nv[3] = {n.x, n.y, n.z};
+        const double phi = psg(v) * flux[f];
+        for (int i = 0; i < 3; ++i)
+        {
+            r[i] += nv[i] * phi;
+            for (int j = 0; j < 3; ++j) M[i][j] += A * nv[i] * nv[j];
+        }
+    }
+    const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                       M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    const double sc = mx(mx(mx(M[0][0], M[1][1]), M[2][2]), 1e-300);
+    if (fabs(det) < 1e-12 * sc * sc * sc) return V3();
+    double I[3][3];
+    I[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / det;
+    I[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) / det;
+    I[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) / det;
+    I[1][0] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) / det;
+    I[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) / det;
+    I[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) / det;
+    I[2][0] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) / det;
+    I[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) / det;
+    I[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) / det;
+    return {I[0][0] * r[0] + I[0][1] * r[1] + I[0][2] * r[2], I[1][0] * r[0] + I[1][1] * r[1] + I[1][2] * r[2],
+            I[2][0] * r[0] + I[2][1] * r[1] + I[2][2] * r[2]};
+}
+
+// MercuryUns/fluid/MercurySources.cpp:104-145 for one fluid cell (+inf if no face restricts it)
+__device__ double cell_dt(const DevMesh &d, const Params &P, int32_t c, const double *wH, const double *wNa, const V3 &b)
+{
+    double dt = __builtin_inf();
+    const double B2 = dot(b, b);
+    const double rho_tot = mx(wH[0] + wNa[0], P.rho_floor_dt);
+    for (int s = 0; s < 2; ++s)
+    {
+        const double *w = s == 0 ? wH : wNa;
+        const double cs2 = P.gamma * w[4] / mx(w[0], 1e-30);
+        double denom = 0;
+        for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+        {
+            const int32_t f = pid(d.c2f[k]);
+            const double A = d.famag[f];
+            const V3 n = (1.0 / A) * L3(d.farea, f);
+            const double vA2 = P.inver_MA2 * B2 / rho_tot;
+            const double vAn2 = P.inver_MA2 * dot(b, n) * dot(b, n) / rho_tot;
+            const double term = cs2 + vA2;
+            const double cf = sqrt(0.5 * (term + sqrt(mx(0.0, term * term - 4 * cs2 * vAn2))));
+            denom += (fabs(w[1] * n.x + w[2] * n.y + w[3] * n.z) + cf) * A;
+        }
+        if (denom > 0) dt = mn(dt, P.cfl * d.cvol[c] / denom);
+    }
+    return dt;
+}
+
+__device__ __forceinline__ double block_min(double v, double *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1)
+    {
+        if (t < s) sh[t] = mn(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    return sh[0];
+}
+} // namespace
+
+// Solver::prepare (MercuryUns/driver/Solver.cpp:94-125) + cell_time_steps + n_e,eff / alpha of the
+// frozen Hall state (Solver.cpp:257-260, ImplicitInduction.cpp:105-111).  One thread per cell.
+__global__ void __launch_bounds__(256) k_prepare(DevMesh d, Params P, const double *U0, const double *U1, const double *Phi,
+                                                 const double *Badd, double *W0, double *W1, double *Bind, double *Btot,
+                                                 double *uplus, double *alpha, double *dt_part)
+{
+    __shared__ double sh[256];
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    double dtc = __builtin_inf();
+    if (c < d.nc)
+    {
+        const Prim a = to_prim(U0 + 5 * size_t(c), P.gamma, P.rho_floor, P.p_floor);
+        const Prim b = to_prim(U1 + 5 * size_t(c), P.gamma, P.rho_floor, P.p_floor);
+        const double wa[5] = {a.rho, a.u, a.v, a.w, a.p}, wb[5] = {b.rho, b.u, b.v, b.w, b.p};
+        for (int m = 0; m < 5; ++m) { W0[5 * size_t(c) + m] = wa[m]; W1[5 * size_t(c) + m] = wb[m]; }
+        const V3 bi = cell_vector(d, c, Phi);
+        const V3 ba = L3(Badd, c);
+        const V3 bt(bi.x + ba.x, bi.y + ba.y, bi.z + ba.z);
+        st3(Bind, c, bi);
+        st3(Btot, c, bt);
+        V3 up;
+        double al = 0.0;
+        if (d.mat[c] == 0)
+        {
+            const NumInfo ni = numinfo(P, wa[0], wb[0]);
+            up = V3(ni.chiH * wa[1] + ni.chiNa * wb[1], ni.chiH * wa[2] + ni.chiNa * wb[2], ni.chiH * wa[3] + ni.chiNa * wb[3]);
+            al = (P.hall && ni.ne_eff > 0) ? d.taper[c] * P.hall_coef / ni.ne_eff : 0.0;
+            dtc = cell_dt(d, P, c, wa, wb, bt);
+        }
+        st3(uplus, c, up);
+        alpha[c] = al;
+    }
+    const double m = block_min(dtc, sh);
+    if (threadIdx.x == 0) dt_part[blockIdx.x] = m;
+}
+
+// Final min of the block minima; out[0] = min(dt, dt_max).
+__global__ void __launch_bounds__(256) k_dt_final(const double *part, int32_t n, double dt_max, double *out)
+{
+    __shared__ double sh[256];
+    double v = __builtin_inf();
+    for (int32_t i = threadIdx.x; i < n; i += blockDim.x) v = mn(v, part[i]);
+    v = block_min(v, sh);
+    if (threadIdx.x == 0) out[0] = mn(v, dt_max);
+}
+
+// Stencil::limited_gradients (Common/fluid/Reconstruction.cpp:80-142) for `ns` species and cell_current
+// (Common/induction/Current.cpp:9-31).  ghost: optional per-face table of 5 primitives per species.
+__global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpC Ws, SpC ghosts, SpW grads,
+                                                      int32_t limit, const double *Bind, double *J)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.nc) return;
+    const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1];
+    if (J)
+    {
+        V3 sum;
+        const V3 bc = L3(Bind, c);
+        for (int32_t k = k0; k < k1; ++k)
+        {
+            const int32_t v = d.c2f[k], f = pid(v);
+            V3 bf = bc;
+            const int32_t l = d.fL[f], r = d.fR[f];
+            if (l >= 0 && r >= 0) bf = 0.5 * (bc + L3(Bind, l == c ? r : l));
+            sum += cross(psg(v) * L3(d.farea, f), bf);
+        }
+        const double invV = 1.0 / d.cvol[c];
+        st3(J, c, V3(sum.x * invV, sum.y * invV, sum.z * invV));
+    }
+    const V3 xc = L3(d.ccen, c);
+    for (int32_t s = 0; s < ns; ++s)
+    {
+        const int32_t sp = s0 + s;
+        double *out = grads.p[s] + 15 * size_t(c);
+        if (!d.svalid[c])
+        {
+            for (int q = 0; q < 15; ++q) out[q] = 0.0;
+            continue;
+        }
+        const double *W = Ws.p[s];
+        const double *gh = ghosts.p[s];
+        const double *fs = P.fs[sp];
+        double wc[5], r[5][3], lo[5], hi[5];
+        for (int m = 0; m < 5; ++m) { wc[m] = W[5 * size_t(c) + m]; lo[m] = hi[m] = wc[m]; r[m][0] = r[m][1] = r[m][2] = 0.0; }
+        for (int32_t k = k0; k < k1; ++k)
+        {
+            const int32_t nb = d.snb[k];
+            if (nb == kNoStencil) continue;
+            double wn[5];
+            if (nb >= 0)
+                for (int m = 0; m < 5; ++m) wn[m] = W[5 * size_t(nb) + m];
+            else
+            {
+                const int32_t f = -(nb + 1);
+                if (gh)
+                    for (int m = 0; m < 5; ++m) wn[m] = gh[5 * size_t(f) + m];
+                else
+                {
+                    // ghost_state (Common/fluid/Reconstruction.cpp:9-24) with the cell value inside
+                    const int8_t kind = d.kind[f];
+                    for (int m = 0; m < 5; ++m) wn[m] = wc[m];
+                    if (kind == kWall)
+                    {
+                        if (sp == 1) { wn[0] = 1e-12; wn[1] = 0; wn[2] = 0; wn[3] = 0; wn[4] = 1e-12; }
+                        else
+                        {
+                            const V3 n = (psg(d.c2f[k]) / d.famag[f]) * L3(d.farea, f);
+                            const double un = wc[1] * n.x + wc[2] * n.y + wc[3] * n.z;
+                            wn[1] -= 2 * un * n.x; wn[2] -= 2 * un * n.y; wn[3] -= 2 * un * n.z;
+                        }
+                    }
+                    else if (kind == kInflow)
+                        for (int m = 0; m < 5; ++m) wn[m] = fs[m];
+                }
+            }
+            const double w = d.sw[k];
+            const V3 dd = L3(d.sd, k);
+            for (int m = 0; m < 5; ++m)
+            {
+                const double dq = wn[m] - wc[m];
+                r[m][0] += w * dd.x * dq; r[m][1] += w * dd.y * dq; r[m][2] += w * dd.z * dq;
+                lo[m] = mn(lo[m], wn[m]);
+                hi[m] = mx(hi[m], wn[m]);
+            }
+        }
+        const double *I6 = d.sinv + 6 * size_t(c);
+        const double I[9] = {I6[0], I6[1], I6[2], I6[1], I6[3], I6[4], I6[2], I6[4], I6[5]};
+        for (int m = 0; m < 5; ++m)
+        {
+            const double g[3] = {I[0] * r[m][0] + I[1] * r[m][1] + I[2] * r[m][2], I[3] * r[m][0] + I[4] * r[m][1] + I[5] * r[m][2],
+                                 I[6] * r[m][0] + I[7] * r[m][1] + I[8] * r[m][2]};
+            double psi = 1.0;
+            if (limit)
+                for (int32_t k = k0; k < k1; ++k)
+                {
+                    const V3 dx = L3(d.fcen, pid(d.c2f[k])) - xc;
+                    const double dl = g[0] * dx.x + g[1] * dx.y + g[2] * dx.z;
+                    if (dl > 1e-300) psi = mn(psi, (hi[m] - wc[m]) / dl);
+                    else if (dl < -1e-300) psi = mn(psi, (lo[m] - wc[m]) / dl);
+                }
+            if (limit && P.rel_limit > 0 && (m == 0 || m == 4))
+                for (int32_t k = k0; k < k1; ++k)
+                {
+                    const V3 dx = L3(d.fcen, pid(d.c2f[k])) - xc;
+                    const double dl = fabs(g[0] * dx.x + g[1] * dx.y + g[2] * dx.z);
+                    if (dl > 1e-300) psi = mn(psi, P.rel_limit * wc[m] / dl);
+                }
+            psi = mx(0.0, mn(1.0, psi));
+            out[3 * m] = psi * g[0]; out[3 * m + 1] = psi * g[1]; out[3 * m + 2] = psi * g[2];
+        }
+    }
+}
+
+namespace
+{
+// Common/fluid/FluxRHS.cpp:16-27
+__device__ __forceinline__ Prim face_state(const double *W, const double *grad, int32_t c, const V3 &dx)
+{
+    const double *w = W + 5 * size_t(c);
+    const double *g = grad + 15 * size_t(c);
+    Prim r;
+    r.rho = w[0] + g[0] * dx.x + g[1] * dx.y + g[2] * dx.z;
+    r.u = w[1] + g[3] * dx.x + g[4] * dx.y + g[5] * dx.z;
+    r.v = w[2] + g[6] * dx.x + g[7] * dx.y + g[8] * dx.z;
+    r.w = w[3] + g[9] * dx.x + g[10] * dx.y + g[11] * dx.z;
+    r.p = w[4] + g[12] * dx.x + g[13] * dx.y + g[14] * dx.z;
+    if (r.rho > 0 && r.p > 0) return r;
+    return Prim{w[0], w[1], w[2], w[3], w[4]};
+}
+
+// Common/physics/Euler.h:29-49
+__device__ void rusanov(const Prim &L, const Prim &R, const double n[3], double gamma, double *F)
+{
+    double UL[5], UR[5], FL[5], FR[5];
+    to_cons(L, gamma, UL);
+    to_cons(R, gamma, UR);
+    const double unL = L.u * n[0] + L.v * n[1] + L.w * n[2];
+    FL[0] = L.rho * unL; FL[1] = L.rho * unL * L.u + L.p * n[0]; FL[2] = L.rho * unL * L.v + L.p * n[1];
+    FL[3] = L.rho * unL * L.w + L.p * n[2]; FL[4] = unL * (UL[4] + L.p);
+    const double unR = R.u * n[0] + R.v * n[1] + R.w * n[2];
+    FR[0] = R.rho * unR; FR[1] = R.rho * unR * R.u + R.p * n[0]; FR[2] = R.rho * unR * R.v + R.p * n[1];
+    FR[3] = R.rho * unR * R.w + R.p * n[2]; FR[4] = unR * (UR[4] + R.p);
+    const double sL = fabs(L.u * n[0] + L.v * n[1] + L.w * n[2]) + sqrt(gamma * L.p / L.rho);
+    const double sR = fabs(R.u * n[0] + R.v * n[1] + R.w * n[2]) + sqrt(gamma * R.p / R.rho);
+    const double s = mx(sL, sR);
+    for (int m = 0; m < 5; ++m) F[m] = 0.5 * (FL[m] + FR[m]) - 0.5 * s * (UR[m] - UL[m]);
+}
+
+// ghost_state (Common/fluid/Reconstruction.cpp:9-24)
+__device__ __forceinline__ Prim ghost_state(int8_t kind, int species, const Prim &in, const V3 &n, const double *fs)
+{
+    if (kind == kWall)
+    {
+        if (species == 1) return Prim{1e-12, 0, 0, 0, 1e-12};
+        const double un = in.u * n.x + in.v * n.y + in.w * n.z;
+        Prim g = in;
+        g.u -= 2 * un * n.x; g.v -= 2 * un * n.y; g.w -= 2 * un * n.z;
+        return g;
+    }
+    if (kind == kInflow) return Prim{fs[0], fs[1], fs[2], fs[3], fs[4]};
+    return in;
+}
+} // namespace
+
+// Face kernel: fluid_flux_rhs's face body (Common/fluid/FluxRHS.cpp:36-73) for `ns` species, storing
+// F*|A| per face and species (FA, 5*ns per face, zero for skipped faces), and optionally the ideal EMF
+// face candidate E_f (Common/induction/IdealEMF.cpp:32-55, 3 per face).
+__global__ void __launch_bounds__(256) k_face(DevMesh d, Params P, int32_t ns, int32_t s0, SpC Ws, SpC grads, SpC ghosts, double *FA,
+                                              const double *uplus, const double *Btot, const double *Bind, const double *Phi,
+                                              const double *PhiAdd, double *Ef)
+{
+    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= d.nf) return;
+    const int32_t l = d.fL[f], r = d.fR[f];
+    const double A = d.famag[f];
+    const V3 n = (1.0 / A) * L3(d.farea, f);
+    if (FA)
+    {
+        const int8_t kind = d.kind[f];
+        const bool any = (l >= 0 && d.mat[l] == 0) || (r >= 0 && d.mat[r] == 0);
+        double *out = FA + size_t(5 * ns) * f;
+        if (kind == kSkip || !any)
+            for (int q = 0; q < 5 * ns; ++q) out[q] = 0.0;
+        else
+        {
+            const double nv[3] = {n.x, n.y, n.z};
+            const V3 xf = L3(d.fcen, f);
+            for (int32_t s = 0; s < ns; ++s)
+            {
+                const int32_t sp = s0 + s;
+                const double *W = Ws.p[s], *G = grads.p[s];
+                Prim L, R;
+                if (l >= 0 && r >= 0 && kind == kInterior)
+                {
+                    L = face_state(W, G, l, xf - L3(d.ccen, l));
+                    R = face_state(W, G, r, xf - L3(d.ccen, r));
+                }
+                else
+                {
+                    // the single fluid cell; f2c order puts the lower cell id first (FluxRHS.cpp:57-62)
+                    int32_t c;
+                    double sg;
+                    if (l >= 0 && r >= 0)
+                    {
+                        const int32_t first = l < r ? l : r, second = l < r ? r : l;
+                        c = d.mat[first] == 1 ? second : first;
+                    }
+                    else c = l >= 0 ? l : r;
+                    sg = c == l ? 1.0 : -1.0;
+                    const Prim in = face_state(W, G, c, xf - L3(d.ccen, c));
+                    Prim gh;
+                    if (ghosts.p[s])
+                    {
+                        const double *q = ghosts.p[s] + 5 * size_t(f);
+                        gh = Prim{q[0], q[1], q[2], q[3], q[4]};
+                    }
+                    else gh = ghost_state(kind, sp, in, sg * n, P.fs[sp]);
+                    if (sg > 0) { L = in; R = gh; } else { R = in; L = gh; }
+                }
+                double F[5];
+                rusanov(L, R, nv, P.gamma, F);
+                for (int m = 0; m < 5; ++m) out[5 * s + m] = F[m] * A;
+            }
+        }
+    }
+    if (Ef)
+    {
+        const double Bn_tot = (PhiAdd ? Phi[f] + PhiAdd[f] : Phi[f]) / A;
+        V3 F;
+        if (l >= 0 && r >= 0)
+        {
+            const V3 uL = L3(uplus, l), uR = L3(uplus, r);
+            const V3 fLx = dot(uL, n) * L3(Btot, l) - Bn_tot * uL;
+            const V3 fRx = dot(uR, n) * L3(Btot, r) - Bn_tot * uR;
+            const double a = mx(fabs(dot(uL, n)), fabs(dot(uR, n)));
+            F = 0.5 * (fLx + fRx) - (0.5 * a) * (L3(Bind, r) - L3(Bind, l));
+        }
+        else
+        {
+            const int32_t c = l >= 0 ? l : r;
+            const V3 u = L3(uplus, c);
+            F = dot(u, n) * L3(Btot, c) - Bn_tot * u;
+        }
+        st3(Ef, f, -1.0 * cross(n, F));
+    }
+}
+
+// Cell owner-gather of the face fluxes in ascending face order (the reference's scatter order,
+// FluxRHS.cpp:71-72), Lorentz sources (MercurySources.cpp:20-63), and - when dt != 0 - the forward Euler
+// update with floors (Solver.cpp:246-252, 171-191).  rhs_out (nullable) receives the rhs.
+// FA nullable (no flux term), rhs_in nullable (start from zero); species of slot s is s0 + s.
+__global__ void __launch_bounds__(256) k_cell_update(DevMesh d, Params P, int32_t ns, int32_t s0, const double *FA, SpC rhs_in,
+                                                     const double *W0, const double *W1, const double *Btot, const double *J,
+                                                     const double *dt_dev, double dt_val, int32_t sources, int32_t update, SpW Us,
+                                                     SpW rhs_out)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.nc) return;
+    const bool fluid = d.mat[c] == 0;
+    const double dt = dt_dev ? dt_dev[0] : dt_val;
+    const double V = d.cvol[c];
+    for (int32_t s = 0; s < ns; ++s)
+    {
+        const int32_t sp = s0 + s;
+        double rhs[5] = {0, 0, 0, 0, 0};
+        if (rhs_in.p[s])
+            for (int m = 0; m < 5; ++m) rhs[m] = rhs_in.p[s][5 * size_t(c) + m];
+        if (fluid)
+        {
+            if (FA)
+            for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+            {
+                const int32_t v = d.cfs[k], f = pid(v);
+                const double *fa = FA + size_t(5 * ns) * f + 5 * s;
+                if (v >= 0) for (int m = 0; m < 5; ++m) rhs[m] -= fa[m] / V;
+                else for (int m = 0; m < 5; ++m) rhs[m] += fa[m] / V;
+            }
+            if (sources)
+            {
+                const double *W = sp == 0 ? W0 : W1;
+                const NumInfo ni = numinfo(P, W0[5 * size_t(c)], W1[5 * size_t(c)]);
+                const double ws = sp == 0 ? ni.wH : ni.wNa;
+                const double rho = mx(W[5 * size_t(c)], 0.0);
+                const V3 u(W[5 * size_t(c) + 1], W[5 * size_t(c) + 2], W[5 * size_t(c) + 3]);
+                V3 S;
+                if (P.lorentz) S += (P.inver_MA2 * ws) * cross(L3(J, c), L3(Btot, c));
+                rhs[1] += S.x; rhs[2] += S.y; rhs[3] += S.z;
+                rhs[4] += dot(u, S) + (rho > 1e-30 ? 0.5 * dt * dot(S, S) / rho : 0.0);
+            }
+        }
+        if (rhs_out.p[s])
+            for (int m = 0; m < 5; ++m) rhs_out.p[s][5 * size_t(c) + m] = rhs[m];
+        if (update && fluid)
+        {
+            double *u = Us.p[s] + 5 * size_t(c);
+            double q[5];
+            for (int m = 0; m < 5; ++m) q[m] = u[m] + dt * rhs[m];
+            const double ke = q[0] > 0 ? 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0] : 0.0;
+            if (!(q[0] >= P.rho_floor)) to_cons(Prim{P.rho_floor, 0, 0, 0, P.p_floor}, P.gamma, q);
+            else if ((P.gamma - 1.0) * (q[4] - ke) < P.p_floor)
+            {
+                const Prim w = to_prim(q, P.gamma, P.rho_floor, P.p_floor);
+                to_cons(w, P.gamma, q);
+            }
+            for (int m = 0; m < 5; ++m) u[m] = q[m];
+        }
+    }
+}
+
+namespace
+{
+// The per-face quantities of the UCT edge sum (IdealEMF.cpp:79-88)
+__device__ __forceinline__ void face_dirs(const DevMesh &d, int32_t f, int32_t e, const V3 &th, V3 &m, V3 &nrm)
+{
+    V3 v = L3(d.fcen, f) - L3(d.emid, e);
+    v -= dot(v, th) * th;
+    const double nv = norm(v);
+    m = nv > 0 ? (1.0 / nv) * v : V3();
+    nrm = (1.0 / d.famag[f]) * L3(d.farea, f);
+}
+__device__ __forceinline__ double edge_amax(const DevMesh &d, const double *u_cell, int32_t c0, int32_t c1, const V3 &m)
+{
+    double a = 0.0;
+    for (int32_t k = c0; k < c1; ++k) a = mx(a, fabs(dot(L3(u_cell, d.ec_cell[k]), m)));
+    return a;
+}
+__device__ __forceinline__ void plane_basis(const V3 &t, V3 &e1, V3 &e2)
+{
+    const V3 a = fabs(t.x) < 0.9 ? V3(1, 0, 0) : V3(0, 1, 0);
+    e1 = cross(t, a);
+    e1 *= 1.0 / norm(e1);
+    e2 = cross(t, e1);
+}
+// boundary EMF of Solver::rhs (MercuryUns/driver/Solver.cpp:162-167)
+__device__ __forceinline__ double edge_bc(const DevMesh &d, const Params &P, int32_t e, double v)
+{
+    const int8_t mode = d.emode[e];
+    if (mode == 1) return 0.0;
+    if (mode == 2) return dot(V3(P.E_inf[0], P.E_inf[1], P.E_inf[2]), L3(d.evec, e));
+    return v;
+}
+} // namespace
+
+// add_ideal_emf edge loop (Common/induction/IdealEMF.cpp:59-128), one thread per edge with valence at most
+// `vmax`; larger edges are left to k_edge_emf_heavy.  E_out[e] = E_in[e] + E_ideal (E_in nullable = 0);
+// apply_bc additionally applies the boundary EMF of Solver::rhs.
+__global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const double *Ef, const double *u_cell, const double *phi_ind,
+                                                  double cuct, int32_t vmax, const double *E_in, double *E_out, int32_t apply_bc)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.ne) return;
+    const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
+    const int32_t nf = f1 - f0;
+    if (nf > vmax) return;
+    double out = E_in ? E_in[e] : 0.0;
+    const int32_t c0 = d.ec_ptr[e], c1 = d.ec_ptr[e + 1];
+    bool any_fluid = false;
+    for (int32_t k = c0; k < c1 && !any_fluid; ++k) any_fluid = d.mat[d.ec_cell[k]] == 0;
+    if (nf > 0 && d.elen[e] > 0 && any_fluid)
+    {
+        const V3 t = L3(d.evec, e);
+        const double L = d.elen[e];
+        const V3 th = (1.0 / L) * t;
+        double Ec = 0;
+        for (int32_t i = f0; i < f1; ++i) Ec += d.efw[i] * dot(L3(Ef, pid(d.e2f[i])), t);
+        V3 e1, e2;
+        plane_basis(th, e1, e2);
+        double M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0;
+        for (int32_t i = f0; i < f1; ++i)
+        {
+            const int32_t f = pid(d.e2f[i]);
+            V3 m, nrm;
+            face_dirs(d, f, e, th, m, nrm);
+            const double aef = edge_amax(d, u_cell, c0, c1, m);
+            const V3 qv = (aef * dot(th, cross(m, nrm))) * nrm;
+            const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
+            M11 += w * qa * qa;
+            M12 += w * qa * qb;
+            M22 += w * qb * qb;
+            g1 += w * qa;
+            g2 += w * qb;
+        }
+        const double tr = M11 + M22;
+        const double eps = 1e-14 * tr;
+        const double det = (M11 + eps) * (M22 + eps) - M12 * M12;
+        double l1 = 0, l2 = 0;
+        if (tr > 0 && det > 0)
+        {
+            l1 = ((M22 + eps) * g1 - M12 * g2) / det;
+            l2 = ((M11 + eps) * g2 - M12 * g1) / det;
+        }
+        double Eu = 0;
+        for (int32_t i = f0; i < f1; ++i)
+        {
+            const int32_t f = pid(d.e2f[i]);
+            V3 m, nrm;
+            face_dirs(d, f, e, th, m, nrm);
+            const double aef = edge_amax(d, u_cell, c0, c1, m);
+            const V3 qv = (aef * dot(th, cross(m, nrm))) * nrm;
+            const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
+            const double omega = w * (1.0 - (qa * l1 + qb * l2));
+            const double sig = dot(th, cross(m, nrm));
+            Eu += omega * aef * sig * (phi_ind[f] / d.famag[f]);
+        }
+        out += Ec + 2.0 * cuct * L * Eu;
+    }
+    E_out[e] = apply_bc ? edge_bc(d, P, e, out) : out;
+}
+
+// The same for one high-valence edge per block (pole axes of a lat-lon net): per-face terms in parallel,
+// every sum sequential in the reference's order.
+__global__ void __launch_bounds__(256) k_edge_emf_heavy(DevMesh d, Params P, const double *Ef, const double *u_cell,
+                                                        const double *phi_ind, double cuct, const int32_t *heavy,
+                                                        const double *E_in, double *E_out, int32_t apply_bc)
+{
+    __shared__ double s_w[kHeavyMax], s_ec[kHeavyMax], s_qa[kHeavyMax], s_qb[kHeavyMax], s_sig[kHeavyMax], s_aef[kHeavyMax];
+    __shared__ double s_l[2];
+    const int32_t e = heavy[blockIdx.x];
+    const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1], nf = f1 - f0;
+    const int32_t c0 = d.ec_ptr[e], c1 = d.ec_ptr[e + 1];
+    bool any_fluid = false;
+    for (int32_t k = c0; k < c1 && !any_fluid; ++k) any_fluid = d.mat[d.ec_cell[k]] == 0;
+    const bool active = nf > 0 && d.elen[e] > 0 && any_fluid;
+    const V3 t = L3(d.evec, e);
+    const double L = d.elen[e];
+    const V3 th = (1.0 / L) * t;
+    V3 e1, e2;
+    plane_basis(th, e1, e2);
+    if (active)
+        for (int32_t i = threadIdx.x; i < nf; i += blockDim.x)
+        {
+            const int32_t f = pid(d.e2f[f0 + i]);
+            s_w[i] = d.efw[f0 + i];
+            s_ec[i] = dot(L3(Ef, f), t);
+            V3 m, nrm;
+            face_dirs(d, f, e, th, m, nrm);
+            const double aef = edge_amax(d, u_cell, c0, c1, m);
+            const double sig = dot(th, cross(m, nrm));
+            const V3 qv = (aef * sig) * nrm;
+            s_qa[i] = dot(qv, e1);
+            s_qb[i] = dot(qv, e2);
+            s_sig[i] = sig;
+            s_aef[i] = aef;
+        }
+    __syncthreads();
+    double Ec = 0;
+    if (threadIdx.x == 0 && active)
+    {
+        double M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0;
+        for (int32_t i = 0; i < nf; ++i)
+        {
+            Ec += s_w[i] * s_ec[i];
+            M11 += s_w[i] * s_qa[i] * s_qa[i];
+            M12 += s_w[i] * s_qa[i] * s_qb[i];
+            M22 += s_w[i] * s_qb[i] * s_qb[i];
+            g1 += s_w[i] * s_qa[i];
+            g2 += s_w[i] * s_qb[i];
+        }
+        const double tr = M11 + M22;
+        const double eps = 1e-14 * tr;
+        const double det = (M11 + eps) * (M22 + eps) - M12 * M12;
+        double l1 = 0, l2 = 0;
+        if (tr > 0 && det > 0)
+        {
+            l1 = ((M22 + eps) * g1 - M12 * g2) / det;
+            l2 = ((M11 + eps) * g2 - M12 * g1) / det;
+        }
+        s_l[0] = l1; s_l[1] = l2;
+    }
+    __syncthreads();
+    if (active)
+        for (int32_t i = threadIdx.x; i < nf; i += blockDim.x)
+        {
+            const int32_t f = pid(d.e2f[f0 + i]);
+            const double omega = s_w[i] * (1.0 - (s_qa[i] * s_l[0] + s_qb[i] * s_l[1]));
+            s_ec[i] = omega * s_aef[i] * s_sig[i] * (phi_ind[f] / d.famag[f]);
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double out = E_in ? E_in[e] : 0.0;
+        if (active)
+        {
+            double Eu = 0;
+            for (int32_t i = 0; i < nf; ++i) Eu += s_ec[i];
+            out += Ec + 2.0 * cuct * L * Eu;
+        }
+        E_out[e] = apply_bc ? edge_bc(d, P, e, out) : out;
+    }
+}
+
+// faraday_update / faraday_update_edge_dt (Common/induction/Faraday.cpp:15-36): B -= dt d1 E.
+__global__ void __launch_bounds__(256) k_faraday(DevMesh d, const double *E, const double *dte, const double *dt_dev, double dt_val,
+                                                 double *B)
+{
+    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= d.nf) return;
+    if (d.fL[f] < 0 && d.fR[f] < 0) return;
+    double circ = 0.0;
+    if (dte)
+    {
+        for (int32_t k = d.f2e_ptr[f]; k < d.f2e_ptr[f + 1]; ++k)
+        {
+            const int32_t v = d.f2e[k], e = pid(v);
+            circ += psg(v) * dte[e] * E[e];
+        }
+        B[f] -= circ;
+    }
+    else
+    {
+        for (int32_t k = d.f2e_ptr[f]; k < d.f2e_ptr[f + 1]; ++k)
+        {
+            const int32_t v = d.f2e[k];
+            circ += psg(v) * E[pid(v)];
+        }
+        const double dt = dt_dev ? dt_dev[0] : dt_val;
+        B[f] -= dt * circ;
+    }
+}
+
+// div_b (Common/diagnostics/DivB.cpp:8-24): per-block (max |s|, first cell attaining it, max rel).
+__global__ void __launch_bounds__(256) k_div_b(DevMesh d, const double *B, const double *extra, double *part_abs, int32_t *part_cell,
+                                               double *part_rel)
+{
+    __shared__ double sa[256], sr[256];
+    __shared__ int32_t sc[256];
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    double va = 0, vr = 0;
+    int32_t vc = INT32_MAX;
+    if (c < d.nc)
+    {
+        double s = 0, a = 0;
+        for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+        {
+            const int32_t v = d.c2f[k], f = pid(v);
+            s += psg(v) * B[f];
+            a += fabs(B[f]) + (extra ? fabs(extra[f]) : 0.0);
+        }
+        vr = a > 0 ? fabs(s) / a : 0.0;
+        va = fabs(s);
+        vc = va > 0 ? c : INT32_MAX;
+    }
+    const int t = threadIdx.x;
+    sa[t] = va; sr[t] = vr; sc[t] = vc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1)
+    {
+        if (t < s)
+        {
+            if (sa[t + s] > sa[t] || (sa[t + s] == sa[t] && sc[t + s] < sc[t])) { sa[t] = sa[t + s]; sc[t] = sc[t + s]; }
+            sr[t] = mx(sr[t], sr[t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) { part_abs[blockIdx.x] = sa[0]; part_cell[blockIdx.x] = sc[0]; part_rel[blockIdx.x] = sr[0]; }
+}
+// reconstruct_cell_vector for every cell
+__global__ void __launch_bounds__(256) k_cell_vector(DevMesh d, const double *flux, double *b)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < d.nc) st3(b, c, cell_vector(d, c, flux));
+}
+
+// cell_time_steps + block minima from given primitives (local_time_step, MercurySources.cpp:104-116)
+__global__ void __launch_bounds__(256) k_local_dt(DevMesh d, Params P, const double *W0, const double *W1, const double *Btot,
+                                                  double *dt_part)
+{
+    __shared__ double sh[256];
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    double dtc = __builtin_inf();
+    if (c < d.nc && d.mat[c] == 0) dtc = cell_dt(d, P, c, W0 + 5 * size_t(c), W1 + 5 * size_t(c), L3(Btot, c));
+    const double m = block_min(dtc, sh);
+    if (threadIdx.x == 0) dt_part[blockIdx.x] = m;
+}
+} // namespace mhdg

@@ -1,0 +1,240 @@
+// Implicit Hall/resistive induction operator (ImplicitInduction::apply_F / apply_A,
+// Common/induction/ImplicitInduction.cpp:127-198) as face, cell and edge gathers, plus the GMRES vector
+// kernels.  Reductions are deterministic: a fixed grid, a fixed per-block tree, and the last block to
+// finish (ticket counter) combines the block partials in a fixed tree, so reruns are bit-identical.
+#include "induction.h"
+
+namespace mhdg
+{
+// ------------------------------------------------------------------------------------ reductions
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1)
+    {
+        if (t < s) sh[t] += sh[t + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Called by every thread of every block with the block's partial value (valid in thread 0).  The last
+// block writes op(sum of partials) to *out.  op: 0 = sum, 1 = sqrt(sum).
+__device__ void grid_finish(double part, RedScratch rs, double *out, int op)
+{
+    __shared__ double sh[kRedThreads];
+    __shared__ int last;
+    if (threadIdx.x == 0)
+    {
+        rs.partial[blockIdx.x] = part;
+        __threadfence();
+        const unsigned prev = atomicAdd(rs.ticket, 1u);
+        last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double v = 0.0;
+    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kRedThreads)
+        v += __hip_atomic_load(&rs.partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0)
+    {
+        *out = op == 1 ? sqrt(v) : v;
+        *rs.ticket = 0u;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRedThreads) k_dot(const T *a, const T *b, int32_t n, RedScratch rs, double *out, int op)
+{
+    __shared__ double sh[kRedThreads];
+    double s = 0.0;
+    for (int32_t i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += gridDim.x * kRedThreads)
+        s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+    grid_finish(block_sum(s, sh), rs, out, op);
+}
+template __global__ void k_dot<double>(const double *, const double *, int32_t, RedScratch, double *, int);
+template __global__ void k_dot<float>(const float *, const float *, int32_t, RedScratch, double *, int);
+
+// Modified Gram-Schmidt step i (ImplicitInduction.cpp:341-347): w -= h[i-1] V_{i-1} (i > 0), then
+// h[i] = <w, V_i>; with `norm` set: w -= h[i-1] V_{i-1}, then h[i] = |w|.
+template <class T>
+__global__ void __launch_bounds__(kRedThreads) k_mgs(T *w, const T *Vprev, const T *Vi, int32_t n, double *h, int32_t i, int32_t norm,
+                                                    RedScratch rs)
+{
+    __shared__ double sh[kRedThreads];
+    const double hp = i > 0 ? h[i - 1] : 0.0;
+    double s = 0.0;
+    for (int32_t e = blockIdx.x * kRedThreads + threadIdx.x; e < n; e += gridDim.x * kRedThreads)
+    {
+        double we = static_cast<double>(w[e]);
+        if (i > 0)
+        {
+            we -= hp * static_cast<double>(Vprev[e]);
+            w[e] = static_cast<T>(we);
+            we = static_cast<double>(w[e]);
+        }
+        s += norm ? we * we : we * static_cast<double>(Vi[e]);
+    }
+    grid_finish(block_sum(s, sh), rs, h + i, norm ? 1 : 0);
+}
+template __global__ void k_mgs<double>(double *, const double *, const double *, int32_t, double *, int32_t, int32_t, RedScratch);
+template __global__ void k_mgs<float>(float *, const float *, const float *, int32_t, double *, int32_t, int32_t, RedScratch);
+
+// V_{j+1} = w / h (reference divides, ImplicitInduction.cpp:342)
+template <class T>
+__global__ void __launch_bounds__(256) k_scale_div(const T *w, const double *h, int32_t n, T *out)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const double hv = h[0];
+    if (e >= n || !(hv > 1e-300)) return;
+    out[e] = static_cast<T>(static_cast<double>(w[e]) / hv);
+}
+template __global__ void k_scale_div<double>(const double *, const double *, int32_t, double *);
+template __global__ void k_scale_div<float>(const float *, const double *, int32_t, float *);
+
+// r / beta into V_0 (ImplicitInduction.cpp:330)
+__global__ void __launch_bounds__(256) k_div_scalar(const double *r, const double *beta, int32_t n, double *out)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = r[e] / beta[0];
+}
+
+// ------------------------------------------------------------------------------------ operator
+// Phi'_f = sum_k f2e_k x_e, x = dir ? 0 : v * invdiag (right Jacobi preconditioner) or x = v.
+__global__ void __launch_bounds__(256) k_d1(DevMesh d, const double *v, const double *invdiag, const int8_t *dir, double *phi)
+{
+    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= d.nf) return;
+    double s = 0.0;
+    for (int32_t k = d.f2e_ptr[f]; k < d.f2e_ptr[f + 1]; ++k)
+    {
+        const int32_t q = d.f2e[k], e = pid(q);
+        const double x = invdiag ? (dir[e] ? 0.0 : v[e] * invdiag[e]) : v[e];
+        s += psg(q) * x;
+    }
+    phi[f] = s;
+}
+
+// b_c = sum_k recon_k Phi_{f_k} (ImplicitInduction.cpp:150-160)
+__global__ void __launch_bounds__(256) k_bcell(DevMesh d, const double *phi, double *b)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.nc) return;
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+    {
+        const double v = phi[pid(d.c2f[k])];
+        bx += d.recon[3 * size_t(k)] * v; by += d.recon[3 * size_t(k) + 1] * v; bz += d.recon[3 * size_t(k) + 2] * v;
+    }
+    b[3 * size_t(c)] = bx; b[3 * size_t(c) + 1] = by; b[3 * size_t(c) + 2] = bz;
+}
+
+// g_c = alpha_c (J_c[b] x B0_c), zero where alpha_c = 0 (ImplicitInduction.cpp:162-174)
+__global__ void __launch_bounds__(256) k_hall_g(DevMesh d, const double *b, const double *alpha, const double *B0, double *g)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.nc) return;
+    const double a = alpha[c];
+    V3 out;
+    if (a != 0.0)
+    {
+        V3 sum;
+        const V3 bc = L3(b, c);
+        for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+        {
+            const int32_t v = d.c2f[k], f = pid(v);
+            V3 bf = bc;
+            const int32_t l = d.fL[f], r = d.fR[f];
+            if (l >= 0 && r >= 0) bf = 0.5 * (bc + L3(b, l == c ? r : l));
+            sum += cross(psg(v) * L3(d.farea, f), bf);
+        }
+        const V3 J = (1.0 / d.cvol[c]) * sum;
+        out = a * cross(J, L3(B0, c));
+    }
+    st3(g, c, out);
+}
+
+// Edge assembly of apply_F (ImplicitInduction.cpp:139-182): part0 = sum_f f2e star2 Phi (faces ascending),
+// part1 = sum_c g_c . (w_ec t_e) (cells ascending), F = m part0 + part1 + 0.
+// mode 0: out = F.  mode 1 (apply_A): out = x + dt F with x = dir ? 0 : v*invdiag (invdiag non-null) or v;
+// with `mask` set, Dirichlet rows of the result are zeroed (Arnoldi vector, ImplicitInduction.cpp:340).
+__global__ void __launch_bounds__(256) k_edge_apply(DevMesh d, int32_t resistive, int32_t hall, const double *phi, const double *g,
+                                                    const double *v, const double *invdiag, const int8_t *dir, double dt, int32_t mode,
+                                                    int32_t mask, const double *bvec, double *out)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.ne) return;
+    double p0 = 0.0, p1 = 0.0;
+    if (resistive)
+        for (int32_t k = d.e2f_ptr[e]; k < d.e2f_ptr[e + 1]; ++k)
+        {
+            const int32_t q = d.e2f[k], f = pid(q);
+            const double val = d.star2[f] * phi[f];
+            p0 += psg(q) * val;
+        }
+    if (hall)
+    {
+        const V3 t = L3(d.evec, e);
+        for (int32_t k = d.ec_ptr[e]; k < d.ec_ptr[e + 1]; ++k)
+        {
+            const double w = d.ecw[k];
+            const V3 wt = w * t;
+            p1 += dot(L3(g, d.ec_cell[k]), wt);
+        }
+    }
+    const double F = d.m_e[e] * p0 + p1 + 0.0;
+    double r;
+    if (mode == 0) r = F;
+    else
+    {
+        const double x = invdiag ? (dir[e] ? 0.0 : v[e] * invdiag[e]) : v[e];
+        r = x + dt * F;
+    }
+    if (mask && dir[e]) r = 0.0;
+    if (bvec) r = dir[e] ? 0.0 : bvec[e] - r; // residual form r = b - A x
+    out[e] = r;
+}
+
+// delta += dir ? 0 : (sum_i y_i V_i) * invdiag  (ImplicitInduction.cpp:366-376)
+__global__ void __launch_bounds__(256) k_gmres_update(const double *const *V, const double *y, int32_t j, const double *invdiag,
+                                                      const int8_t *dir, int32_t n, double *delta)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double s = 0.0;
+    for (int32_t i = 0; i < j; ++i) s += y[i] * V[i][e];
+    delta[e] += dir[e] ? 0.0 : s * invdiag[e];
+}
+
+// elementwise helpers of solve()
+__global__ void __launch_bounds__(256) k_solve_init(const double *E_expl, const double *corr, const int8_t *dir, const double *m_e,
+                                                    const double *d2, double dt, int32_t n, double *x0, double *invdiag)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double x = E_expl[e];
+    if (corr && !dir[e]) x += corr[e];
+    x0[e] = x;
+    invdiag[e] = 1.0 / (1.0 + dt * m_e[e] * d2[e]);
+}
+__global__ void __launch_bounds__(256) k_add(const double *a, const double *b, int32_t n, double *out)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = a[e] + b[e];
+}
+__global__ void __launch_bounds__(256) k_solve_final(const double *x0, const double *delta, const double *E_expl, const int8_t *dir,
+                                                     int32_t n, double *E, double *corr)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double v = x0[e] + delta[e];
+    E[e] = v;
+    corr[e] = dir[e] ? 0.0 : v - E_expl[e];
+}
+} // namespace mhdg

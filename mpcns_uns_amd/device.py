@@ -1,0 +1,136 @@
+"""One mesh resident on the GPU (include/mhdg.h).
+
+Methods mirror the reference's mhd_common calls (host arrays in and out) and the fused Mercury step on the
+resident state.  Every call goes through the HIP kernels of libmhdg.so; a failure raises.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib, ptr, f64
+
+DERIVED = {"W_H": 0, "W_Na": 1, "B_ind": 2, "B_tot": 3, "J": 4, "u_plus": 5, "E": 6}
+
+
+class Device:
+    def __init__(self, mesh, cfg):
+        self.mesh = mesh
+        self.cfg = cfg
+        self.nc, self.nf, self.ne = mesh.n_cells, mesh.n_faces, mesh.n_edges
+        h = C.c_void_p()
+        if lib().mhdg_create(C.byref(mesh.desc()), C.byref(cfg), C.byref(h)):
+            raise RuntimeError("mhdg_create: " + lib().mhdg_last_error(None).decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mhdg_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _ok(self, rc):
+        if rc:
+            raise RuntimeError(lib().mhdg_last_error(self._h).decode())
+
+    @staticmethod
+    def info():
+        buf = C.create_string_buffer(256)
+        lib().mhdg_device_info(buf, 256)
+        return buf.value.decode()
+
+    # ------------------------------------------------------------------ mhd_common calls
+    def cell_vector(self, flux):
+        out = np.zeros(3 * self.nc)
+        self._ok(lib().mhdg_cell_vector(self._h, ptr(f64(flux)), ptr(out)))
+        return out
+
+    def cell_current(self, B):
+        out = np.zeros(3 * self.nc)
+        self._ok(lib().mhdg_cell_current(self._h, ptr(f64(B)), ptr(out)))
+        return out
+
+    def limited_gradients(self, W, species, fs, limit=True, ghost=None):
+        out = np.zeros(15 * self.nc)
+        g = None if ghost is None else f64(ghost)
+        self._ok(lib().mhdg_limited_gradients(self._h, ptr(f64(W)), species, ptr(f64(fs)), int(limit), ptr(g), ptr(out)))
+        return out
+
+    def flux_rhs(self, species, fs, W, grad, ghost=None):
+        out = np.zeros(5 * self.nc)
+        g = None if ghost is None else f64(ghost)
+        self._ok(lib().mhdg_flux_rhs(self._h, species, ptr(f64(fs)), ptr(f64(W)), ptr(f64(grad)), ptr(g), ptr(out)))
+        return out
+
+    def ideal_emf(self, u, B_tot, B_ind, phi_tot, phi_ind, cuct=0.5, E=None):
+        E = np.zeros(self.ne) if E is None else f64(E).copy()
+        self._ok(lib().mhdg_ideal_emf(self._h, ptr(f64(u)), ptr(f64(B_tot)), ptr(f64(B_ind)), ptr(f64(phi_tot)),
+                                      ptr(f64(phi_ind)), cuct, ptr(E)))
+        return E
+
+    def faraday(self, E, Phi, dt):
+        Phi = f64(Phi).copy()
+        self._ok(lib().mhdg_faraday(self._h, ptr(f64(E)), ptr(Phi), dt))
+        return Phi
+
+    def faraday_edge_dt(self, E, dt_edge, Phi):
+        Phi = f64(Phi).copy()
+        self._ok(lib().mhdg_faraday_edge_dt(self._h, ptr(f64(E)), ptr(f64(dt_edge)), ptr(Phi)))
+        return Phi
+
+    def div_b(self, B, extra=None):
+        a, r, w = C.c_double(), C.c_double(), C.c_int64()
+        self._ok(lib().mhdg_div_b(self._h, ptr(f64(B)), ptr(None if extra is None else f64(extra)), C.byref(a), C.byref(r), C.byref(w)))
+        return a.value, r.value, w.value
+
+    def local_dt(self, W_H, W_Na, B_tot):
+        dt = C.c_double()
+        self._ok(lib().mhdg_local_dt(self._h, ptr(f64(W_H)), ptr(f64(W_Na)), ptr(f64(B_tot)), C.byref(dt)))
+        return dt.value
+
+    def add_sources(self, species, W_H, W_Na, B_tot, J, dt, rhs):
+        rhs = f64(rhs).copy()
+        self._ok(lib().mhdg_add_sources(self._h, species, ptr(f64(W_H)), ptr(f64(W_Na)), ptr(f64(B_tot)), ptr(f64(J)), dt, ptr(rhs)))
+        return rhs
+
+    def ind_set_frozen(self, B0, ne_eff):
+        self._ok(lib().mhdg_ind_set_frozen(self._h, ptr(f64(B0)), ptr(f64(ne_eff))))
+
+    def ind_apply_F(self, phi):
+        out = np.zeros(self.ne)
+        self._ok(lib().mhdg_ind_apply_F(self._h, ptr(f64(phi)), ptr(out)))
+        return out
+
+    def ind_solve(self, Phi_n, E_expl, dirichlet, dt, precision=0):
+        E = np.zeros(self.ne)
+        it, rr = C.c_int32(), C.c_double()
+        d = np.ascontiguousarray(dirichlet, dtype=np.int8)
+        self._ok(lib().mhdg_ind_solve(self._h, ptr(f64(Phi_n)), ptr(f64(E_expl)), ptr(d), dt, precision, ptr(E),
+                                      C.byref(it), C.byref(rr)))
+        return E, it.value, rr.value
+
+    # ------------------------------------------------------------------ resident state
+    def set_state(self, U_H, U_Na, Phi):
+        self._ok(lib().mhdg_set_state(self._h, ptr(f64(U_H)), ptr(f64(U_Na)), ptr(f64(Phi))))
+
+    def get_state(self):
+        UH, UNa, Phi = np.zeros(5 * self.nc), np.zeros(5 * self.nc), np.zeros(self.nf)
+        self._ok(lib().mhdg_get_state(self._h, ptr(UH), ptr(UNa), ptr(Phi)))
+        return UH, UNa, Phi
+
+    def set_background(self, PhiAdd, BaddCell):
+        self._ok(lib().mhdg_set_background(self._h, ptr(f64(PhiAdd)), ptr(f64(BaddCell))))
+
+    def step(self, dt_max=1e9, precision=0):
+        dt, it, rr = C.c_double(), C.c_int32(), C.c_double()
+        self._ok(lib().mhdg_step(self._h, dt_max, precision, C.byref(dt), C.byref(it), C.byref(rr)))
+        return dt.value, it.value, rr.value
+
+    def derived(self, name):
+        n = {0: 5, 1: 5, 6: None}.get(DERIVED[name], 3)
+        out = np.zeros(self.ne if n is None else n * self.nc)
+        self._ok(lib().mhdg_get_derived(self._h, DERIVED[name], ptr(out)))
+        return out
+
+    def sync(self):
+        self._ok(lib().mhdg_sync(self._h))

@@ -1,0 +1,142 @@
+"""Host-side mesh: complex, geometry, boundary classification, background field and seeded states.
+
+Wraps include/mhdg_mesh.h.  All arrays are copied into numpy so a Mesh outlives nothing native except the
+handle used to build descriptors.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import MeshDesc, Config, lib, ptr, f64
+
+BC_TYPES = {"wall": 0, "inflow": 1, "outflow": 2, "farfield": 3}
+
+
+class Mesh:
+    """A cell complex with its geometry (one handle of the native library)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        cnt = np.zeros(8, np.int64)
+        lib().mhdg_mesh_counts(self._h, ptr(cnt))
+        self.n_nodes, self.n_edges, self.n_faces, self.n_cells = (int(x) for x in cnt[:4])
+        self._desc = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().mhdg_mesh_free(h)
+            self._h = None
+
+    # ---------------------------------------------------------------- construction
+    @classmethod
+    def shell(cls, n_lat=24, n_lon=48, n_r=9, r_in=1.0, r_out=4.0, ratio=1.05, jitter=0.05, seed=20261202):
+        """Seeded tetrahedral lat-lon shell (tags 1 = inner, 2 = outer)."""
+        h = C.c_void_p()
+        if lib().mhdg_mesh_shell(n_lat, n_lon, n_r, r_in, r_out, ratio, jitter, seed, C.byref(h)):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+        return cls(h)
+
+    @classmethod
+    def from_cells(cls, xyz, cell_loops, material=None, tags=()):
+        """cell_loops: per cell a list of outward node loops; tags: iterable of (nodes, tag_id)."""
+        xyz = f64(xyz).reshape(-1, 3)
+        cell_ptr, loop_ptr, nodes = [0], [0], []
+        for loops in cell_loops:
+            for lp in loops:
+                nodes.extend(int(v) for v in lp)
+                loop_ptr.append(len(nodes))
+            cell_ptr.append(len(loop_ptr) - 1)
+        tag_ptr, tag_nodes, tag_id = [0], [], []
+        for nds, t in tags:
+            tag_nodes.extend(int(v) for v in nds)
+            tag_ptr.append(len(tag_nodes))
+            tag_id.append(int(t))
+        a = lambda v: np.ascontiguousarray(np.asarray(v, dtype=np.int32))
+        cp, lp, nd, tp, tn, ti = a(cell_ptr), a(loop_ptr), a(nodes), a(tag_ptr), a(tag_nodes), a(tag_id)
+        mat = None if material is None else a(material)
+        h = C.c_void_p()
+        if lib().mhdg_mesh_from_cells(len(xyz), ptr(xyz), len(cell_loops), ptr(cp), ptr(lp), ptr(nd), ptr(mat),
+                                      len(ti), ptr(tp), ptr(tn), ptr(ti), C.byref(h)):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+        m = cls(h)
+        m._loops = (cp, lp, nd, tp, tn, ti, mat)
+        return m
+
+    # ---------------------------------------------------------------- arrays
+    def array(self, name):
+        p, n, es = C.c_void_p(), C.c_int64(), C.c_int32()
+        if lib().mhdg_mesh_array(self._h, name.encode(), C.byref(p), C.byref(n), C.byref(es)):
+            raise KeyError(name)
+        if n.value == 0:
+            return np.zeros(0)
+        if name.endswith(".ptr") or name == "ec_ptr":
+            dt = np.int64
+        elif name.endswith(".sgn") or name in ("face_kind", "edge_mode"):
+            dt = np.int8
+        elif name.endswith(".idx") or name in ("tets", "face_tag", "material", "ec_cell", "ec_misoriented"):
+            dt = np.int32
+        else:
+            dt = np.float64
+        assert np.dtype(dt).itemsize == es.value, name
+        buf = (C.c_char * (n.value * es.value)).from_address(p.value)
+        return np.frombuffer(buf, dtype=dt).copy()
+
+    def boundary(self, tag_types=None, default="outflow", u_inf=(-1.0, 0.0, 0.0)):
+        """Face kinds and edge modes from face tags (mhd::BoundaryInfo::build); default: inner wall, outer farfield."""
+        if tag_types is None:
+            tag_types = {1: "wall", 2: "farfield"}
+        n = max(tag_types) + 1 if tag_types else 1
+        tt = np.full(n, -1, np.int32)
+        for k, v in tag_types.items():
+            tt[k] = BC_TYPES[v]
+        u = f64(u_inf)
+        if lib().mhdg_mesh_boundary(self._h, ptr(tt), n, BC_TYPES[default], ptr(u)):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+        self._desc = None
+        return self
+
+    def desc(self):
+        """mhdg_mesh_desc pointing into this mesh (valid while the Mesh lives)."""
+        if self._desc is None:
+            d = MeshDesc()
+            if lib().mhdg_mesh_desc_fill(self._h, C.byref(d)):
+                raise RuntimeError(lib().mhdg_mesh_error().decode())
+            self._desc = d
+        return self._desc
+
+    def background(self, B_imf=(0, 0, 0), dipole_m=None, dipole_c=(0, 0, 0), soft=1e-3):
+        PhiAdd = np.zeros(self.n_faces)
+        Badd = np.zeros(3 * self.n_cells)
+        B, m, c = f64(B_imf), f64(dipole_m if dipole_m is not None else (0, 0, 0)), f64(dipole_c)
+        lib().mhdg_mesh_background(self._h, ptr(B), int(dipole_m is not None), ptr(m), ptr(c), soft, ptr(PhiAdd), ptr(Badd))
+        return PhiAdd, Badd
+
+    def cell_vector(self, flux):
+        out = np.zeros(3 * self.n_cells)
+        lib().mhdg_mesh_cell_vector(self._h, ptr(f64(flux)), ptr(out))
+        return out
+
+    def seeded_state(self, cfg, noise, seed, Badd):
+        UH, UNa = np.zeros(5 * self.n_cells), np.zeros(5 * self.n_cells)
+        Phi = np.zeros(self.n_faces)
+        lib().mhdg_mesh_seeded_state(self._h, C.byref(cfg), noise, seed, ptr(f64(Badd)), ptr(UH), ptr(UNa), ptr(Phi))
+        return UH, UNa, Phi
+
+
+def default_config(**kw):
+    """mhdg_config with the reference defaults, then keyword overrides (fs as 2x5, u_inf, B_imf as 3)."""
+    c = Config()
+    lib().mhdg_config_defaults(C.byref(c))
+    for k, v in kw.items():
+        if k == "fs":
+            for s in range(2):
+                for m in range(5):
+                    c.fs[s][m] = float(v[s][m])
+        elif k in ("u_inf", "B_imf"):
+            arr = getattr(c, k)
+            for i in range(3):
+                arr[i] = float(v[i])
+        else:
+            setattr(c, k, v)
+    return c

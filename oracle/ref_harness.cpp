@@ -1,0 +1,282 @@
+// ORACLE — test infrastructure only.  C-ABI harness around the compiled reference (MPCNS_Uns sources
+// built in place from /root/reference by oracle/Makefile into oracle/_ref/).  It constructs the
+// reference's own objects (ComplexBuilder, compute_geometry, BoundaryInfo, Stencil, ImplicitInduction,
+// mercury::Solver) on a mesh given as plain arrays and calls them, one rank (MPI_COMM_SELF).
+// Used only by tests/ to pin the oracle restatement and the host setup against the reference.
+#include <mpi.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "boundary/BoundaryInfo.h"
+#include "config/Config.h"
+#include "diagnostics/DivB.h"
+#include "driver/Solver.h"
+#include "fluid/FluxRHS.h"
+#include "fluid/MercurySources.h"
+#include "fluid/Reconstruction.h"
+#include "induction/Current.h"
+#include "induction/EdgeEMF.h"
+#include "induction/Faraday.h"
+#include "induction/IdealEMF.h"
+#include "induction/ImplicitInduction.h"
+#include "initial/Initial.h"
+#include "uns/comm/Exchanger.h"
+#include "uns/complex/ComplexBuilder.h"
+#include "uns/complex/Validate.h"
+#include "uns/geometry/Geometry.h"
+#include "uns/geometry/Reconstruct.h"
+
+#include "../include/mhdg.h"
+
+using namespace uns;
+using namespace mhd;
+
+struct ref_ctx
+{
+    LocalMesh lm;
+    Geometry geo;
+    Config cfg;
+    std::unique_ptr<comm::Exchanger> ex;
+    std::unique_ptr<BoundaryInfo> bc;
+    std::unique_ptr<Stencil> st;
+    std::unique_ptr<ImplicitInduction> ind;
+    std::unique_ptr<mercury::Solver> solver;
+    std::vector<char> active;
+    std::vector<double> PhiAdd, BaddCell;
+    std::string err;
+};
+
+namespace
+{
+Config make_config(const mhdg_config &c, int dipole, const double *m, const double *cc, double soft)
+{
+    Config k;
+    k.sc = Scales::simple(c.gamma, c.inver_MA2, c.M_Na / c.M_H);
+    k.sc.M_ref = c.M_ref; k.sc.M_H = c.M_H; k.sc.M_Na = c.M_Na;
+    k.sc.ne_floor = c.ne_floor;
+    k.sc.hall_coef = c.hall_coef;
+    k.sc.inver_Rem = c.inver_Rem;
+    k.cfl = c.cfl;
+    k.second_order = c.second_order != 0;
+    k.recon_rel_limit = c.recon_rel_limit;
+    k.rho_floor = c.rho_floor; k.p_floor = c.p_floor; k.rho_floor_dt = c.rho_floor_dt;
+    k.lorentz_force = c.lorentz_force != 0;
+    k.time_order = 1;
+    k.hall = c.hall != 0;
+    k.resistive = c.resistive != 0;
+    k.eta_r_inner = c.eta_r_inner; k.eta_r_outer = c.eta_r_outer; k.eta_width = c.eta_width;
+    k.hall_r_min = c.hall_r_min; k.hall_r_max = c.hall_r_max;
+    k.ind_rtol = c.ind_rtol; k.ind_max_it = c.ind_max_it; k.ind_restart = c.ind_restart;
+    k.ind_ilu = false;
+    k.ideal.cuct = c.cuct;
+    k.fsH = {c.fs[0][0], c.fs[0][4]};
+    k.fsNa = {c.fs[1][0], c.fs[1][4]};
+    k.u_inf = {c.u_inf[0], c.u_inf[1], c.u_inf[2]};
+    k.na_seed_at_rest = !(c.fs[1][1] == c.u_inf[0] && c.fs[1][2] == c.u_inf[1] && c.fs[1][3] == c.u_inf[2]);
+    k.B_imf = {c.B_imf[0], c.B_imf[1], c.B_imf[2]};
+    k.dipole = dipole != 0;
+    if (m) k.dipole_m = {m[0], m[1], m[2]};
+    if (cc) k.dipole_c = {cc[0], cc[1], cc[2]};
+    k.dipole_soft = soft;
+    k.write_vtk = false; k.write_fields = false; k.write_profile = false; k.log_every = 0;
+    k.bc_default = "outflow";
+    return k;
+}
+Prim prim5(const double *p) { return Prim{p[0], p[1], p[2], p[3], p[4]}; }
+template <class T> void put(const std::vector<T> &v, T *out) { std::memcpy(out, v.data(), v.size() * sizeof(T)); }
+void put3(const std::vector<Vec3> &v, double *out)
+{
+    for (size_t i = 0; i < v.size(); ++i) { out[3 * i] = v[i].x; out[3 * i + 1] = v[i].y; out[3 * i + 2] = v[i].z; }
+}
+} // namespace
+
+extern "C" {
+// cells: cell c has loops [cell_ptr[c], cell_ptr[c+1]) ; loop l has nodes loop_nodes[loop_ptr[l] .. loop_ptr[l+1])
+// tags: tag t has nodes tag_nodes[tag_ptr[t] .. tag_ptr[t+1]) and id tag_id[t]; tag names "inner" = 1, "outer" = 2, box faces 1..6
+int ref_create(int32_t n_nodes, const double *xyz, int64_t n_cells, const int32_t *cell_ptr, const int32_t *loop_ptr,
+               const int32_t *loop_nodes, const int32_t *material, int64_t n_tags, const int32_t *tag_ptr, const int32_t *tag_nodes,
+               const int32_t *tag_id, int tagset, const mhdg_config *cfg, int dipole, const double *dm, const double *dc, double soft,
+               ref_ctx **out)
+{
+    try
+    {
+        int init = 0;
+        MPI_Initialized(&init);
+        if (!init) MPI_Init(nullptr, nullptr);
+        auto *r = new ref_ctx();
+        ComplexBuilder b(n_nodes);
+        std::vector<std::vector<Id>> loops;
+        for (int64_t c = 0; c < n_cells; ++c)
+        {
+            loops.clear();
+            for (int32_t l = cell_ptr[c]; l < cell_ptr[c + 1]; ++l) loops.emplace_back(loop_nodes + loop_ptr[l], loop_nodes + loop_ptr[l + 1]);
+            b.add_cell(loops, material ? material[c] : kFluid);
+        }
+        for (int64_t t = 0; t < n_tags; ++t) b.tag_face(std::vector<Id>(tag_nodes + tag_ptr[t], tag_nodes + tag_ptr[t + 1]), tag_id[t]);
+        r->lm.mesh.cx = b.finish();
+        r->lm.mesh.x.resize(n_nodes);
+        for (int32_t i = 0; i < n_nodes; ++i) r->lm.mesh.x[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        if (tagset == 0) r->lm.mesh.tag_names = {"", "inner", "outer"};
+        else r->lm.mesh.tag_names = {"", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax"};
+        r->cfg = make_config(*cfg, dipole, dm, dc, soft);
+        r->cfg.bc_by_tag["inner"] = "wall";
+        r->cfg.bc_by_tag["outer"] = "farfield";
+        r->geo = compute_geometry(r->lm.mesh);
+        r->ex = std::make_unique<comm::Exchanger>(MPI_COMM_SELF, r->lm);
+        r->bc = std::make_unique<BoundaryInfo>(BoundaryInfo::build(r->lm, r->geo, r->cfg));
+        r->st = std::make_unique<Stencil>(r->lm, r->geo, *r->bc);
+        r->st->set_rel_limit(r->cfg.recon_rel_limit);
+        r->ind = std::make_unique<ImplicitInduction>(r->lm, r->geo, *r->ex, r->cfg);
+        r->active = active_faces(r->lm.mesh.cx);
+        mercury::set_background(r->lm, r->geo, r->cfg, r->PhiAdd, r->BaddCell);
+        *out = r;
+        return 0;
+    }
+    catch (const std::exception &e)
+    {
+        std::fprintf(stderr, "ref_create: %s\n", e.what());
+        return 1;
+    }
+}
+void ref_destroy(ref_ctx *r) { delete r; }
+
+// counts: nodes, edges, faces, cells, nnz(e2n), nnz(f2e), nnz(c2f), nnz(ec)
+void ref_counts(ref_ctx *r, int64_t *n)
+{
+    const CellComplex &cx = r->lm.mesh.cx;
+    n[0] = cx.n_nodes(); n[1] = cx.n_edges(); n[2] = cx.n_faces(); n[3] = cx.n_cells();
+    n[4] = cx.e2n.nnz(); n[5] = cx.f2e.nnz(); n[6] = cx.c2f.nnz(); n[7] = static_cast<int64_t>(r->geo.ec_cell.size());
+}
+// which: 0 e2n 1 f2e 2 c2f 3 e2f 4 f2c
+void ref_incidence(ref_ctx *r, int which, int64_t *ptr, int32_t *idx, int8_t *sgn)
+{
+    const CellComplex &cx = r->lm.mesh.cx;
+    const Incidence *t[5] = {&cx.e2n, &cx.f2e, &cx.c2f, &cx.e2f, &cx.f2c};
+    put(t[which]->ptr, ptr); put(t[which]->idx, idx); put(t[which]->sgn, sgn);
+}
+void ref_face_tags(ref_ctx *r, int32_t *tags) { put(r->lm.mesh.cx.face_tag, tags); }
+// topology report: euler, boundary faces, d1d0 violations, d2d1 violations, ok
+void ref_validate(ref_ctx *r, int64_t *out)
+{
+    const TopologyReport t = validate_topology(r->lm.mesh.cx);
+    out[0] = t.euler; out[1] = t.boundary_faces; out[2] = t.d1d0_violations; out[3] = t.d2d1_violations; out[4] = t.ok;
+}
+// geometry field by name
+int ref_geometry(ref_ctx *r, const char *name, void *out)
+{
+    const Geometry &g = r->geo;
+    const std::string n(name);
+    if (n == "edge_vec") put3(g.edge_vec, (double *)out);
+    else if (n == "edge_mid") put3(g.edge_mid, (double *)out);
+    else if (n == "edge_len") put(g.edge_len, (double *)out);
+    else if (n == "face_area") put3(g.face_area, (double *)out);
+    else if (n == "face_center") put3(g.face_center, (double *)out);
+    else if (n == "face_area_mag") put(g.face_area_mag, (double *)out);
+    else if (n == "cell_center") put3(g.cell_center, (double *)out);
+    else if (n == "cell_volume") put(g.cell_volume, (double *)out);
+    else if (n == "edge_dual_area") put3(g.edge_dual_area, (double *)out);
+    else if (n == "ec_ptr") put(g.ec_ptr, (int64_t *)out);
+    else if (n == "ec_cell") put(g.ec_cell, (int32_t *)out);
+    else if (n == "ec_weight") put(g.ec_weight, (double *)out);
+    else if (n == "ef_weight") put(g.ef_weight, (double *)out);
+    else if (n == "star1") put(g.star1, (double *)out);
+    else if (n == "star2") put(g.star2, (double *)out);
+    else if (n == "star3") put(g.star3, (double *)out);
+    else return 1;
+    return 0;
+}
+void ref_boundary(ref_ctx *r, int8_t *face_kind, int8_t *edge_mode)
+{
+    for (size_t f = 0; f < r->bc->face_kind.size(); ++f) face_kind[f] = static_cast<int8_t>(r->bc->face_kind[f]);
+    for (size_t e = 0; e < r->bc->edge_mode.size(); ++e) edge_mode[e] = static_cast<int8_t>(r->bc->edge_mode[e]);
+}
+void ref_background(ref_ctx *r, double *PhiAdd, double *BaddCell) { put(r->PhiAdd, PhiAdd); put(r->BaddCell, BaddCell); }
+
+void ref_cell_vector(ref_ctx *r, const double *flux, double *b)
+{
+    const CellComplex &cx = r->lm.mesh.cx;
+    put3(reconstruct_cell_vector(cx, r->geo, flux, cx.n_cells()), b);
+}
+void ref_cell_current(ref_ctx *r, const double *B, double *J) { cell_current(r->lm, r->geo, B, J); }
+void ref_limited_gradients(ref_ctx *r, const double *W, int species, const double *fs, int limit, double *grad)
+{
+    r->st->limited_gradients(r->lm, r->geo, *r->bc, W, species, prim5(fs), limit != 0, grad);
+}
+void ref_flux_rhs(ref_ctx *r, int species, const double *fs, const double *W, const double *grad, double *rhs)
+{
+    fluid_flux_rhs(r->lm, r->geo, *r->bc, r->cfg, species, prim5(fs), W, grad, rhs);
+}
+void ref_ideal_emf(ref_ctx *r, const double *u, const double *Bt, const double *Bi, const double *pt, const double *pi, double cuct, double *E)
+{
+    IdealInputs in;
+    in.u_cell = u; in.B_cell_tot = Bt; in.B_cell_ind = Bi; in.phi_tot = pt; in.phi_ind = pi;
+    IdealParams p;
+    p.cuct = cuct;
+    add_ideal_emf(r->lm, r->geo, in, p, E);
+}
+void ref_faraday(ref_ctx *r, const double *E, double *Phi, double dt) { faraday_update(r->lm.mesh.cx, r->active, E, Phi, dt); }
+void ref_div_b(ref_ctx *r, const double *B, double *max_abs, double *max_rel, int64_t *worst)
+{
+    const DivBReport d = div_b(r->lm.mesh.cx, B);
+    *max_abs = d.max_abs; *max_rel = d.max_rel; *worst = d.worst_cell_gid;
+}
+double ref_local_dt(ref_ctx *r, const double *WH, const double *WNa, const double *B) { return mercury::local_time_step(r->lm, r->geo, r->cfg, WH, WNa, B); }
+void ref_add_sources(ref_ctx *r, int s, const double *WH, const double *WNa, const double *up, const double *B, const double *J, double dt, double *rhs)
+{
+    mercury::SourceFields sf;
+    mercury::add_sources(r->lm, r->cfg, s, WH, WNa, up, B, J, sf, dt, rhs);
+}
+void ref_ind_set_frozen(ref_ctx *r, const double *B0, const double *ne) { r->ind->set_frozen_state(B0, ne); }
+void ref_ind_apply_F(ref_ctx *r, const double *phi, double *out) { r->ind->apply_F(phi, out); }
+int ref_ind_solve(ref_ctx *r, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res)
+{
+    std::vector<char> d(dir, dir + r->lm.mesh.cx.n_edges());
+    return r->ind->solve(Phi_n, E_expl, d, dt, E, rel_res);
+}
+
+// ---- full step through mercury::Solver (MercuryUns/driver/Solver.cpp:228-277)
+int ref_solver_init(ref_ctx *r)
+{
+    try
+    {
+        r->solver = std::make_unique<mercury::Solver>(MPI_COMM_SELF, r->lm, r->cfg);
+        r->solver->initialize();
+        return 0;
+    }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_solver_init: %s\n", e.what()); return 1; }
+}
+void ref_set_state(ref_ctx *r, const double *UH, const double *UNa, const double *Phi)
+{
+    auto &s = r->solver->state();
+    std::memcpy(s.U[0].data(), UH, s.U[0].size() * 8);
+    std::memcpy(s.U[1].data(), UNa, s.U[1].size() * 8);
+    std::memcpy(s.Phi.data(), Phi, s.Phi.size() * 8);
+}
+void ref_get_state(ref_ctx *r, double *UH, double *UNa, double *Phi)
+{
+    const auto &s = r->solver->state();
+    put(s.U[0], UH); put(s.U[1], UNa); put(s.Phi, Phi);
+}
+int ref_step(ref_ctx *r, double dt_max, double *dt)
+{
+    try { *dt = r->solver->step_first_order(dt_max); return 0; }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_step: %s\n", e.what()); return 1; }
+}
+// 0 W_H 1 W_Na 2 B_ind 3 B_tot 4 J
+void ref_get_derived(ref_ctx *r, int which, double *out)
+{
+    switch (which)
+    {
+    case 0: put(r->solver->prim(0), out); break;
+    case 1: put(r->solver->prim(1), out); break;
+    case 2: put(r->solver->B_induced_cell(), out); break;
+    case 3: put(r->solver->B_total_cell(), out); break;
+    case 4: put(r->solver->current_cell(), out); break;
+    }
+}
+double ref_last_induction_residual(ref_ctx *r) { return r->solver->last_induction_residual(); }
+}

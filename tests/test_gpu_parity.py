@@ -1,0 +1,147 @@
+"""Parity of the HIP path (C-ABI) with the CPU oracle on seeded shell meshes.
+
+The oracle reproduces the reference bit for bit (tests/test_oracle_reference.py); the device kernels mirror
+the same evaluation order without contraction, so every explicit operation is compared for exact equality.
+The GMRES solve uses parallel (fixed-tree) dot products, so its result is compared to a tolerance.
+"""
+import numpy as np
+import pytest
+
+from conftest import maxrel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pair(small_hall):
+    from mpcns_uns_amd import Device
+    from oracle.oracle import Oracle
+
+    w = small_hall
+    dev, orc = Device(w["mesh"], w["cfg"]), Oracle(w["mesh"], w["cfg"])
+    for x in (dev, orc):
+        x.set_background(*w["background"])
+        x.set_state(*w["state"])
+    orc.step()  # derived fields of the initial state
+    fields = {n: orc.derived(n) for n in ("W_H", "W_Na", "B_ind", "B_tot", "J", "u_plus")}
+    return dev, orc, w, fields
+
+
+def test_cell_vector_and_current(pair):
+    dev, orc, w, f = pair
+    Phi = w["state"][2]
+    assert np.array_equal(dev.cell_vector(Phi), orc.cell_vector(Phi))
+    assert np.array_equal(dev.cell_current(f["B_ind"]), orc.cell_current(f["B_ind"]))
+
+
+@pytest.mark.parametrize("species", [0, 1])
+@pytest.mark.parametrize("limit", [True, False])
+def test_limited_gradients(pair, species, limit):
+    dev, orc, w, f = pair
+    W = f["W_H"] if species == 0 else f["W_Na"]
+    fs = np.array(w["cfg"].fs[species][:])
+    assert np.array_equal(dev.limited_gradients(W, species, fs, limit), orc.limited_gradients(W, species, fs, limit))
+
+
+@pytest.mark.parametrize("species", [0, 1])
+def test_flux_rhs(pair, species):
+    dev, orc, w, f = pair
+    W = f["W_H"] if species == 0 else f["W_Na"]
+    fs = np.array(w["cfg"].fs[species][:])
+    g = orc.limited_gradients(W, species, fs, True)
+    assert np.array_equal(dev.flux_rhs(species, fs, W, g), orc.flux_rhs(species, fs, W, g))
+
+
+def test_flux_rhs_ghost_table(pair):
+    dev, orc, w, f = pair
+    rng = np.random.default_rng(3)
+    ghost = np.abs(rng.normal(1.0, 0.1, 5 * dev.nf))
+    fs = np.array(w["cfg"].fs[0][:])
+    g = orc.limited_gradients(f["W_H"], 0, fs, True, ghost)
+    assert np.array_equal(dev.limited_gradients(f["W_H"], 0, fs, True, ghost), g)
+    assert np.array_equal(dev.flux_rhs(0, fs, f["W_H"], g, ghost), orc.flux_rhs(0, fs, f["W_H"], g, ghost))
+
+
+def test_ideal_emf_heavy_edges(pair):
+    """Random u and B: every edge class (one thread per edge, pole axes with valence n_lon by a block)."""
+    dev, orc, w, f = pair
+    rng = np.random.default_rng(11)
+    u = rng.uniform(-1, 1, 3 * dev.nc)
+    Bt, Bi = rng.normal(size=3 * dev.nc), rng.normal(size=3 * dev.nc)
+    pt, pi = rng.normal(size=dev.nf), rng.normal(size=dev.nf)
+    E0 = rng.normal(size=dev.ne)
+    assert np.array_equal(dev.ideal_emf(u, Bt, Bi, pt, pi, 0.5, E0), orc.ideal_emf(u, Bt, Bi, pt, pi, 0.5, E0))
+    valence = np.diff(w["mesh"].array("e2f.ptr"))
+    assert valence.max() == 48 > 32  # pole axes of the 12 x 48 net go through the block-per-edge kernel
+
+
+def test_faraday_divb_dt_sources(pair):
+    dev, orc, w, f = pair
+    rng = np.random.default_rng(5)
+    E = rng.normal(size=dev.ne)
+    Phi = w["state"][2]
+    assert np.array_equal(dev.faraday(E, Phi, 0.37), orc.faraday(E, Phi, 0.37))
+    dte = rng.uniform(0.1, 1, dev.ne)
+    assert np.array_equal(dev.faraday_edge_dt(E, dte, Phi), orc.faraday_edge_dt(E, dte, Phi))
+    B = dev.faraday(E, Phi, 0.37)
+    assert dev.div_b(B) == orc.div_b(B)
+    PA = w["background"][0]
+    assert dev.div_b(B, PA) == orc.div_b(B, PA)
+    assert dev.local_dt(f["W_H"], f["W_Na"], f["B_tot"]) == orc.local_dt(f["W_H"], f["W_Na"], f["B_tot"])
+    rhs = rng.normal(size=5 * dev.nc)
+    for s in (0, 1):
+        assert np.array_equal(dev.add_sources(s, f["W_H"], f["W_Na"], f["B_tot"], f["J"], 1e-3, rhs),
+                              orc.add_sources(s, f["W_H"], f["W_Na"], f["B_tot"], f["J"], 1e-3, rhs))
+
+
+def test_apply_F_and_solve(pair):
+    dev, orc, w, f = pair
+    rng = np.random.default_rng(7)
+    ne_eff = np.sqrt((f["W_H"][0::5] + f["W_Na"][0::5] / 23.0) ** 2 + 1e-4)
+    for x in (dev, orc):
+        x.ind_set_frozen(f["B_tot"], ne_eff)
+    phi = rng.normal(size=dev.nf)
+    assert np.array_equal(dev.ind_apply_F(phi), orc.ind_apply_F(phi))
+    E_expl = rng.normal(size=dev.ne) * 1e-3
+    dirichlet = (w["mesh"].array("edge_mode") == 2).astype(np.int8)
+    Ed, itd, rd = dev.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3)
+    Eo, ito, ro = orc.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3)
+    assert rd < 1e-8 and ro < 1e-8
+    assert abs(itd - ito) <= 1
+    assert maxrel(Ed, Eo) < 1e-7
+
+
+def _fresh(w):
+    from mpcns_uns_amd import Device
+    from oracle.oracle import Oracle
+
+    dev, orc = Device(w["mesh"], w["cfg"]), Oracle(w["mesh"], w["cfg"])
+    for x in (dev, orc):
+        x.set_background(*w["background"])
+        x.set_state(*w["state"])
+    return dev, orc
+
+
+def test_step_explicit_bitwise(small_explicit):
+    dev, orc = _fresh(small_explicit)
+    for _ in range(3):
+        dd = dev.step()
+        do = orc.step()
+        assert dd == do
+        for a, b in zip(dev.get_state(), orc.get_state()):
+            assert np.array_equal(a, b)
+        for n in ("W_H", "W_Na", "B_ind", "B_tot", "J", "u_plus", "E"):
+            assert np.array_equal(dev.derived(n), orc.derived(n)), n
+
+
+def test_step_hall(small_hall):
+    dev, orc = _fresh(small_hall)
+    for _ in range(3):
+        dt_d, it_d, r_d = dev.step()
+        dt_o, it_o, r_o = orc.step()
+        assert r_d < 1e-8 and abs(it_d - it_o) <= 1
+        sd, so = dev.get_state(), orc.get_state()
+        for a, b in zip(sd, so):
+            assert maxrel(a, b) < 1e-12
+        _, rel, _ = dev.div_b(sd[2], small_hall["background"][0])
+        assert rel < 1e-12

@@ -1,0 +1,180 @@
+"""Benchmark of the Hall-MHD step on one MI355X: cell-steps/s of the Mercury step (explicit stage + implicit
+Hall/resistive GMRES solve to 1e-8 + constrained transport) on the seeded 7.83M-cell tetrahedral shell.
+
+Prints one JSON line.  `value` = cells x timed steps / wall seconds of the timed region (state resident in
+HBM).  `roofline` is the dominant kernel's algorithmic HBM bytes per launch over its average launch time,
+measured with HIP events on the context stream during the timed region.  `cpu_baseline` times the CPU
+code (the compiled reference where present, else the oracle restatement) on a bounded sample of the same
+workload in a subprocess.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def kernel_bytes(mesh, name, precision_bytes=8):
+    """Compulsory HBM bytes of one launch of a kernel: every array it reads or writes counted once."""
+    C, F, E = mesh.n_cells, mesh.n_faces, mesh.n_edges
+    nc2f, nf2e, nec = 4 * C, 3 * F, int(mesh.array("ec_ptr")[-1])
+    b = precision_bytes
+    table = {
+        # x (E) read, phi (F) write, f2e ptr+idx
+        "d1": E * b + F * b + 4 * (F + 1) + 4 * nf2e,
+        # phi (F), recon (3 per c2f entry), b (3C) write, c2f ptr+idx
+        "bcell": F * b + 3 * nc2f * 8 + 3 * C * b + 4 * (C + 1) + 4 * nc2f,
+        # b (3C), alpha (C), B0 (3C), area (3F), f2c (2F), volume, g (3C) write, c2f
+        "hall_g": 3 * C * b + C * 8 + 3 * C * 8 + 3 * F * 8 + 8 * F + C * 8 + 3 * C * b + 4 * (C + 1) + 4 * nc2f,
+        # phi, star2, e2f, g (3C), ec lists + weights, edge vec, m_e, v, invdiag, dir, out
+        "edge_apply": F * b + F * 8 + 4 * (E + 1) + 4 * nf2e + 3 * C * b + 4 * (E + 1) + 12 * nec + 24 * E + 8 * E
+                      + 2 * E * b + 8 * E + E + E * b,
+        # one fused MGS step: w read + write, two basis vectors
+        "mgs": 4 * E * b,
+        "prepare": 2 * 5 * C * 8 * 2 + F * 8 + 3 * C * 8 * 5 + C * 8 + 4 * F * 8 + 4 * nc2f + 4 * (C + 1) + C * 8 + C,
+        "grad_current": 2 * (5 * C * 8 + 15 * C * 8) + 3 * C * 8 * 2 + 6 * C * 8 + nc2f * (4 + 8 + 24) + 3 * F * 8 * 2 + 3 * C * 8
+                        + 8 * F + C * 8 + C + 4 * (C + 1),
+        "face": 2 * (5 * C * 8 + 15 * C * 8) + 10 * F * 8 + 3 * F * 8 + 3 * C * 8 * 3 + F * 8 * 2 + 3 * F * 8 * 2 + F * 8 + 8 * F
+                + 3 * C * 8 + F,
+        "cell_update": 10 * F * 8 + 2 * 5 * C * 8 * 3 + 3 * C * 8 * 2 + C * 8 + C + 4 * nc2f + 4 * (C + 1),
+        "edge_emf": 3 * F * 8 + 3 * C * 8 + F * 8 + 4 * (E + 1) + 4 * nf2e + 8 * nf2e + 4 * (E + 1) + 4 * nec + 3 * F * 8 * 2 + F * 8
+                    + 3 * E * 8 * 2 + E * 8 + E * 8 + E,
+        "faraday": E * 8 + F * 8 * 2 + 4 * (F + 1) + 4 * nf2e + 8 * F,
+    }
+    return table.get(name)
+
+
+def run_steps(dev, steps, precision, dt_max):
+    its, res = [], []
+    for _ in range(steps):
+        _, it, rr = dev.step(dt_max, precision)
+        its.append(it)
+        res.append(rr)
+    return its, res
+
+
+def cpu_baseline(args):
+    cmd = [sys.executable, "-u", os.path.join(HERE, "oracle", "cpu_baseline.py"), "--net", *map(str, args.cpu_net),
+           "--steps", str(args.cpu_steps)]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+        return json.loads(line[-1]) if line else {"error": (r.stderr or r.stdout)[-400:]}
+    except Exception as e:  # the baseline is reported, never required
+        return {"error": str(e)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="shell_8m_mixed")
+    ap.add_argument("--net", type=int, nargs=3, default=None, help="override the lat-lon net (n_lat n_lon n_r)")
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--no-fp64-reference", action="store_true", help="skip the plain FP64 step measurement")
+    ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
+    ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
+    ap.add_argument("--cpu-net", type=int, nargs=3, default=(96, 192, 10))
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; the path shards by mesh partition (see DESIGN.md), "
+                                   "multi-GPU runs are not part of it", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from mpcns_uns_amd import Device, workloads
+
+    t0 = time.time()
+    w = workloads.build(args.workload, net=tuple(args.net) if args.net else None)
+    mesh, cfg = w["mesh"], w["cfg"]
+    setup_s = time.time() - t0
+    print(f"# workload {args.workload} cells={mesh.n_cells} faces={mesh.n_faces} edges={mesh.n_edges} setup={setup_s:.1f}s",
+          file=sys.stderr, flush=True)
+    dev = Device(mesh, cfg)
+    dev.set_background(*w["background"])
+    precision = 1 if args.precision == "mixed" else 0
+    dt_max = 1e9
+
+    def measure(prec, profile):
+        dev.set_state(*w["state"])
+        run_steps(dev, args.warmup, prec, dt_max)
+        dev.sync()
+        if profile:
+            dev.profile(None)
+            dev.profile(True)
+        t = time.perf_counter()
+        its, res = run_steps(dev, args.steps, prec, dt_max)
+        dev.sync()
+        el = time.perf_counter() - t
+        prof = None
+        if profile:
+            dev.profile(False)
+            prof = dev.profile_report()
+        return el, its, res, prof
+
+    # main measurement: profiled timed region (events are recorded on the context stream)
+    el, its, res, prof = measure(precision, True)
+    value = mesh.n_cells * args.steps / el
+    print(f"# {args.precision}: {el / args.steps * 1e3:.2f} ms/step, GMRES it/step {np.mean(its):.1f}, max res {max(res):.2e}",
+          file=sys.stderr, flush=True)
+    fp64 = None
+    if precision == 1 and not args.no_fp64_reference:
+        el64, its64, res64, _ = measure(0, False)
+        fp64 = dict(value=mesh.n_cells * args.steps / el64, ms_per_step=el64 / args.steps * 1e3,
+                    gmres_iterations_per_step=float(np.mean(its64)), max_rel_residual=float(max(res64)))
+        print(f"# fp64: {fp64['ms_per_step']:.2f} ms/step, GMRES it/step {np.mean(its64):.1f}", file=sys.stderr, flush=True)
+
+    # roofline of the dominant kernel (largest total time in the timed region)
+    roof = None
+    kernels = {}
+    if prof:
+        total_ms = sum(v[1] for v in prof.values())
+        for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+            kernels[k] = dict(launches=n, ms_total=round(ms, 4), share=round(ms / total_ms, 4))
+        dom = max(prof, key=lambda k: prof[k][1] if kernel_bytes(mesh, k) else -1)
+        n, ms = prof[dom]
+        byt = kernel_bytes(mesh, dom)
+        avg_s = ms / n / 1e3
+        achieved = byt / avg_s
+        roof = dict(kernel=dom, bound="hbm", achieved=round(achieved / 1e9, 1), peak=PEAK_HBM / 1e9, unit="GB/s",
+                    frac=round(achieved / PEAK_HBM, 4), traffic=None, bytes_per_launch=int(byt),
+                    avg_launch_us=round(avg_s * 1e6, 2), launches=n)
+
+    cpu = cpu_baseline(args) if args.cpu else None
+    out = {
+        "metric": "cell-steps/s, mixed-prec vs FP64 GMRES+MG Hall step at 1e-8 rel. residual",
+        "value": round(value, 1),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == 0 else "f64+f32",
+        "data": "synthetic (seeded tetrahedral shell, seeded noisy state)",
+        "config": {"workload": args.workload, "cells": mesh.n_cells, "faces": mesh.n_faces, "edges": mesh.n_edges,
+                   "net": list(w["spec"]["net"]), "precision": args.precision, "hall": True, "gmres_restart": 30,
+                   "rtol": 1e-8, "gmres_iterations_per_step": float(np.mean(its)), "max_rel_residual": float(max(res)),
+                   "setup_s": round(setup_s, 1)},
+        "fp64_step": fp64,
+        "vs_fp64": round(value / fp64["value"], 3) if fp64 else None,
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,10 @@ typedef struct mhdg_config {
     double cuct;
     double fs[2][5];                /* free-stream primitive state (rho, u, v, w, p) per species */
     double u_inf[3], B_imf[3];
+    /* Dirichlet edges of the implicit solve: 0 = imposed inflow edges only (mercury::Solver,
+     * MercuryUns/driver/Solver.cpp:261-262); 1 = inflow and wall edges (mars::MarsSolver,
+     * MarsUns/driver/MarsSolver.cpp:730), for a conducting wall that is the domain boundary. */
+    int32_t wall_dirichlet;
 } mhdg_config;
 
 /* Fills `cfg` with the reference defaults (Config.h:19-96 initialisers, Scales::simple(5/3, 1, 1)). */
@@ -137,6 +141,11 @@ int mhdg_step(mhdg_ctx *ctx, double dt_max, int precision, double *dt_used, int3
 int mhdg_get_derived(mhdg_ctx *ctx, int which, double *out);
 /* Waits for the context's stream. */
 int mhdg_sync(mhdg_ctx *ctx);
+/* Per-kernel timing with HIP events recorded on the context's stream around every kernel the step and
+ * the solves launch: enable = 1 on, 0 off, -1 clear the totals.  The report lists one line per kernel
+ * name: "<name> <launches> <milliseconds>". */
+int mhdg_profile(mhdg_ctx *ctx, int enable);
+int mhdg_profile_report(mhdg_ctx *ctx, char *buf, int buflen);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,7 @@ class Config(C.Structure):
         ("hall_r_min", C.c_double), ("hall_r_max", C.c_double),
         ("ind_rtol", C.c_double), ("ind_max_it", C.c_int32), ("ind_restart", C.c_int32),
         ("cuct", C.c_double), ("fs", (C.c_double * 5) * 2), ("u_inf", C.c_double * 3), ("B_imf", C.c_double * 3),
+        ("wall_dirichlet", C.c_int32),
     ]
 
 
@@ -85,6 +86,7 @@ def lib():
             "mhdg_set_state": [P, P, P, P], "mhdg_get_state": [P, P, P, P],
             "mhdg_set_background": [P, P, P], "mhdg_step": [P, D, C.c_int, P, P, P],
             "mhdg_get_derived": [P, C.c_int, P], "mhdg_sync": [P], "mhdg_device_info": [C.c_char_p, C.c_int],
+            "mhdg_profile": [P, C.c_int], "mhdg_profile_report": [P, C.c_char_p, C.c_int],
         }.items():
             getattr(L, name).argtypes = args
         _lib = L

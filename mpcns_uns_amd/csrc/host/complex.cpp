@@ -1,8 +1,14 @@
 #include "complex.h"
 
 #include <algorithm>
+#include <parallel/algorithm>
 #include <stdexcept>
 #include <unordered_map>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+
+#include <omp.h>
 
 namespace mhdg
 {
@@ -271,6 +277,184 @@ Complex Builder::finish()
         if (f >= 0) cx.face_tag[f] = t.second; // later tags win, as with a map assignment
     }
     cx.build_up();
+    return cx;
+}
+// ------------------------------------------------------------------------------- sort-based builder
+// Same numbering as Builder (faces by first appearance in cell/loop order, edges by first appearance in
+// face/loop-position order of newly created faces), computed with parallel sorts and scans instead of a
+// hash map: the occurrence of each face (edge) is keyed by its sorted node set, sorted by (key,
+// occurrence), and the first occurrence of every key, ranked in occurrence order, is its number.
+namespace
+{
+struct KeyOcc
+{
+    uint64_t key;
+    int64_t occ;
+    bool operator<(const KeyOcc &o) const { return key < o.key || (key == o.key && occ < o.occ); }
+};
+
+// ids[i] = number of the key group of entry i, groups numbered by their first occurrence.
+// Returns the number of groups; first[g] = first occurrence of group g.
+int64_t number_by_first(std::vector<KeyOcc> &v, int64_t n_occ, std::vector<int32_t> &id_of_occ, std::vector<int64_t> &first)
+{
+    __gnu_parallel::sort(v.begin(), v.end());
+    const int64_t n = static_cast<int64_t>(v.size());
+    std::vector<uint8_t> is_first(n_occ, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+        if (i == 0 || v[i].key != v[i - 1].key) is_first[v[i].occ] = 1;
+    // exclusive scan of is_first over occurrences
+    std::vector<int32_t> rank(n_occ);
+    int64_t total = 0;
+    {
+        const int nt = std::max(1, omp_get_max_threads());
+        std::vector<int64_t> part(nt + 1, 0);
+#pragma omp parallel num_threads(nt)
+        {
+            const int t = omp_get_thread_num();
+            const int64_t lo = n_occ * t / nt, hi = n_occ * (t + 1) / nt;
+            int64_t s = 0;
+            for (int64_t i = lo; i < hi; ++i) s += is_first[i];
+            part[t + 1] = s;
+#pragma omp barrier
+#pragma omp single
+            for (int k = 0; k < nt; ++k) part[k + 1] += part[k];
+            int64_t acc = part[t];
+            for (int64_t i = lo; i < hi; ++i) { rank[i] = static_cast<int32_t>(acc); acc += is_first[i]; }
+        }
+        total = part[nt];
+    }
+    first.assign(total, 0);
+    id_of_occ.assign(n_occ, -1);
+    // group start positions
+    std::vector<int64_t> starts;
+    starts.reserve(total + 1);
+    for (int64_t i = 0; i < n; ++i)
+        if (i == 0 || v[i].key != v[i - 1].key) starts.push_back(i);
+    starts.push_back(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < static_cast<int64_t>(starts.size()) - 1; ++g)
+    {
+        const int64_t f0 = v[starts[g]].occ;
+        const int32_t id = rank[f0];
+        first[id] = f0;
+        for (int64_t i = starts[g]; i < starts[g + 1]; ++i) id_of_occ[v[i].occ] = id;
+    }
+    return total;
+}
+} // namespace
+
+Complex build_triangle_complex(int32_t n_nodes, const int32_t *tri_loops, int64_t n_cells, int faces_per_cell, const int32_t *tags,
+                               int64_t n_tags)
+{
+    if (n_nodes >= (1 << 21)) throw std::runtime_error("sort-based builder: node ids must fit in 21 bits");
+    const bool tm = std::getenv("MHDG_SETUP_TIMING") != nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto lap = [&](const char *what) {
+        if (!tm) return;
+        const auto t = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "  complex %-12s %.2fs\n", what, std::chrono::duration<double>(t - t0).count());
+        t0 = t;
+    };
+    const int64_t L = n_cells * faces_per_cell;
+    auto pack3 = [](uint32_t a, uint32_t b, uint32_t c) {
+        uint32_t s[3] = {a, b, c};
+        std::sort(s, s + 3);
+        return (uint64_t(s[0]) << 42) | (uint64_t(s[1]) << 21) | uint64_t(s[2]);
+    };
+    // faces
+    std::vector<KeyOcc> occ(L);
+    std::vector<uint8_t> flipped(L);
+    std::vector<int32_t> canon(3 * L);
+#pragma omp parallel for schedule(static)
+    for (int64_t l = 0; l < L; ++l)
+    {
+        int32_t lp[3] = {tri_loops[3 * l], tri_loops[3 * l + 1], tri_loops[3 * l + 2]};
+        const int sft = static_cast<int>(std::min_element(lp, lp + 3) - lp);
+        std::rotate(lp, lp + sft, lp + 3);
+        const bool fl = lp[1] > lp[2];
+        if (fl) std::swap(lp[1], lp[2]);
+        flipped[l] = fl;
+        canon[3 * l] = lp[0]; canon[3 * l + 1] = lp[1]; canon[3 * l + 2] = lp[2];
+        occ[l] = {pack3(lp[0], lp[1], lp[2]), l};
+    }
+    lap("face keys");
+    std::vector<int32_t> face_of;
+    std::vector<int64_t> face_first;
+    const int64_t nf = number_by_first(occ, L, face_of, face_first);
+    lap("face numbers");
+    Complex cx;
+    cx.nn = n_nodes;
+    cx.nc = static_cast<int32_t>(n_cells);
+    cx.nf = static_cast<int32_t>(nf);
+    cx.c2f.ptr.resize(n_cells + 1);
+    cx.c2f.idx.resize(L);
+    cx.c2f.sgn.resize(L);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c <= n_cells; ++c) cx.c2f.ptr[c] = c * faces_per_cell;
+#pragma omp parallel for schedule(static)
+    for (int64_t l = 0; l < L; ++l) { cx.c2f.idx[l] = face_of[l]; cx.c2f.sgn[l] = flipped[l] ? -1 : 1; }
+    // edges: occurrences (face, loop position) of the canonical loops of the faces
+    std::vector<KeyOcc> eocc(3 * nf);
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < nf; ++f)
+    {
+        const int32_t *lp = &canon[3 * face_first[f]];
+        for (int i = 0; i < 3; ++i)
+        {
+            const uint32_t a = lp[i], b = lp[(i + 1) % 3];
+            const uint32_t lo = std::min(a, b), hi = std::max(a, b);
+            eocc[3 * f + i] = {(uint64_t(lo) << 21) | hi, 3 * f + i};
+        }
+    }
+    std::vector<int32_t> edge_of;
+    std::vector<int64_t> edge_first;
+    lap("edge keys");
+    const int64_t ne = number_by_first(eocc, 3 * nf, edge_of, edge_first);
+    lap("edge numbers");
+    cx.ne = static_cast<int32_t>(ne);
+    cx.f2e.ptr.resize(nf + 1);
+    cx.f2e.idx.resize(3 * nf);
+    cx.f2e.sgn.resize(3 * nf);
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f <= nf; ++f) cx.f2e.ptr[f] = 3 * f;
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < nf; ++f)
+    {
+        const int32_t *lp = &canon[3 * face_first[f]];
+        for (int i = 0; i < 3; ++i)
+        {
+            cx.f2e.idx[3 * f + i] = edge_of[3 * f + i];
+            cx.f2e.sgn[3 * f + i] = lp[i] < lp[(i + 1) % 3] ? 1 : -1;
+        }
+    }
+    cx.e2n.ptr.resize(ne + 1);
+    cx.e2n.idx.resize(2 * ne);
+    cx.e2n.sgn.resize(2 * ne);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; ++e)
+    {
+        const int64_t o = edge_first[e];
+        const int32_t *lp = &canon[3 * face_first[o / 3]];
+        const int i = static_cast<int>(o % 3);
+        const int32_t a = lp[i], b = lp[(i + 1) % 3];
+        cx.e2n.ptr[e] = 2 * e;
+        cx.e2n.idx[2 * e] = std::min(a, b); cx.e2n.sgn[2 * e] = -1;
+        cx.e2n.idx[2 * e + 1] = std::max(a, b); cx.e2n.sgn[2 * e + 1] = 1;
+    }
+    cx.e2n.ptr[ne] = 2 * ne;
+    cx.material.assign(n_cells, 0);
+    cx.face_tag.assign(nf, 0);
+    // tags: look the sorted key up among the first occurrences (occ is sorted by key)
+    for (int64_t t = 0; t < n_tags; ++t)
+    {
+        const uint64_t k = pack3(tags[4 * t], tags[4 * t + 1], tags[4 * t + 2]);
+        const auto it = std::lower_bound(occ.begin(), occ.end(), KeyOcc{k, -1});
+        if (it != occ.end() && it->key == k) cx.face_tag[face_of[it->occ]] = tags[4 * t + 3];
+    }
+    lap("tables");
+    cx.build_up();
+    lap("transposes");
     return cx;
 }
 } // namespace mhdg

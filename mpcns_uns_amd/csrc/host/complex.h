@@ -50,6 +50,11 @@ struct TopoReport
 };
 TopoReport validate(const Complex &cx);
 
+// Same complex as Builder for cells whose faces are all triangles, `faces_per_cell` loops of 3 nodes per
+// cell in tri_loops (nodes < 2^21), built with parallel sorts.  tags: (n0, n1, n2, tag) per tagged face.
+Complex build_triangle_complex(int32_t n_nodes, const int32_t *tri_loops, int64_t n_cells, int faces_per_cell, const int32_t *tags,
+                               int64_t n_tags);
+
 // Builds a Complex from cells given as outward (counter-clockwise seen from outside) node loops.
 // Loops of 3 or 4 nodes (tetrahedra, prisms, hexahedra).
 class Builder

@@ -1,6 +1,9 @@
 #include "geometry.h"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 
 #include "vec3.h"
@@ -36,6 +39,14 @@ inline V3 dual_piece(const Complex &cx, const Geometry &g, int64_t fe, int s, co
 
 Geometry compute_geometry(const Complex &cx, const double *xyz)
 {
+    const bool tm = std::getenv("MHDG_SETUP_TIMING") != nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto lap = [&](const char *what) {
+        if (!tm) return;
+        const auto t = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "  geometry %-12s %.2fs\n", what, std::chrono::duration<double>(t - t0).count());
+        t0 = t;
+    };
     const int32_t ne = cx.ne, nf = cx.nf, nc = cx.nc;
     Geometry g;
     g.edge_vec.resize(3 * size_t(ne)); g.edge_mid.resize(3 * size_t(ne)); g.edge_len.resize(ne);
@@ -78,6 +89,7 @@ Geometry compute_geometry(const Complex &cx, const double *xyz)
         g.face_area_mag[f] = norm(A);
     }
 
+    lap("edges+faces");
     // edge -> incident cells (sorted unique)
     g.ec_ptr.assign(size_t(ne) + 1, 0);
     {
@@ -108,6 +120,7 @@ Geometry compute_geometry(const Complex &cx, const double *xyz)
         }
     }
 
+    lap("ec lists");
     // cell centres (pass 1) and volumes (pass 2): independent per cell
     g.cell_center.resize(3 * size_t(nc)); g.cell_volume.assign(nc, 0.0); g.cell_min_tet.assign(nc, 0.0);
     g.star3.assign(nc, 0.0);
@@ -161,6 +174,7 @@ Geometry compute_geometry(const Complex &cx, const double *xyz)
         g.star3[c] = vol > 0 ? 1.0 / vol : 0.0;
     }
 
+    lap("cells");
     // w_{e,c}: the pieces of cell c at edge e, in (c2f, loop position) order
     g.ec_weight.assign(g.ec_cell.size(), 0.0);
     g.edge_dual_area.assign(3 * size_t(ne), 0.0);
@@ -195,6 +209,7 @@ Geometry compute_geometry(const Complex &cx, const double *xyz)
         g.ec_misoriented[e] = mis;
     }
 
+    lap("w_ec");
     // w_{e,f}: pieces of the cells of f (ascending) through f at e; dual length of faces
     g.ef_weight.assign(cx.e2f.idx.size(), 0.0);
 #pragma omp parallel for schedule(static)
@@ -215,6 +230,7 @@ Geometry compute_geometry(const Complex &cx, const double *xyz)
             g.ef_weight[k] = w;
         }
 
+    lap("w_ef");
     g.star1.assign(ne, 0.0);
     g.star2.assign(nf, 0.0);
 #pragma omp parallel for schedule(static)
@@ -237,6 +253,7 @@ Geometry compute_geometry(const Complex &cx, const double *xyz)
         for (int64_t q = cx.f2c.ptr[f]; q < cx.f2c.ptr[f + 1]; ++q) dl += norm(cf - ld3(g.cell_center.data(), cx.f2c.idx[q]));
         g.star2[f] = g.face_area_mag[f] > 0 ? dl / g.face_area_mag[f] : 0.0;
     }
+    lap("stars");
     return g;
 }
 

@@ -115,8 +115,15 @@ ShellMesh make_shell(const ShellSpec &s)
     return m;
 }
 
-Complex build_tet_complex(int32_t n_nodes, const int32_t *tets, int64_t n_cells, const int32_t *tags, int64_t n_tags)
+Complex build_tet_complex(int32_t n_nodes, const int32_t *tets, int64_t n_cells, const int32_t *tags, int64_t n_tags, bool sorted_builder)
 {
+    if (sorted_builder && n_nodes < (1 << 21))
+    {
+        std::vector<int32_t> loops(12 * size_t(n_cells));
+#pragma omp parallel for schedule(static)
+        for (int64_t c = 0; c < n_cells; ++c) tet_loops(tets + 4 * c, &loops[12 * c]);
+        return build_triangle_complex(n_nodes, loops.data(), n_cells, 4, tags, n_tags);
+    }
     Builder b(n_nodes, n_cells);
     const int32_t lptr[5] = {0, 3, 6, 9, 12};
     int32_t L[12];

@@ -34,6 +34,9 @@ ShellMesh make_shell(const ShellSpec &s);
 // Outward loops of a positively oriented tetrahedron (a, b, c, d): [a c b], [a b d], [a d c], [b c d].
 void tet_loops(const int32_t *t, int32_t loops[12]);
 
-// Complex of a tetrahedral mesh (cells in order, faces/edges by first appearance), tags applied.
-Complex build_tet_complex(int32_t n_nodes, const int32_t *tets, int64_t n_cells, const int32_t *tags, int64_t n_tags);
+// Complex of a tetrahedral mesh (cells in order, faces/edges by first appearance), tags applied.  The
+// sort-based parallel builder is used when `sorted_builder` is set and node ids fit in 21 bits; both
+// builders give identical complexes.
+Complex build_tet_complex(int32_t n_nodes, const int32_t *tets, int64_t n_cells, const int32_t *tags, int64_t n_tags,
+                          bool sorted_builder = true);
 } // namespace mhdg

@@ -16,6 +16,7 @@ struct mhdg_ctx
     mhdg::Params P{};
     mhdg::DevMesh d{};
     int32_t nc = 0, nf = 0, ne = 0, n_heavy = 0;
+    size_t c2f_nnz = 0, ec_nnz = 0;
     void *heavy = nullptr;
     std::vector<double> taper_h;
     int8_t *dir_mercury = nullptr;
@@ -37,6 +38,14 @@ struct mhdg_ctx
     std::vector<void *> owned;
     std::map<int, std::pair<void *, size_t>> staging;
     std::string err;
+    // per-kernel timing with HIP events on the context's stream (enabled by mhdg_profile)
+    bool prof_on = false;
+    struct ProfRec { const char *name; hipEvent_t a, b; };
+    std::vector<ProfRec> prof_pending;
+    std::map<std::string, std::pair<long, double>> prof_acc; // name -> (launches, milliseconds)
+    void prof_begin(const char *name);
+    void prof_end();
+    void prof_flush();
 
     ~mhdg_ctx();
     void build(const mhdg_mesh_desc &m, const mhdg_config &c);
@@ -52,6 +61,7 @@ struct mhdg_ctx
     void apply_op(const double *phi, const double *v, const double *inv, const int8_t *dir, double dt, int mode, int mask, const double *b,
                   double *out);
     int solve64(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
+    void mixed_build();
     int solve_mixed(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
     double step(double dt_max, int precision, int *iters, double *res);
 };

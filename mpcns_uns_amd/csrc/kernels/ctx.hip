@@ -26,6 +26,15 @@ void hip_check(hipError_t e, const char *what)
 inline int nblk(int64_t n, int bs = 256) { return static_cast<int>((n + bs - 1) / bs); }
 } // namespace
 
+// kernel launch on the context stream, bracketed by timing events when profiling is on
+#define LAUNCH(NAME, ...)                                                                                                    \
+    do                                                                                                                       \
+    {                                                                                                                        \
+        prof_begin(NAME);                                                                                                    \
+        hipLaunchKernelGGL(__VA_ARGS__);                                                                                     \
+        prof_end();                                                                                                          \
+    } while (0)
+
 void *mhdg_ctx::alloc(size_t bytes)
 {
     void *p = nullptr;
@@ -72,8 +81,41 @@ void mhdg_ctx::check_launch(const char *what)
     hip_check(hipGetLastError(), what);
 }
 
+void mhdg_ctx::prof_begin(const char *name)
+{
+    if (!prof_on) return;
+    ProfRec r{name, nullptr, nullptr};
+    hip_check(hipEventCreate(&r.a), "event");
+    hip_check(hipEventCreate(&r.b), "event");
+    hip_check(hipEventRecord(r.a, stream), "event");
+    prof_pending.push_back(r);
+}
+void mhdg_ctx::prof_end()
+{
+    if (!prof_on) return;
+    hip_check(hipEventRecord(prof_pending.back().b, stream), "event");
+}
+void mhdg_ctx::prof_flush()
+{
+    if (prof_pending.empty()) return;
+    hip_check(hipStreamSynchronize(stream), "sync");
+    for (auto &r : prof_pending)
+    {
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, r.a, r.b), "elapsed");
+        auto &acc = prof_acc[r.name];
+        acc.first += 1;
+        acc.second += ms;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    prof_pending.clear();
+}
+
 mhdg_ctx::~mhdg_ctx()
 {
+    delete mixed;
+    for (auto &r : prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (void *p : owned) (void)hipFree(p);
     for (auto &kv : staging) if (kv.second.first) (void)hipFree(kv.second.first);
     if (host_pin) (void)hipHostFree(host_pin);
@@ -106,6 +148,9 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
     if (t.max_valence > kHeavyMax) throw std::runtime_error("edge valence exceeds the supported maximum");
     nc = t.nc; nf = t.nf; ne = t.ne;
     n_heavy = static_cast<int32_t>(t.heavy.size());
+    c2f_nnz = t.c2f.size();
+    ec_nnz = t.ec_cell.size();
+    if (c.ind_restart > 60) throw std::runtime_error("ind_restart above 60 is not supported");
     const size_t E = ne, F = nf, C = nc;
     d.nc = nc; d.nf = nf; d.ne = ne;
     d.c2f_ptr = upload(t.c2f_ptr); d.c2f = upload(t.c2f); d.cfs = upload(t.cfs);
@@ -125,7 +170,7 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
     heavy = upload(t.heavy);
     taper_h = t.taper;
     std::vector<int8_t> dir(E);
-    for (size_t e = 0; e < E; ++e) dir[e] = m.edge_mode[e] == 2;
+    for (size_t e = 0; e < E; ++e) dir[e] = c.wall_dirichlet ? m.edge_mode[e] != 0 : m.edge_mode[e] == 2;
     dir_mercury = upload(dir);
 
     auto D = [&](size_t n) { return static_cast<double *>(alloc(n * sizeof(double))); };
@@ -160,16 +205,16 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
 void mhdg_ctx::launch_edge_emf(const double *Efp, const double *u, const double *phi_ind, double cuct, const double *Ein, double *Eout,
                                int bc)
 {
-    hipLaunchKernelGGL(k_edge_emf, dim3(nblk(ne)), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct, kLightMax, Ein, Eout, bc);
+    LAUNCH("edge_emf", k_edge_emf, dim3(nblk(ne)), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct, kLightMax, Ein, Eout, bc);
     if (n_heavy)
-        hipLaunchKernelGGL(k_edge_emf_heavy, dim3(n_heavy), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct,
+        LAUNCH("edge_emf_heavy", k_edge_emf_heavy, dim3(n_heavy), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct,
                            static_cast<const int32_t *>(heavy), Ein, Eout, bc);
     check_launch("edge emf");
 }
 
 double mhdg_ctx::dot(const double *a, const double *b, int op, int slot)
 {
-    hipLaunchKernelGGL(k_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, a, b, ne, rs, scal + slot, op);
+    LAUNCH("dot", k_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, a, b, ne, rs, scal + slot, op);
     check_launch("dot");
     double v;
     d2h(&v, scal + slot, 8);
@@ -184,16 +229,19 @@ void mhdg_ctx::apply_op(const double *phi_in, const double *v, const double *inv
     const double *phi = phi_in;
     if (!phi)
     {
-        hipLaunchKernelGGL(k_d1, dim3(nblk(nf)), dim3(256), 0, stream, d, v, inv, dir, phi_w);
+        LAUNCH("d1", k_d1, dim3(nblk(nf)), dim3(256), 0, stream, d, v, inv, dir, phi_w);
         phi = phi_w;
     }
     if (P.hall)
     {
-        hipLaunchKernelGGL(k_bcell, dim3(nblk(nc)), dim3(256), 0, stream, d, phi, bcell);
-        hipLaunchKernelGGL(k_hall_g, dim3(nblk(nc)), dim3(256), 0, stream, d, bcell, alpha, B0, gcell);
+        LAUNCH("bcell", k_bcell, dim3(nblk(nc)), dim3(256), 0, stream, d, phi, bcell);
+        LAUNCH("hall_g", k_hall_g, dim3(nblk(nc)), dim3(256), 0, stream, d, bcell, alpha, B0, gcell);
     }
-    hipLaunchKernelGGL(k_edge_apply, dim3(nblk(ne)), dim3(256), 0, stream, d, P.resistive, P.hall, phi, gcell, v, inv, dir, dt, mode,
+    LAUNCH("edge_apply", k_edge_apply, dim3(nblk(ne)), dim3(256), 0, stream, d, P.resistive, P.hall, phi, gcell, v, inv, dir, dt, mode,
                        mask, b, out);
+    if (n_heavy)
+        LAUNCH("edge_apply_heavy", k_edge_apply_heavy, dim3(n_heavy), dim3(256), 0, stream, d, P.resistive, P.hall,
+               static_cast<const int32_t *>(heavy), phi, gcell, v, inv, dir, dt, mode, mask, b, out);
     check_launch("apply_op");
 }
 
@@ -204,10 +252,10 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
     const int m = restart;
     const int max_it = cfg.ind_max_it;
     const double rtol = cfg.ind_rtol;
-    hipLaunchKernelGGL(k_solve_init, dim3(nblk(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt, ne,
+    LAUNCH("solve_init", k_solve_init, dim3(nblk(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt, ne,
                        x0, invdiag);
     apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp); // F(Phi^n)
-    hipLaunchKernelGGL(k_add, dim3(nblk(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
+    LAUNCH("axpy", k_add, dim3(nblk(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
     apply_op(nullptr, x0, nullptr, dir, dt, 1, 0, bvec, rvec); // r = b - A x0
     hip_check(hipMemsetAsync(delta, 0, size_t(ne) * 8, stream), "memset");
     const double bb = dot(bvec, bvec, 0, 0);
@@ -218,7 +266,7 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
     std::vector<double> cs(m), sn(m), g(m + 1), yv(m), hc(m + 2);
     while (beta > rtol * bnorm && total < max_it)
     {
-        hipLaunchKernelGGL(k_div_scalar, dim3(nblk(ne)), dim3(256), 0, stream, rvec, scal + 1, ne, V[0]);
+        LAUNCH("scale", k_div_scalar, dim3(nblk(ne)), dim3(256), 0, stream, rvec, scal + 1, ne, V[0]);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int j = 0;
@@ -226,10 +274,10 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
         {
             apply_op(nullptr, V[j], invdiag, dir, dt, 1, 1, nullptr, wvec);
             for (int i = 0; i <= j; ++i)
-                hipLaunchKernelGGL(k_mgs<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, wvec, i > 0 ? V[i - 1] : nullptr, V[i], ne,
+                LAUNCH("mgs", k_mgs<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, wvec, i > 0 ? V[i - 1] : nullptr, V[i], ne,
                                    hcol, i, 0, rs);
-            hipLaunchKernelGGL(k_mgs<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, wvec, V[j], nullptr, ne, hcol, j + 1, 1, rs);
-            hipLaunchKernelGGL(k_scale_div<double>, dim3(nblk(ne)), dim3(256), 0, stream, wvec, hcol + j + 1, ne, V[j + 1]);
+            LAUNCH("mgs", k_mgs<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, wvec, V[j], nullptr, ne, hcol, j + 1, 1, rs);
+            LAUNCH("scale", k_scale_div<double>, dim3(nblk(ne)), dim3(256), 0, stream, wvec, hcol + j + 1, ne, V[j + 1]);
             check_launch("arnoldi");
             d2h(hc.data(), hcol, sizeof(double) * (j + 2));
             for (int i = 0; i <= j + 1; ++i) H[i][j] = hc[i];
@@ -252,13 +300,13 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
             yv[i] = s / H[i][i];
         }
         hip_check(hipMemcpyAsync(ydev, yv.data(), sizeof(double) * std::max(j, 1), hipMemcpyHostToDevice, stream), "H2D");
-        hipLaunchKernelGGL(k_gmres_update, dim3(nblk(ne)), dim3(256), 0, stream, static_cast<const double *const *>(Vptrs), ydev, j,
+        LAUNCH("gmres_update", k_gmres_update, dim3(nblk(ne)), dim3(256), 0, stream, static_cast<const double *const *>(Vptrs), ydev, j,
                            invdiag, dir, ne, delta);
-        hipLaunchKernelGGL(k_add, dim3(nblk(ne)), dim3(256), 0, stream, x0, delta, ne, tmp);
+        LAUNCH("axpy", k_add, dim3(nblk(ne)), dim3(256), 0, stream, x0, delta, ne, tmp);
         apply_op(nullptr, tmp, nullptr, dir, dt, 1, 0, bvec, rvec);
         beta = dot(rvec, rvec, 1, 1);
     }
-    hipLaunchKernelGGL(k_solve_final, dim3(nblk(ne)), dim3(256), 0, stream, x0, delta, Eex, dir, ne, Eout, corr);
+    LAUNCH("solve_final", k_solve_final, dim3(nblk(ne)), dim3(256), 0, stream, x0, delta, Eex, dir, ne, Eout, corr);
     check_launch("solve final");
     have_corr = true;
     if (rel_res) *rel_res = beta / bnorm;
@@ -269,9 +317,9 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
 double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
 {
     const int nb = nblk(nc);
-    hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(256), 0, stream, d, P, U[0], U[1], Phi, Badd, W[0], W[1], Bind, Btot, uplus, alpha,
+    LAUNCH("prepare", k_prepare, dim3(nb), dim3(256), 0, stream, d, P, U[0], U[1], Phi, Badd, W[0], W[1], Bind, Btot, uplus, alpha,
                        dt_part);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, stream, dt_part, nb, dt_max, dt_dev);
+    LAUNCH("dt_final", k_dt_final, dim3(1), dim3(256), 0, stream, dt_part, nb, dt_max, dt_dev);
     double *dth = static_cast<double *>(host_pin);
     hip_check(hipMemcpyAsync(dth, dt_dev, 8, hipMemcpyDeviceToHost, stream), "D2H");
     hipEvent_t ev;
@@ -284,20 +332,20 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     gradsC.p[0] = grad[0]; gradsC.p[1] = grad[1];
     Us.p[0] = U[0]; Us.p[1] = U[1];
     if (P.second_order)
-        hipLaunchKernelGGL(k_grad_current, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, Ws, none, grads, 1, Bind, J);
+        LAUNCH("grad_current", k_grad_current, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, Ws, none, grads, 1, Bind, J);
     else
     {
         hip_check(hipMemsetAsync(grad[0], 0, 15 * size_t(nc) * 8, stream), "memset");
         hip_check(hipMemsetAsync(grad[1], 0, 15 * size_t(nc) * 8, stream), "memset");
-        hipLaunchKernelGGL(k_grad_current, dim3(nb), dim3(256), 0, stream, d, P, 0, 0, Ws, none, grads, 1, Bind, J);
+        LAUNCH("grad_current", k_grad_current, dim3(nb), dim3(256), 0, stream, d, P, 0, 0, Ws, none, grads, 1, Bind, J);
     }
-    hipLaunchKernelGGL(k_face, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 2, 0, Ws, gradsC, none, FA, uplus, Btot, Bind, Phi, PhiAdd, Ef);
+    LAUNCH("face", k_face, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 2, 0, Ws, gradsC, none, FA, uplus, Btot, Bind, Phi, PhiAdd, Ef);
     check_launch("explicit stage");
     hip_check(hipEventSynchronize(ev), "event sync");
     (void)hipEventDestroy(ev);
     const double gdt = dth[0];
     if (!std::isfinite(gdt) || gdt <= 0) throw std::runtime_error("invalid time step");
-    hipLaunchKernelGGL(k_cell_update, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, static_cast<const double *>(FA), none, W[0], W[1], Btot,
+    LAUNCH("cell_update", k_cell_update, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, static_cast<const double *>(FA), none, W[0], W[1], Btot,
                        J, dt_dev, 0.0, 1, 1, Us, noneW);
     launch_edge_emf(Ef, uplus, Phi, P.cuct, nullptr, E_expl, 1);
     *iters = 0;
@@ -312,8 +360,9 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     }
     else
         hip_check(hipMemcpyAsync(E_last, E_expl, size_t(ne) * 8, hipMemcpyDeviceToDevice, stream), "copy");
-    hipLaunchKernelGGL(k_faraday, dim3(nblk(nf)), dim3(256), 0, stream, d, Euse, nullptr, dt_dev, 0.0, Phi);
+    LAUNCH("faraday", k_faraday, dim3(nblk(nf)), dim3(256), 0, stream, d, Euse, nullptr, dt_dev, 0.0, Phi);
     check_launch("faraday");
+    if (prof_on) prof_flush();
     return gdt;
 }
 
@@ -375,6 +424,28 @@ int mhdg_device_info(char *buf, int len)
     return 0;
 }
 int mhdg_sync(mhdg_ctx *c) { MHDG_TRY(c, hip_check(hipStreamSynchronize(c->stream), "sync")) }
+int mhdg_profile(mhdg_ctx *c, int enable)
+{
+    MHDG_TRY(c, {
+        c->prof_flush();
+        if (enable < 0) c->prof_acc.clear();
+        else c->prof_on = enable != 0;
+    })
+}
+int mhdg_profile_report(mhdg_ctx *c, char *buf, int len)
+{
+    MHDG_TRY(c, {
+        c->prof_flush();
+        std::string r;
+        char line[160];
+        for (const auto &kv : c->prof_acc)
+        {
+            std::snprintf(line, sizeof(line), "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+            r += line;
+        }
+        std::snprintf(buf, len, "%s", r.c_str());
+    })
+}
 
 int mhdg_cell_vector(mhdg_ctx *c, const double *flux, double *b)
 {

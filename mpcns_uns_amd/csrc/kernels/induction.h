@@ -21,6 +21,9 @@ __global__ void k_bcell(DevMesh d, const double *phi, double *b);
 __global__ void k_hall_g(DevMesh d, const double *b, const double *alpha, const double *B0, double *g);
 __global__ void k_edge_apply(DevMesh d, int32_t resistive, int32_t hall, const double *phi, const double *g, const double *v,
                              const double *invdiag, const int8_t *dir, double dt, int32_t mode, int32_t mask, const double *bvec, double *out);
+__global__ void k_edge_apply_heavy(DevMesh d, int32_t resistive, int32_t hall, const int32_t *heavy, const double *phi, const double *g,
+                                   const double *v, const double *invdiag, const int8_t *dir, double dt, int32_t mode, int32_t mask,
+                                   const double *bvec, double *out);
 __global__ void k_gmres_update(const double *const *V, const double *y, int32_t j, const double *invdiag, const int8_t *dir, int32_t n,
                                double *delta);
 __global__ void k_solve_init(const double *E_expl, const double *corr, const int8_t *dir, const double *m_e, const double *d2, double dt,

@@ -170,6 +170,7 @@ __global__ void __launch_bounds__(256) k_edge_apply(DevMesh d, int32_t resistive
 {
     const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= d.ne) return;
+    if (d.e2f_ptr[e + 1] - d.e2f_ptr[e] > kLightMax) return; // k_edge_apply_heavy
     double p0 = 0.0, p1 = 0.0;
     if (resistive)
         for (int32_t k = d.e2f_ptr[e]; k < d.e2f_ptr[e + 1]; ++k)
@@ -198,6 +199,50 @@ __global__ void __launch_bounds__(256) k_edge_apply(DevMesh d, int32_t resistive
     }
     if (mask && dir[e]) r = 0.0;
     if (bvec) r = dir[e] ? 0.0 : bvec[e] - r; // residual form r = b - A x
+    out[e] = r;
+}
+
+// k_edge_apply for the high-valence edges, one block per edge: terms in parallel into LDS, sums sequential
+// in the reference's order (faces ascending, cells ascending).
+__global__ void __launch_bounds__(256) k_edge_apply_heavy(DevMesh d, int32_t resistive, int32_t hall, const int32_t *heavy, const double *phi,
+                                                          const double *g, const double *v, const double *invdiag, const int8_t *dir,
+                                                          double dt, int32_t mode, int32_t mask, const double *bvec, double *out)
+{
+    __shared__ double t0[kHeavyMax], t1[2 * kHeavyMax];
+    const int32_t e = heavy[blockIdx.x];
+    const int32_t f0 = d.e2f_ptr[e], nf = d.e2f_ptr[e + 1] - f0;
+    const int32_t c0 = d.ec_ptr[e], ncl = d.ec_ptr[e + 1] - c0;
+    if (resistive)
+        for (int32_t i = threadIdx.x; i < nf; i += blockDim.x)
+        {
+            const int32_t q = d.e2f[f0 + i], f = pid(q);
+            const double val = d.star2[f] * phi[f];
+            t0[i] = psg(q) * val;
+        }
+    if (hall)
+    {
+        const V3 t = L3(d.evec, e);
+        for (int32_t i = threadIdx.x; i < ncl; i += blockDim.x)
+        {
+            const V3 wt = d.ecw[c0 + i] * t;
+            t1[i] = dot(L3(g, d.ec_cell[c0 + i]), wt);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double p0 = 0.0, p1 = 0.0;
+    if (resistive) for (int32_t i = 0; i < nf; ++i) p0 += t0[i];
+    if (hall) for (int32_t i = 0; i < ncl; ++i) p1 += t1[i];
+    const double F = d.m_e[e] * p0 + p1 + 0.0;
+    double r;
+    if (mode == 0) r = F;
+    else
+    {
+        const double x = invdiag ? (dir[e] ? 0.0 : v[e] * invdiag[e]) : v[e];
+        r = x + dt * F;
+    }
+    if (mask && dir[e]) r = 0.0;
+    if (bvec) r = dir[e] ? 0.0 : bvec[e] - r;
     out[e] = r;
 }
 

@@ -1,7 +1,47 @@
 #pragma once
-// Mixed-precision implicit induction solver (FP32 multigrid-preconditioned inner FGMRES inside FP64
-// iterative refinement).
+// Mixed-precision implicit induction solve: the operator A = I + dt F d1 and the Krylov basis in FP32, dot
+// products accumulated in FP64, Hessenberg least squares and Givens rotations in FP64 on the host, and an
+// FP64 iterative-refinement loop whose residual b - A x uses the FP64 operator.
+#include <vector>
+
+#include "induction.h"
+
 namespace mhdg
 {
-struct MixedSolver;
+// FP32 copies of the metric arrays the operator reads
+struct DevMesh32
+{
+    const float *star2, *farea, *cvol, *recon, *ecw, *evec, *m_e, *invdiag;
+    float *alpha, *B0;
+};
+
+constexpr int kDotChunk = 16; // basis vectors per multi-dot pass
+
+__global__ void k_d1_f32(DevMesh d, const float *v, const float *invdiag, const int8_t *dir, float *phi);
+__global__ void k_bcell_f32(DevMesh d, DevMesh32 s, const float *phi, float *b);
+__global__ void k_hall_g_f32(DevMesh d, DevMesh32 s, const float *b, float *g);
+__global__ void k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi, const float *g, const float *v,
+                                 const int8_t *dir, float dt, float *out);
+__global__ void k_edge_apply_f32_heavy(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const int32_t *heavy, const float *phi,
+                                       const float *g, const float *v, const int8_t *dir, float dt, float *out);
+__global__ void k_multidot_f32(const float *const *V, int32_t i0, int32_t nv, const float *w, int32_t n, double *part, int32_t with_ww,
+                               RedScratch rs, double *out);
+__global__ void k_multiaxpy_f32(const float *const *V, int32_t nv, const double *h, float *w, int32_t n, RedScratch rs, double *nrm);
+__global__ void k_to_f32_scaled(const double *x, const double *s, int32_t n, float *out);
+__global__ void k_scale_f32(const float *w, const double *h, int32_t n, float *out);
+__global__ void k_update_mixed(const float *const *V, const double *y, int32_t j, const float *invdiag, const int8_t *dir, int32_t n,
+                               double *delta);
+__global__ void k_f64_to_f32(const double *x, int64_t n, float *out);
+__global__ void k_alpha_f32(const double *alpha, const double *B0, int32_t nc, float *alpha32, float *B032);
+
+struct MixedSolver
+{
+    DevMesh32 s{};
+    float *phi = nullptr, *b = nullptr, *g = nullptr, *w = nullptr;
+    std::vector<float *> V;
+    float **Vptrs = nullptr;
+    double *h = nullptr, *part = nullptr;
+    int m = 30;
+    bool built = false;
+};
 } // namespace mhdg

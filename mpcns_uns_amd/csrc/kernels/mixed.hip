@@ -1,9 +1,477 @@
-// Mixed-precision implicit induction solve.
+// Mixed-precision implicit induction solve (ImplicitInduction::solve's system, Common/induction/
+// ImplicitInduction.cpp:299-390, solved to the same FP64 relative residual):
+//   outer FP64 iterative refinement   r = b - A x (FP64 operator), x += d
+//   inner GMRES(m) on A d = r         FP32 operator and FP32 basis, right Jacobi preconditioner,
+//                                     classical Gram-Schmidt with FP64 accumulation (one multi-dot pass and
+//                                     one multi-axpy pass per step, re-orthogonalised when the DGKS test
+//                                     asks for it), Hessenberg and Givens in FP64 on the host.
+#include <cmath>
 #include <stdexcept>
 
 #include "ctx.h"
 
-int mhdg_ctx::solve_mixed(const double *, const double *, const int8_t *, double, double *, double *)
+namespace mhdg
 {
-    throw std::runtime_error("mixed-precision solve not available in this build");
+namespace
+{
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 f3(const float *p, long i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ __forceinline__ F3 fcross(const F3 &a, const F3 &b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ float fdot(const F3 &a, const F3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float fsg(int32_t v) { return v >= 0 ? 1.0f : -1.0f; }
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+} // namespace
+
+__global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, const float *v, const float *invdiag, const int8_t *dir, float *phi)
+{
+    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= d.nf) return;
+    float s = 0.0f;
+    for (int32_t k = d.f2e_ptr[f]; k < d.f2e_ptr[f + 1]; ++k)
+    {
+        const int32_t q = d.f2e[k], e = pid(q);
+        const float x = dir[e] ? 0.0f : v[e] * invdiag[e];
+        s += fsg(q) * x;
+    }
+    phi[f] = s;
+}
+
+__global__ void __launch_bounds__(256) k_bcell_f32(DevMesh d, DevMesh32 s, const float *phi, float *b)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.nc) return;
+    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+    {
+        const float v = phi[pid(d.c2f[k])];
+        bx += s.recon[3 * size_t(k)] * v; by += s.recon[3 * size_t(k) + 1] * v; bz += s.recon[3 * size_t(k) + 2] * v;
+    }
+    b[3 * size_t(c)] = bx; b[3 * size_t(c) + 1] = by; b[3 * size_t(c) + 2] = bz;
+}
+
+__global__ void __launch_bounds__(256) k_hall_g_f32(DevMesh d, DevMesh32 s, const float *b, float *g)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.nc) return;
+    const float a = s.alpha[c];
+    F3 out{0.0f, 0.0f, 0.0f};
+    if (a != 0.0f)
+    {
+        F3 sum{0.0f, 0.0f, 0.0f};
+        const F3 bc = f3(b, c);
+        for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+        {
+            const int32_t v = d.c2f[k], f = pid(v);
+            F3 bf = bc;
+            const int32_t l = d.fL[f], r = d.fR[f];
+            if (l >= 0 && r >= 0)
+            {
+                const F3 bo = f3(b, l == c ? r : l);
+                bf = {0.5f * (bc.x + bo.x), 0.5f * (bc.y + bo.y), 0.5f * (bc.z + bo.z)};
+            }
+            const F3 A = f3(s.farea, f);
+            const float sg = fsg(v);
+            const F3 c3 = fcross({sg * A.x, sg * A.y, sg * A.z}, bf);
+            sum.x += c3.x; sum.y += c3.y; sum.z += c3.z;
+        }
+        const float iv = 1.0f / s.cvol[c];
+        const F3 J{iv * sum.x, iv * sum.y, iv * sum.z};
+        const F3 q = fcross(J, f3(s.B0, c));
+        out = {a * q.x, a * q.y, a * q.z};
+    }
+    g[3 * size_t(c)] = out.x; g[3 * size_t(c) + 1] = out.y; g[3 * size_t(c) + 2] = out.z;
+}
+
+// out = dir ? 0 : (A P - I) v = (x - v) + dt F(phi) with x = v * invdiag (the shifted operator: same Krylov
+// space as A P, and the O(1) identity part never enters the FP32 vector)
+__global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi,
+                                                        const float *g, const float *v, const int8_t *dir, float dt, float *out)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.ne) return;
+    if (dir[e]) { out[e] = 0.0f; return; }
+    if (d.e2f_ptr[e + 1] - d.e2f_ptr[e] > kLightMax) return; // k_edge_apply_f32_heavy
+    float p0 = 0.0f, p1 = 0.0f;
+    if (resistive)
+        for (int32_t k = d.e2f_ptr[e]; k < d.e2f_ptr[e + 1]; ++k)
+        {
+            const int32_t q = d.e2f[k], f = pid(q);
+            p0 += fsg(q) * (s.star2[f] * phi[f]);
+        }
+    if (hall)
+    {
+        const F3 t = f3(s.evec, e);
+        for (int32_t k = d.ec_ptr[e]; k < d.ec_ptr[e + 1]; ++k)
+        {
+            const float w = s.ecw[k];
+            p1 += w * fdot(f3(g, d.ec_cell[k]), t);
+        }
+    }
+    out[e] = (v[e] * s.invdiag[e] - v[e]) + dt * (s.m_e[e] * p0 + p1);
+}
+
+// The same for the high-valence edges, one block per edge (tree sums: order-free in FP32)
+__global__ void __launch_bounds__(256) k_edge_apply_f32_heavy(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const int32_t *heavy,
+                                                              const float *phi, const float *g, const float *v, const int8_t *dir, float dt,
+                                                              float *out)
+{
+    __shared__ float sh[2][4];
+    const int32_t e = heavy[blockIdx.x];
+    float p0 = 0.0f, p1 = 0.0f;
+    if (resistive)
+        for (int32_t k = d.e2f_ptr[e] + threadIdx.x; k < d.e2f_ptr[e + 1]; k += 256)
+        {
+            const int32_t q = d.e2f[k], f = pid(q);
+            p0 += fsg(q) * (s.star2[f] * phi[f]);
+        }
+    if (hall)
+    {
+        const F3 t = f3(s.evec, e);
+        for (int32_t k = d.ec_ptr[e] + threadIdx.x; k < d.ec_ptr[e + 1]; k += 256) p1 += s.ecw[k] * fdot(f3(g, d.ec_cell[k]), t);
+    }
+    for (int o = 32; o > 0; o >>= 1) { p0 += __shfl_down(p0, o, 64); p1 += __shfl_down(p1, o, 64); }
+    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = p0; sh[1][threadIdx.x >> 6] = p1; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        const float P0 = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3], P1 = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+        out[e] = dir[e] ? 0.0f : (v[e] * s.invdiag[e] - v[e]) + dt * (s.m_e[e] * P0 + P1);
+    }
+}
+
+// h[i0+k] = <w, V_{i0+k}> for k < nv (FP64 accumulation, 4 elements per thread and iteration); with_ww
+// also <w, w> into out[63]
+__global__ void __launch_bounds__(256) k_multidot_f32(const float *const *V, int32_t i0, int32_t nv, const float *w, int32_t n,
+                                                      double *part, int32_t with_ww, RedScratch rs, double *out)
+{
+    __shared__ double sh[4][kDotChunk + 1];
+    __shared__ int last;
+    double acc[kDotChunk + 1];
+#pragma unroll
+    for (int k = 0; k <= kDotChunk; ++k) acc[k] = 0.0;
+    const float *Vk[kDotChunk];
+#pragma unroll
+    for (int k = 0; k < kDotChunk; ++k) Vk[k] = k < nv ? V[i0 + k] : V[i0];
+    const int32_t n4 = n >> 2;
+    const float4 *w4 = reinterpret_cast<const float4 *>(w);
+    for (int32_t q = blockIdx.x * 256 + threadIdx.x; q < n4; q += gridDim.x * 256)
+    {
+        const float4 a = w4[q];
+#pragma unroll
+        for (int k = 0; k < kDotChunk; ++k)
+            if (k < nv)
+            {
+                const float4 b = reinterpret_cast<const float4 *>(Vk[k])[q];
+                acc[k] += (double(a.x) * b.x + double(a.y) * b.y) + (double(a.z) * b.z + double(a.w) * b.w);
+            }
+        acc[kDotChunk] += (double(a.x) * a.x + double(a.y) * a.y) + (double(a.z) * a.z + double(a.w) * a.w);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3))
+    {
+        const int32_t e = 4 * n4 + threadIdx.x;
+        const double a = w[e];
+#pragma unroll
+        for (int k = 0; k < kDotChunk; ++k)
+            if (k < nv) acc[k] += a * Vk[k][e];
+        acc[kDotChunk] += a * a;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k <= kDotChunk; ++k)
+    {
+        const double sm = wave_sum(acc[k]);
+        if (lane == 0) sh[wv][k] = sm;
+    }
+    __syncthreads();
+    if (threadIdx.x <= kDotChunk)
+    {
+        const int k = threadIdx.x;
+        part[size_t(blockIdx.x) * (kDotChunk + 1) + k] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(rs.ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double t[kDotChunk + 1];
+#pragma unroll
+    for (int k = 0; k <= kDotChunk; ++k) t[k] = 0.0;
+    for (uint32_t bl = threadIdx.x; bl < gridDim.x; bl += 256)
+#pragma unroll
+        for (int k = 0; k <= kDotChunk; ++k)
+            t[k] += __hip_atomic_load(&part[size_t(bl) * (kDotChunk + 1) + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int k = 0; k <= kDotChunk; ++k)
+    {
+        const double s2 = wave_sum(t[k]);
+        if (lane == 0) sh[wv][k] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x <= kDotChunk)
+    {
+        const int k = threadIdx.x;
+        const double s2 = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
+        if (k < nv) out[i0 + k] = s2;
+        if (k == kDotChunk && with_ww) out[63] = s2;
+    }
+    if (threadIdx.x == 0) *rs.ticket = 0u;
+}
+
+// w -= sum_{i < nv} h_i V_i (FP64 arithmetic, FP32 storage, 4 elements per thread); nrm[0] = |w| after
+__global__ void __launch_bounds__(256) k_multiaxpy_f32(const float *const *V, int32_t nv, const double *h, float *w, int32_t n,
+                                                       RedScratch rs, double *nrm)
+{
+    __shared__ double sh[4];
+    __shared__ int last;
+    __shared__ double hs[64];
+    __shared__ const float *vp[64];
+    if (threadIdx.x < nv) { hs[threadIdx.x] = h[threadIdx.x]; vp[threadIdx.x] = V[threadIdx.x]; }
+    __syncthreads();
+    double acc = 0.0;
+    const int32_t n4 = n >> 2;
+    float4 *w4 = reinterpret_cast<float4 *>(w);
+    for (int32_t q = blockIdx.x * 256 + threadIdx.x; q < n4; q += gridDim.x * 256)
+    {
+        const float4 a = w4[q];
+        double x = a.x, y = a.y, z = a.z, u = a.w;
+        for (int32_t i = 0; i < nv; ++i)
+        {
+            const float4 b = reinterpret_cast<const float4 *>(vp[i])[q];
+            const double hi = hs[i];
+            x -= hi * b.x; y -= hi * b.y; z -= hi * b.z; u -= hi * b.w;
+        }
+        const float4 r = make_float4(float(x), float(y), float(z), float(u));
+        w4[q] = r;
+        acc += (double(r.x) * r.x + double(r.y) * r.y) + (double(r.z) * r.z + double(r.w) * r.w);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3))
+    {
+        const int32_t e = 4 * n4 + threadIdx.x;
+        double we = w[e];
+        for (int32_t i = 0; i < nv; ++i) we -= hs[i] * static_cast<double>(vp[i][e]);
+        const float wf = static_cast<float>(we);
+        w[e] = wf;
+        acc += double(wf) * wf;
+    }
+    const double s = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        rs.partial[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        __threadfence();
+        last = atomicAdd(rs.ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double t = 0.0;
+    for (uint32_t bl = threadIdx.x; bl < gridDim.x; bl += 256)
+        t += __hip_atomic_load(&rs.partial[bl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double s2 = wave_sum(t);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        nrm[0] = sqrt(((sh[0] + sh[1]) + sh[2]) + sh[3]);
+        *rs.ticket = 0u;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_to_f32_scaled(const double *x, const double *s, int32_t n, float *out)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = static_cast<float>(x[e] / s[0]);
+}
+__global__ void __launch_bounds__(256) k_scale_f32(const float *w, const double *h, int32_t n, float *out)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const double hv = h[0];
+    if (e < n && hv > 1e-300) out[e] = static_cast<float>(static_cast<double>(w[e]) / hv);
+}
+__global__ void __launch_bounds__(256) k_update_mixed(const float *const *V, const double *y, int32_t j, const float *invdiag,
+                                                      const int8_t *dir, int32_t n, double *delta)
+{
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n || dir[e]) return;
+    double s = 0.0;
+    for (int32_t i = 0; i < j; ++i) s += y[i] * static_cast<double>(V[i][e]);
+    delta[e] += s * static_cast<double>(invdiag[e]);
+}
+__global__ void __launch_bounds__(256) k_f64_to_f32(const double *x, int64_t n, float *out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = static_cast<float>(x[i]);
+}
+__global__ void __launch_bounds__(256) k_alpha_f32(const double *alpha, const double *B0, int32_t nc, float *a32, float *b32)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    a32[c] = static_cast<float>(alpha[c]);
+    for (int k = 0; k < 3; ++k) b32[3 * size_t(c) + k] = static_cast<float>(B0[3 * size_t(c) + k]);
+}
+} // namespace mhdg
+
+using namespace mhdg;
+
+namespace
+{
+inline int nb(int64_t n) { return static_cast<int>((n + 255) / 256); }
+void chk(hipError_t e, const char *w)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(w) + ": " + hipGetErrorString(e));
+}
+} // namespace
+
+#define LAUNCH(NAME, ...)                                                                                                    \
+    do                                                                                                                       \
+    {                                                                                                                        \
+        prof_begin(NAME);                                                                                                    \
+        hipLaunchKernelGGL(__VA_ARGS__);                                                                                     \
+        prof_end();                                                                                                          \
+    } while (0)
+
+void mhdg_ctx::mixed_build()
+{
+    if (!mixed) mixed = new MixedSolver();
+    MixedSolver &M = *mixed;
+    if (M.built) return;
+    const size_t E = ne, F = nf, C = nc;
+    auto f32 = [&](const double *src, size_t n) {
+        float *p = static_cast<float *>(alloc(n * sizeof(float)));
+        hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(n)), dim3(256), 0, stream, src, int64_t(n), p);
+        return p;
+    };
+    const size_t nc2f = c2f_nnz, nec = ec_nnz;
+    M.s.star2 = f32(d.star2, F);
+    M.s.farea = f32(d.farea, 3 * F);
+    M.s.cvol = f32(d.cvol, C);
+    M.s.recon = f32(d.recon, 3 * nc2f);
+    M.s.ecw = f32(d.ecw, nec);
+    M.s.evec = f32(d.evec, 3 * E);
+    M.s.m_e = f32(d.m_e, E);
+    M.s.invdiag = static_cast<float *>(alloc(E * sizeof(float)));
+    M.s.alpha = static_cast<float *>(alloc(C * sizeof(float)));
+    M.s.B0 = static_cast<float *>(alloc(3 * C * sizeof(float)));
+    M.phi = static_cast<float *>(alloc(F * sizeof(float)));
+    M.b = static_cast<float *>(alloc(3 * C * sizeof(float)));
+    M.g = static_cast<float *>(alloc(3 * C * sizeof(float)));
+    M.w = static_cast<float *>(alloc(E * sizeof(float)));
+    M.m = restart;
+    M.V.resize(M.m + 1);
+    for (auto &v : M.V) v = static_cast<float *>(alloc(E * sizeof(float)));
+    M.Vptrs = static_cast<float **>(alloc(sizeof(float *) * M.V.size()));
+    chk(hipMemcpy(M.Vptrs, M.V.data(), sizeof(float *) * M.V.size(), hipMemcpyHostToDevice), "upload");
+    M.h = static_cast<double *>(alloc(64 * sizeof(double)));
+    M.part = static_cast<double *>(alloc(size_t(kRedBlocks) * (kDotChunk + 1) * sizeof(double)));
+    chk(hipStreamSynchronize(stream), "sync");
+    M.built = true;
+}
+
+int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *dir, double dt, double *Eout, double *rel_res)
+{
+    mixed_build();
+    MixedSolver &M = *mixed;
+    const int m = M.m;
+    const int max_it = cfg.ind_max_it;
+    const double rtol = cfg.ind_rtol;
+    const double inner_rtol = 1e-4;
+    // frozen state and Jacobi diagonal in FP32
+    LAUNCH("alpha_f32", k_alpha_f32, dim3(nb(nc)), dim3(256), 0, stream, alpha, B0, nc, M.s.alpha, M.s.B0);
+    LAUNCH("solve_init", k_solve_init, dim3(nb(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt, ne, x0,
+           invdiag);
+    LAUNCH("f64_to_f32", k_f64_to_f32, dim3(nb(ne)), dim3(256), 0, stream, invdiag, int64_t(ne), const_cast<float *>(M.s.invdiag));
+    apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp);
+    LAUNCH("axpy", k_add, dim3(nb(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
+    chk(hipMemsetAsync(delta, 0, size_t(ne) * 8, stream), "memset");
+    const double bb = dot(bvec, bvec, 0, 0);
+    const double bnorm = std::sqrt(std::max(bb, 1e-300));
+    const float dtf = static_cast<float>(dt);
+    int total = 0;
+    double beta = 0;
+    std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
+    std::vector<double> cs(m), sn(m), g(m + 1), yv(m), hc(64), c2(64);
+    const float *const *Vp = M.Vptrs;
+    for (int outer = 0; outer < 200; ++outer)
+    {
+        // FP64 residual of x0 + delta
+        LAUNCH("axpy", k_add, dim3(nb(ne)), dim3(256), 0, stream, x0, delta, ne, tmp);
+        apply_op(nullptr, tmp, nullptr, dir, dt, 1, 0, bvec, rvec);
+        beta = dot(rvec, rvec, 1, 1);
+        if (!(beta > rtol * bnorm) || total >= max_it) break;
+        LAUNCH("scale", k_to_f32_scaled, dim3(nb(ne)), dim3(256), 0, stream, rvec, scal + 1, ne, M.V[0]);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        const double stop = std::max(inner_rtol * beta, 0.25 * rtol * bnorm);
+        int j = 0;
+        for (; j < m && total < max_it; ++j, ++total)
+        {
+            // w = A P V_j in FP32
+            LAUNCH("d1_f32", k_d1_f32, dim3(nb(nf)), dim3(256), 0, stream, d, M.V[j], M.s.invdiag, dir, M.phi);
+            if (P.hall)
+            {
+                LAUNCH("bcell_f32", k_bcell_f32, dim3(nb(nc)), dim3(256), 0, stream, d, M.s, M.phi, M.b);
+                LAUNCH("hall_g_f32", k_hall_g_f32, dim3(nb(nc)), dim3(256), 0, stream, d, M.s, M.b, M.g);
+            }
+            LAUNCH("edge_apply_f32", k_edge_apply_f32, dim3(nb(ne)), dim3(256), 0, stream, d, M.s, P.resistive, P.hall, M.phi, M.g, M.V[j],
+                   dir, dtf, M.w);
+            if (n_heavy)
+                LAUNCH("edge_apply_f32_heavy", k_edge_apply_f32_heavy, dim3(n_heavy), dim3(256), 0, stream, d, M.s, P.resistive, P.hall,
+                       static_cast<const int32_t *>(heavy), M.phi, M.g, M.V[j], dir, dtf, M.w);
+            // classical Gram-Schmidt against V_0..V_j
+            for (int i0 = 0; i0 <= j; i0 += kDotChunk)
+                LAUNCH("multidot", k_multidot_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, i0, std::min(kDotChunk, j + 1 - i0), M.w, ne,
+                       M.part, i0 == 0 ? 1 : 0, rs, M.h);
+            LAUNCH("multiaxpy", k_multiaxpy_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, j + 1, M.h, M.w, ne, rs, M.h + j + 1);
+            d2h(hc.data(), M.h, sizeof(double) * 64);
+            double hn = hc[j + 1];
+            if (hn < 0.7071 * std::sqrt(std::max(hc[63], 0.0)))
+            {
+                // DGKS: one more projection
+                for (int i0 = 0; i0 <= j; i0 += kDotChunk)
+                    LAUNCH("multidot", k_multidot_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, i0, std::min(kDotChunk, j + 1 - i0), M.w,
+                           ne, M.part, 0, rs, M.h);
+                LAUNCH("multiaxpy", k_multiaxpy_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, j + 1, M.h, M.w, ne, rs, M.h + j + 1);
+                d2h(c2.data(), M.h, sizeof(double) * 64);
+                for (int i = 0; i <= j; ++i) hc[i] += c2[i];
+                hn = c2[j + 1];
+                chk(hipMemcpyAsync(M.h + j + 1, &hn, 8, hipMemcpyHostToDevice, stream), "H2D");
+            }
+            LAUNCH("scale", k_scale_f32, dim3(nb(ne)), dim3(256), 0, stream, M.w, M.h + j + 1, ne, M.V[j + 1]);
+            for (int i = 0; i <= j; ++i) H[i][j] = hc[i];
+            H[j][j] += 1.0; // Arnoldi ran on A P - I
+            H[j + 1][j] = hn;
+            for (int i = 0; i < j; ++i)
+            {
+                const double tt = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
+                H[i + 1][j] = -sn[i] * H[i][j] + cs[i] * H[i + 1][j];
+                H[i][j] = tt;
+            }
+            const double den = std::hypot(H[j][j], H[j + 1][j]);
+            cs[j] = den > 0 ? H[j][j] / den : 1.0; sn[j] = den > 0 ? H[j + 1][j] / den : 0.0;
+            H[j][j] = den; H[j + 1][j] = 0.0;
+            g[j + 1] = -sn[j] * g[j]; g[j] = cs[j] * g[j];
+            if (std::abs(g[j + 1]) <= stop || !(hn > 1e-300)) { ++j; ++total; break; }
+        }
+        for (int i = j - 1; i >= 0; --i)
+        {
+            double s = g[i];
+            for (int k = i + 1; k < j; ++k) s -= H[i][k] * yv[k];
+            yv[i] = s / H[i][i];
+        }
+        chk(hipMemcpyAsync(ydev, yv.data(), sizeof(double) * std::max(j, 1), hipMemcpyHostToDevice, stream), "H2D");
+        LAUNCH("update_mixed", k_update_mixed, dim3(nb(ne)), dim3(256), 0, stream, Vp, ydev, j, M.s.invdiag, dir, ne, delta);
+    }
+    LAUNCH("solve_final", k_solve_final, dim3(nb(ne)), dim3(256), 0, stream, x0, delta, Eex, dir, ne, Eout, corr);
+    check_launch("mixed solve");
+    have_corr = true;
+    if (rel_res) *rel_res = beta / bnorm;
+    return total;
 }

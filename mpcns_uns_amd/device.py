@@ -134,3 +134,17 @@ class Device:
 
     def sync(self):
         self._ok(lib().mhdg_sync(self._h))
+
+    def profile(self, enable=True):
+        """Per-kernel HIP-event timing on the context stream (True on, False off, None clear)."""
+        self._ok(lib().mhdg_profile(self._h, -1 if enable is None else int(bool(enable))))
+
+    def profile_report(self):
+        """{kernel name: (launches, total milliseconds)} accumulated while profiling was on."""
+        buf = C.create_string_buffer(1 << 16)
+        self._ok(lib().mhdg_profile_report(self._h, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out

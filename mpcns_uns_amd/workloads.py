@@ -4,6 +4,9 @@ shell_60k      FP64 ideal two-ion MHD, Hall off, one explicit stage; 24x48 net, 
 shell_2m_hall  Hall + uniform resistivity, GMRES(30) to 1e-8, all FP64; 128x256 net, 10 layers (1.95M)
 shell_8m_mixed Hall + resistivity, mixed-precision GMRES + multigrid; 256x512 net, 10 layers (7.83M)
 
+Hall runs hold the EMF of the conducting wall at zero in the implicit solve as well (Dirichlet wall edges,
+the MarsSolver convention): with free wall edges (the Mercury convention, whose wall is a solid-planet
+interface) the semi-implicit Hall step grows without bound at the wall next to the poles of this mesh.
 Common physics: rho_H = 1, rho_Na = 1e-3, v = (-1, 0, 0), p_H = 0.1 with Na+ at the same temperature
 (p_Na = 1e-4); centred dipole of equatorial surface strength 0.2 (m = -0.2 z); IMF (0, 0, -0.05); wall at
 r = 1, far field at r = 4 (inflow on the day side); gamma = 5/3, inver_MA2 = 1, M_Na/M_H = 23; CFL 0.4,
@@ -32,7 +35,8 @@ def physics_config(hall, **kw):
                 ind_rtol=1e-8, ind_max_it=2000, ind_restart=30)
     if hall:
         base.update(hall=1, hall_coef=0.5, hall_r_min=-100.0, hall_r_max=-99.0,
-                    resistive=1, inver_Rem=1e-3, eta_r_inner=-100.0, eta_r_outer=100.0, eta_width=1.0)
+                    resistive=1, inver_Rem=1e-3, eta_r_inner=-100.0, eta_r_outer=100.0, eta_width=1.0,
+                    wall_dirichlet=1)
     base.update(kw)
     return default_config(**base)
 

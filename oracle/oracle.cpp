@@ -931,7 +931,7 @@ double step(orc_ctx &o, double dt_max, int *iters, double *res)
         for (size_t i = 0; i < o.B0_.size(); ++i) o.B0_[i] = o.Btot[i];
         for (int32_t i = 0; i < nc; ++i) o.alpha_[i] = (o.hall && ne_eff[i] > 0) ? o.taper_[i] * c.hall_coef / ne_eff[i] : 0.0;
         std::vector<int8_t> dir(ne);
-        for (int32_t e = 0; e < ne; ++e) dir[e] = o.emode[e] == 2;
+        for (int32_t e = 0; e < ne; ++e) dir[e] = c.wall_dirichlet ? o.emode[e] != 0 : o.emode[e] == 2;
         *iters = ind_solve(o, o.Phi.data(), o.E.data(), dir.data(), gdt, Eout.data(), res);
     }
     faraday(o, Eout.data(), nullptr, o.Phi.data(), gdt);

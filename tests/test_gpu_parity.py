@@ -145,3 +145,40 @@ def test_step_hall(small_hall):
             assert maxrel(a, b) < 1e-12
         _, rel, _ = dev.div_b(sd[2], small_hall["background"][0])
         assert rel < 1e-12
+
+
+def test_mixed_solve_matches_fp64(pair):
+    """Mixed precision (FP32 inner GMRES, FP64 refinement) reaches the same FP64 relative residual."""
+    dev, orc, w, f = pair
+    rng = np.random.default_rng(8)
+    ne_eff = np.sqrt((f["W_H"][0::5] + f["W_Na"][0::5] / 23.0) ** 2 + 1e-4)
+    dev.ind_set_frozen(f["B_tot"], ne_eff)
+    E_expl = rng.normal(size=dev.ne) * 1e-3
+    dirichlet = (w["mesh"].array("edge_mode") != 0).astype(np.int8)
+    from mpcns_uns_amd import Device
+
+    d64 = Device(w["mesh"], w["cfg"])
+    d64.ind_set_frozen(f["B_tot"], ne_eff)
+    E64, it64, r64 = d64.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, precision=0)
+    E32, it32, r32 = dev.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, precision=1)
+    assert r64 < 1e-8 and r32 < 1e-8
+    # both solutions are within the solver tolerance of the exact one
+    assert maxrel(E32, E64) < 1e-6
+
+
+def test_step_mixed_vs_fp64(small_hall):
+    from mpcns_uns_amd import Device
+
+    w = small_hall
+    a, b = Device(w["mesh"], w["cfg"]), Device(w["mesh"], w["cfg"])
+    for x in (a, b):
+        x.set_background(*w["background"])
+        x.set_state(*w["state"])
+    for _ in range(3):
+        dta, _, ra = a.step(precision=0)
+        dtb, _, rb = b.step(precision=1)
+        assert ra < 1e-8 and rb < 1e-8
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert maxrel(y, x) < 1e-8
+    _, rel, _ = b.div_b(b.get_state()[2], w["background"][0])
+    assert rel < 1e-12

@@ -1,0 +1,126 @@
+"""The oracle and the host mesh layer against the compiled reference (oracle/_ref), bit for bit.
+
+The reference library is built from the MPCNS_Uns sources by `make -C oracle ref` where those sources are
+present; these tests skip otherwise (the GPU box has the built library but the oracle is also pinned by
+tests/test_oracle_known_answers.py, which needs no reference).
+"""
+import numpy as np
+import pytest
+
+from meshes import box_mesh
+
+pytestmark = pytest.mark.skipif(not __import__("oracle.oracle", fromlist=["have_ref"]).have_ref(),
+                                reason="reference library oracle/_ref not built")
+
+GEOMETRY = ("edge_vec", "edge_mid", "edge_len", "face_area", "face_center", "face_area_mag", "cell_center", "cell_volume",
+            "edge_dual_area", "ec_ptr", "ec_cell", "ec_weight", "ef_weight", "star1", "star2", "star3")
+
+
+def _shell(hall, wall_dirichlet=0):
+    from mpcns_uns_amd import workloads
+    from oracle.oracle import reference_for_mesh
+
+    w = workloads.build(net=(12, 48, 3), hall=hall, wall_dirichlet=wall_dirichlet)
+    ref = reference_for_mesh(w["mesh"], w["cfg"], dipole_m=workloads.DIPOLE_M)
+    return w, ref
+
+
+@pytest.fixture(scope="module")
+def shell():
+    return _shell(True)
+
+
+def test_complex_and_geometry_bitwise(shell):
+    w, ref = shell
+    m = w["mesh"]
+    assert ref.validate()["ok"] and ref.validate()["euler"] == 2
+    for tb in ("e2n", "f2e", "c2f", "e2f", "f2c"):
+        p, i, s = ref.incidence(tb)
+        assert np.array_equal(p, m.array(tb + ".ptr")) and np.array_equal(i, m.array(tb + ".idx"))
+        assert np.array_equal(s, m.array(tb + ".sgn"))
+    assert np.array_equal(ref.face_tags(), m.array("face_tag"))
+    for g in GEOMETRY:
+        assert np.array_equal(ref.geometry(g), m.array(g)), g
+    fk, em = ref.boundary()
+    assert np.array_equal(fk, m.array("face_kind")) and np.array_equal(em, m.array("edge_mode"))
+    PA, BA = ref.background()
+    assert np.array_equal(PA, w["background"][0]) and np.array_equal(BA, w["background"][1])
+
+
+def test_hex_box_complex_and_geometry_bitwise():
+    from mpcns_uns_amd import default_config
+    from oracle.oracle import Reference
+
+    m = box_mesh(4, 3, 2, hi=(2.0, 1.5, 1.0))
+    xyz, cells, tags = m._cells
+    ref = Reference(xyz, cells, default_config(), tags=tags, tagset=1)
+    for tb in ("e2n", "f2e", "c2f", "e2f", "f2c"):
+        p, i, s = ref.incidence(tb)
+        assert np.array_equal(p, m.array(tb + ".ptr")) and np.array_equal(i, m.array(tb + ".idx"))
+        assert np.array_equal(s, m.array(tb + ".sgn"))
+    for g in GEOMETRY:
+        assert np.array_equal(ref.geometry(g), m.array(g)), g
+
+
+def test_operations_bitwise(shell):
+    from oracle.oracle import Oracle
+
+    w, ref = shell
+    m, cfg = w["mesh"], w["cfg"]
+    orc = Oracle(m, cfg)
+    orc.set_background(*w["background"])
+    orc.set_state(*w["state"])
+    orc.step()
+    f = {n: orc.derived(n) for n in ("W_H", "W_Na", "B_ind", "B_tot", "J", "u_plus")}
+    Phi = w["state"][2]
+    assert np.array_equal(orc.cell_vector(Phi), ref.cell_vector(Phi))
+    assert np.array_equal(orc.cell_current(f["B_ind"]), ref.cell_current(f["B_ind"]))
+    for s, W in ((0, f["W_H"]), (1, f["W_Na"])):
+        fs = np.array(cfg.fs[s][:])
+        for limit in (True, False):
+            g = ref.limited_gradients(W, s, fs, limit)
+            assert np.array_equal(orc.limited_gradients(W, s, fs, limit), g)
+        assert np.array_equal(orc.flux_rhs(s, fs, W, g), ref.flux_rhs(s, fs, W, g))
+    rng = np.random.default_rng(4)
+    u, Bt, Bi = rng.uniform(-1, 1, 3 * m.n_cells), rng.normal(size=3 * m.n_cells), rng.normal(size=3 * m.n_cells)
+    pt, pi = rng.normal(size=m.n_faces), rng.normal(size=m.n_faces)
+    assert np.array_equal(orc.ideal_emf(u, Bt, Bi, pt, pi, 0.5), ref.ideal_emf(u, Bt, Bi, pt, pi, 0.5))
+    E = rng.normal(size=m.n_edges)
+    assert np.array_equal(orc.faraday(E, Phi, 0.3), ref.faraday(E, Phi, 0.3))
+    assert orc.div_b(orc.faraday(E, Phi, 0.3))[:2] == ref.div_b(orc.faraday(E, Phi, 0.3))[:2]
+    assert orc.local_dt(f["W_H"], f["W_Na"], f["B_tot"]) == ref.local_dt(f["W_H"], f["W_Na"], f["B_tot"])
+    rhs = rng.normal(size=5 * m.n_cells)
+    for s in (0, 1):
+        assert np.array_equal(orc.add_sources(s, f["W_H"], f["W_Na"], f["B_tot"], f["J"], 1e-3, rhs),
+                              ref.add_sources(s, f["W_H"], f["W_Na"], f["u_plus"], f["B_tot"], f["J"], 1e-3, rhs))
+    ne_eff = np.sqrt((f["W_H"][0::5] + f["W_Na"][0::5] / 23.0) ** 2 + 1e-4)
+    orc = Oracle(m, cfg)  # no warm-start correction carried over from the step above
+    for x in (orc, ref):
+        x.ind_set_frozen(f["B_tot"], ne_eff)
+    phi = rng.normal(size=m.n_faces)
+    assert np.array_equal(orc.ind_apply_F(phi), ref.ind_apply_F(phi))
+    dirichlet = (m.array("edge_mode") == 2).astype(np.int8)
+    E_expl = rng.normal(size=m.n_edges) * 1e-3
+    for _ in range(2):  # the second solve is warm-started by both
+        Eo, ito, ro = orc.ind_solve(Phi, E_expl, dirichlet, 2e-3)
+        Er, itr, rr = ref.ind_solve(Phi, E_expl, dirichlet, 2e-3)
+        assert ito == itr and ro == rr and np.array_equal(Eo, Er)
+
+
+@pytest.mark.parametrize("hall", [False, True])
+def test_full_steps_bitwise(hall):
+    """mercury::Solver::step_first_order of the reference vs the oracle: identical states after 3 steps."""
+    from oracle.oracle import Oracle
+
+    w, ref = _shell(hall)
+    orc = Oracle(w["mesh"], w["cfg"])
+    orc.set_background(*w["background"])
+    orc.set_state(*w["state"])
+    ref.solver_init()
+    ref.set_state(*w["state"])
+    for _ in range(3):
+        assert ref.step() == orc.step()[0]
+        for a, b in zip(ref.get_state(), orc.get_state()):
+            assert np.array_equal(a, b)
+        for n in ("W_H", "W_Na", "B_ind", "B_tot", "J"):
+            assert np.array_equal(ref.derived(n), orc.derived(n))

@@ -48,6 +48,11 @@ def kernel_bytes(mesh, name, precision_bytes=8):
         "edge_emf": 3 * F * 8 + 3 * C * 8 + F * 8 + 4 * (E + 1) + 4 * nf2e + 8 * nf2e + 4 * (E + 1) + 4 * nec + 3 * F * 8 * 2 + F * 8
                     + 3 * E * 8 * 2 + E * 8 + E * 8 + E,
         "faraday": E * 8 + F * 8 * 2 + 4 * (F + 1) + 4 * nf2e + 8 * F,
+        # FP32 operator of the mixed solve (int32 incidence, FP32 values)
+        "d1_f32": 4 * (F + 1) + 4 * nf2e + 4 * E + 4 * E + E + 4 * F,
+        "bcell_f32": 4 * (C + 1) + 4 * nc2f + 12 * nc2f + 4 * F + 12 * C,
+        "hall_g_f32": 4 * C + 12 * C + 4 * (C + 1) + 4 * nc2f + 8 * F + 12 * F + 4 * C + 12 * C + 12 * C,
+        "edge_apply_f32": E + 4 * (E + 1) + 4 * nf2e + 4 * F + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 4,
     }
     return table.get(name)
 
@@ -85,7 +90,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     ap.add_argument("--cpu-net", type=int, nargs=3, default=(96, 192, 10))
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a device-to-device copy of GIB GiB five times (scale of the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import torch
+
+        n = int(args.copy_calibration * 2 ** 30) // 4
+        x = torch.ones(n, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "repeats": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the path shards by mesh partition (see DESIGN.md), "
                                    "multi-GPU runs are not part of it", "n_gpus": args.gpus}))

@@ -95,13 +95,8 @@ __device__ __forceinline__ NumInfo numinfo(const Params &P, double rhoH, double 
     return n;
 }
 
-// Logical block index remapped so that the blocks one XCD receives (dispatch is round-robin over the
-// 8 XCDs) cover a contiguous index range: neighbours gathered by adjacent blocks share one L2.
-__device__ __forceinline__ int32_t xcd_block()
-{
-    const uint32_t b = blockIdx.x, n = gridDim.x;
-    const uint32_t per = n / 8u, x = b % 8u, q = b / 8u;
-    if (q >= per) return static_cast<int32_t>(b); // tail blocks (n not a multiple of 8) keep their index
-    return static_cast<int32_t>(x * per + q);
-}
+// Global thread index of a one-entity-per-thread launch.  (An XCD-contiguous block order, which keeps the
+// blocks of one XCD on one index range, was measured on the gather kernels: no gain, the 256 MB MALL already
+// serves the data the eight L2s share.)
+__device__ __forceinline__ int32_t gid() { return static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); }
 } // namespace mhdg

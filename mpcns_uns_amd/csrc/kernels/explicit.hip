@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) k_prepare(DevMesh d, Params P, const doub
                                                  double *uplus, double *alpha, double *dt_part)
 {
     __shared__ double sh[256];
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = gid();
     double dtc = __builtin_inf();
     if (c < d.nc)
     {
@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(256) k_dt_final(const double *part, int32_t n,
 __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpC Ws, SpC ghosts, SpW grads,
                                                       int32_t limit, const double *Bind, double *J)
 {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = gid();
     if (c >= d.nc) return;
     const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1];
     if (J)
@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(256) k_face(DevMesh d, Params P, int32_t ns, i
                                               const double *uplus, const double *Btot, const double *Bind, const double *Phi,
                                               const double *PhiAdd, double *Ef)
 {
-    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t f = gid();
     if (f >= d.nf) return;
     const int32_t l = d.fL[f], r = d.fR[f];
     const double A = d.famag[f];
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) k_cell_update(DevMesh d, Params P, int32_
                                                      const double *dt_dev, double dt_val, int32_t sources, int32_t update, SpW Us,
                                                      SpW rhs_out)
 {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = gid();
     if (c >= d.nc) return;
     const bool fluid = d.mat[c] == 0;
     const double dt = dt_dev ? dt_dev[0] : dt_val;
@@ -458,6 +458,21 @@ __device__ __forceinline__ void plane_basis(const V3 &t, V3 &e1, V3 &e2)
     e1 *= 1.0 / norm(e1);
     e2 = cross(t, e1);
 }
+// eps-regularised 2x2 solve of the omega correction (IdealEMF.cpp:110-118)
+__device__ __forceinline__ void uct_solve(double M11, double M12, double M22, double g1, double g2, double &l1, double &l2)
+{
+    const double tr = M11 + M22;
+    const double eps = 1e-14 * tr;
+    const double det = (M11 + eps) * (M22 + eps) - M12 * M12;
+    l1 = 0;
+    l2 = 0;
+    if (tr > 0 && det > 0)
+    {
+        l1 = ((M22 + eps) * g1 - M12 * g2) / det;
+        l2 = ((M11 + eps) * g2 - M12 * g1) / det;
+    }
+}
+constexpr int kEmfFast = 6; // valence served from registers by k_edge_emf
 // boundary EMF of Solver::rhs (MercuryUns/driver/Solver.cpp:162-167)
 __device__ __forceinline__ double edge_bc(const DevMesh &d, const Params &P, int32_t e, double v)
 {
@@ -474,7 +489,7 @@ __device__ __forceinline__ double edge_bc(const DevMesh &d, const Params &P, int
 __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const double *Ef, const double *u_cell, const double *phi_ind,
                                                   double cuct, int32_t vmax, const double *E_in, double *E_out, int32_t apply_bc)
 {
-    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t e = gid();
     if (e >= d.ne) return;
     const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
     const int32_t nf = f1 - f0;
@@ -492,42 +507,78 @@ __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const dou
         for (int32_t i = f0; i < f1; ++i) Ec += d.efw[i] * dot(L3(Ef, pid(d.e2f[i])), t);
         V3 e1, e2;
         plane_basis(th, e1, e2);
-        double M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0;
-        for (int32_t i = f0; i < f1; ++i)
+        double M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0, Eu = 0;
+        const int32_t nc = c1 - c0;
+        if (nf <= kEmfFast && nc <= kEmfFast)
         {
-            const int32_t f = pid(d.e2f[i]);
-            V3 m, nrm;
-            face_dirs(d, f, e, th, m, nrm);
-            const double aef = edge_amax(d, u_cell, c0, c1, m);
-            const V3 qv = (aef * dot(th, cross(m, nrm))) * nrm;
-            const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
-            M11 += w * qa * qa;
-            M12 += w * qa * qb;
-            M22 += w * qb * qb;
-            g1 += w * qa;
-            g2 += w * qb;
+            // common valence: the edge's cell velocities and the per-face terms stay in registers, so the
+            // second pass re-reads nothing (same operations in the same order as the general path)
+            V3 uc[kEmfFast];
+#pragma unroll
+            for (int k = 0; k < kEmfFast; ++k)
+                if (k < nc) uc[k] = L3(u_cell, d.ec_cell[c0 + k]);
+            double fqa[kEmfFast], fqb[kEmfFast], fw[kEmfFast], faef[kEmfFast], fsig[kEmfFast], fpf[kEmfFast];
+#pragma unroll
+            for (int i = 0; i < kEmfFast; ++i)
+                if (i < nf)
+                {
+                    const int32_t f = pid(d.e2f[f0 + i]);
+                    V3 m, nrm;
+                    face_dirs(d, f, e, th, m, nrm);
+                    double aef = 0.0;
+#pragma unroll
+                    for (int k = 0; k < kEmfFast; ++k)
+                        if (k < nc) aef = mx(aef, fabs(dot(uc[k], m)));
+                    const double sig = dot(th, cross(m, nrm));
+                    const V3 qv = (aef * sig) * nrm;
+                    const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[f0 + i];
+                    M11 += w * qa * qa;
+                    M12 += w * qa * qb;
+                    M22 += w * qb * qb;
+                    g1 += w * qa;
+                    g2 += w * qb;
+                    fqa[i] = qa; fqb[i] = qb; fw[i] = w; faef[i] = aef; fsig[i] = sig; fpf[i] = phi_ind[f] / d.famag[f];
+                }
+            double l1, l2;
+            uct_solve(M11, M12, M22, g1, g2, l1, l2);
+#pragma unroll
+            for (int i = 0; i < kEmfFast; ++i)
+                if (i < nf)
+                {
+                    const double omega = fw[i] * (1.0 - (fqa[i] * l1 + fqb[i] * l2));
+                    Eu += omega * faef[i] * fsig[i] * fpf[i];
+                }
         }
-        const double tr = M11 + M22;
-        const double eps = 1e-14 * tr;
-        const double det = (M11 + eps) * (M22 + eps) - M12 * M12;
-        double l1 = 0, l2 = 0;
-        if (tr > 0 && det > 0)
+        else
         {
-            l1 = ((M22 + eps) * g1 - M12 * g2) / det;
-            l2 = ((M11 + eps) * g2 - M12 * g1) / det;
-        }
-        double Eu = 0;
-        for (int32_t i = f0; i < f1; ++i)
-        {
-            const int32_t f = pid(d.e2f[i]);
-            V3 m, nrm;
-            face_dirs(d, f, e, th, m, nrm);
-            const double aef = edge_amax(d, u_cell, c0, c1, m);
-            const V3 qv = (aef * dot(th, cross(m, nrm))) * nrm;
-            const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
-            const double omega = w * (1.0 - (qa * l1 + qb * l2));
-            const double sig = dot(th, cross(m, nrm));
-            Eu += omega * aef * sig * (phi_ind[f] / d.famag[f]);
+            for (int32_t i = f0; i < f1; ++i)
+            {
+                const int32_t f = pid(d.e2f[i]);
+                V3 m, nrm;
+                face_dirs(d, f, e, th, m, nrm);
+                const double aef = edge_amax(d, u_cell, c0, c1, m);
+                const V3 qv = (aef * dot(th, cross(m, nrm))) * nrm;
+                const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
+                M11 += w * qa * qa;
+                M12 += w * qa * qb;
+                M22 += w * qb * qb;
+                g1 += w * qa;
+                g2 += w * qb;
+            }
+            double l1, l2;
+            uct_solve(M11, M12, M22, g1, g2, l1, l2);
+            for (int32_t i = f0; i < f1; ++i)
+            {
+                const int32_t f = pid(d.e2f[i]);
+                V3 m, nrm;
+                face_dirs(d, f, e, th, m, nrm);
+                const double aef = edge_amax(d, u_cell, c0, c1, m);
+                const V3 qv = (aef * dot(th, cross(m, nrm))) * nrm;
+                const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
+                const double omega = w * (1.0 - (qa * l1 + qb * l2));
+                const double sig = dot(th, cross(m, nrm));
+                Eu += omega * aef * sig * (phi_ind[f] / d.famag[f]);
+            }
         }
         out += Ec + 2.0 * cuct * L * Eu;
     }
@@ -620,7 +671,7 @@ __global__ void __launch_bounds__(256) k_edge_emf_heavy(DevMesh d, Params P, con
 __global__ void __launch_bounds__(256) k_faraday(DevMesh d, const double *E, const double *dte, const double *dt_dev, double dt_val,
                                                  double *B)
 {
-    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t f = gid();
     if (f >= d.nf) return;
     if (d.fL[f] < 0 && d.fR[f] < 0) return;
     double circ = 0.0;
@@ -684,7 +735,7 @@ __global__ void __launch_bounds__(256) k_div_b(DevMesh d, const double *B, const
 // reconstruct_cell_vector for every cell
 __global__ void __launch_bounds__(256) k_cell_vector(DevMesh d, const double *flux, double *b)
 {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = gid();
     if (c < d.nc) st3(b, c, cell_vector(d, c, flux));
 }
 
@@ -693,7 +744,7 @@ __global__ void __launch_bounds__(256) k_local_dt(DevMesh d, Params P, const dou
                                                   double *dt_part)
 {
     __shared__ double sh[256];
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = gid();
     double dtc = __builtin_inf();
     if (c < d.nc && d.mat[c] == 0) dtc = cell_dt(d, P, c, W0 + 5 * size_t(c), W1 + 5 * size_t(c), L3(Btot, c));
     const double m = block_min(dtc, sh);

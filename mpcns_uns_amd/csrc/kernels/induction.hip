@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_div_scalar(const double *r, const doubl
 // Phi'_f = sum_k f2e_k x_e, x = dir ? 0 : v * invdiag (right Jacobi preconditioner) or x = v.
 __global__ void __launch_bounds__(256) k_d1(DevMesh d, const double *v, const double *invdiag, const int8_t *dir, double *phi)
 {
-    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t f = gid();
     if (f >= d.nf) return;
     double s = 0.0;
     for (int32_t k = d.f2e_ptr[f]; k < d.f2e_ptr[f + 1]; ++k)
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) k_d1(DevMesh d, const double *v, const do
 // b_c = sum_k recon_k Phi_{f_k} (ImplicitInduction.cpp:150-160)
 __global__ void __launch_bounds__(256) k_bcell(DevMesh d, const double *phi, double *b)
 {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = gid();
     if (c >= d.nc) return;
     double bx = 0.0, by = 0.0, bz = 0.0;
     for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) k_bcell(DevMesh d, const double *phi, dou
 // g_c = alpha_c (J_c[b] x B0_c), zero where alpha_c = 0 (ImplicitInduction.cpp:162-174)
 __global__ void __launch_bounds__(256) k_hall_g(DevMesh d, const double *b, const double *alpha, const double *B0, double *g)
 {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = gid();
     if (c >= d.nc) return;
     const double a = alpha[c];
     V3 out;
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(256) k_edge_apply(DevMesh d, int32_t resistive
                                                     const double *v, const double *invdiag, const int8_t *dir, double dt, int32_t mode,
                                                     int32_t mask, const double *bvec, double *out)
 {
-    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t e = gid();
     if (e >= d.ne) return;
     if (d.e2f_ptr[e + 1] - d.e2f_ptr[e] > kLightMax) return; // k_edge_apply_heavy
     double p0 = 0.0, p1 = 0.0;

@@ -29,7 +29,7 @@ __device__ __forceinline__ double wave_sum(double v)
 
 __global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, const float *v, const float *invdiag, const int8_t *dir, float *phi)
 {
-    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t f = gid();
     if (f >= d.nf) return;
     float s = 0.0f;
     for (int32_t k = d.f2e_ptr[f]; k < d.f2e_ptr[f + 1]; ++k)
@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, const float *v, const
 
 __global__ void __launch_bounds__(256) k_bcell_f32(DevMesh d, DevMesh32 s, const float *phi, float *b)
 {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = gid();
     if (c >= d.nc) return;
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
     for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) k_bcell_f32(DevMesh d, DevMesh32 s, const
 
 __global__ void __launch_bounds__(256) k_hall_g_f32(DevMesh d, DevMesh32 s, const float *b, float *g)
 {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t c = gid();
     if (c >= d.nc) return;
     const float a = s.alpha[c];
     F3 out{0.0f, 0.0f, 0.0f};
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) k_hall_g_f32(DevMesh d, DevMesh32 s, cons
 __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi,
                                                         const float *g, const float *v, const int8_t *dir, float dt, float *out)
 {
-    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t e = gid();
     if (e >= d.ne) return;
     if (dir[e]) { out[e] = 0.0f; return; }
     if (d.e2f_ptr[e + 1] - d.e2f_ptr[e] > kLightMax) return; // k_edge_apply_f32_heavy

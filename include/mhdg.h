@@ -70,6 +70,9 @@ typedef struct mhdg_config {
      * MercuryUns/driver/Solver.cpp:261-262); 1 = inflow and wall edges (mars::MarsSolver,
      * MarsUns/driver/MarsSolver.cpp:730), for a conducting wall that is the domain boundary. */
     int32_t wall_dirichlet;
+    /* cfg.implicit_ideal (Config.h:52): the ideal EMF's part linear in the induced flux joins the implicit
+     * operator once mhdg_ind_set_ideal_state has supplied u (ImplicitInduction.cpp:150-157). */
+    int32_t implicit_ideal;
 } mhdg_config;
 
 /* Fills `cfg` with the reference defaults (Config.h:19-96 initialisers, Scales::simple(5/3, 1, 1)). */
@@ -118,12 +121,23 @@ int mhdg_add_sources(mhdg_ctx *ctx, int species, const double *W_H, const double
 /* ImplicitInduction (Common/induction/ImplicitInduction.h:27-94). */
 int mhdg_ind_set_frozen(mhdg_ctx *ctx, const double *B0_cell, const double *ne_eff_cell);
 int mhdg_ind_apply_F(mhdg_ctx *ctx, const double *phi, double *out);
-/* ImplicitInduction::solve (ImplicitInduction.cpp:299-390).  precision: 0 = FP64 GMRES(m) with
- * Jacobi and modified Gram-Schmidt (the reference iteration); 1 = mixed precision (FP32 multigrid-
- * preconditioned inner FGMRES inside FP64 iterative refinement).  The warm-start correction is kept
- * in the context between calls, as the reference keeps corr_. */
+/* ImplicitInduction::active / has_ideal / has_hall (ImplicitInduction.h:31-33). */
+int mhdg_ind_flags(mhdg_ctx *ctx, int32_t *active, int32_t *has_ideal, int32_t *has_hall);
+/* ImplicitInduction::set_ideal_state (ImplicitInduction.cpp:113-117): u 3C, mask E (non-zero: the edge
+ * carries the ideal EMF).  The arrays are copied at the call (the reference keeps the u pointer): call
+ * again after u changes. */
+int mhdg_ind_set_ideal_state(mhdg_ctx *ctx, const double *u_cell, const int8_t *mask);
+/* ImplicitInduction::ideal_linear (ImplicitInduction.cpp:119-125): apply_F with only the ideal part. */
+int mhdg_ind_ideal_linear(mhdg_ctx *ctx, const double *phi, double *out);
+/* ImplicitInduction::solve (ImplicitInduction.cpp:299-390).  dt_edge (nullable, E values) replaces the
+ * scalar dt: system (I + F d1 diag(dt_e)) E = b.  precision: 0 = FP64 GMRES(m) with Jacobi and modified
+ * Gram-Schmidt (the reference iteration); 1 = mixed precision (FP32 operator and Krylov basis, classical
+ * Gram-Schmidt with FP64 accumulation, inside FP64 iterative refinement to the same relative residual;
+ * not available with the implicit ideal part).  The warm-start correction is kept in the context between
+ * calls, as the reference keeps corr_. */
 int mhdg_ind_solve(mhdg_ctx *ctx, const double *Phi_n, const double *E_expl, const int8_t *dirichlet,
-                   double dt, int precision, double *E, int32_t *iterations, double *rel_res);
+                   double dt, const double *dt_edge, int precision, double *E, int32_t *iterations,
+                   double *rel_res);
 
 /* --------------------------------------------------------------- device-resident step (fused path) */
 /* State: U_H, U_Na (5C each, AoS), induced flux Phi (F); background PhiAdd (F) and BaddCell (3C). */
@@ -136,6 +150,12 @@ int mhdg_set_background(mhdg_ctx *ctx, const double *PhiAdd, const double *BaddC
  * mhdg_ind_solve.  Outputs the dt used, GMRES iterations and final relative residual. */
 int mhdg_step(mhdg_ctx *ctx, double dt_max, int precision, double *dt_used, int32_t *iterations,
               double *rel_res);
+/* The same step with the implicit solve switched off (ideal MHD: E = E_expl), and the same step that
+ * requires the implicit Hall/resistive solve to be active (error otherwise).  Fused device-resident
+ * forms of the explicit and the Hall configurations of step_first_order. */
+int mhdg_step_explicit(mhdg_ctx *ctx, double dt_max, double *dt_used);
+int mhdg_step_hall(mhdg_ctx *ctx, double dt_max, int precision, double *dt_used, int32_t *iterations,
+                   double *rel_res);
 /* Derived cell fields of the last prepared state (Solver.h:97-101): which = 0 W_H, 1 W_Na (5C),
  * 2 B_ind, 3 B_tot, 4 J, 5 u_plus (3C), 6 E of the last step (E). */
 int mhdg_get_derived(mhdg_ctx *ctx, int which, double *out);

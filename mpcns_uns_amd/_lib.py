@@ -43,6 +43,7 @@ class Config(C.Structure):
         ("ind_rtol", C.c_double), ("ind_max_it", C.c_int32), ("ind_restart", C.c_int32),
         ("cuct", C.c_double), ("fs", (C.c_double * 5) * 2), ("u_inf", C.c_double * 3), ("B_imf", C.c_double * 3),
         ("wall_dirichlet", C.c_int32),
+        ("implicit_ideal", C.c_int32),
     ]
 
 
@@ -82,7 +83,9 @@ def lib():
             "mhdg_div_b": [P, P, P, P, P, P], "mhdg_local_dt": [P, P, P, P, P],
             "mhdg_add_sources": [P, C.c_int, P, P, P, P, D, P],
             "mhdg_ind_set_frozen": [P, P, P], "mhdg_ind_apply_F": [P, P, P],
-            "mhdg_ind_solve": [P, P, P, P, D, C.c_int, P, P, P],
+            "mhdg_ind_solve": [P, P, P, P, D, P, C.c_int, P, P, P],
+            "mhdg_ind_flags": [P, P, P, P], "mhdg_ind_set_ideal_state": [P, P, P], "mhdg_ind_ideal_linear": [P, P, P],
+            "mhdg_step_explicit": [P, D, P], "mhdg_step_hall": [P, D, C.c_int, P, P, P],
             "mhdg_set_state": [P, P, P, P], "mhdg_get_state": [P, P, P, P],
             "mhdg_set_background": [P, P, P], "mhdg_step": [P, D, C.c_int, P, P, P],
             "mhdg_get_derived": [P, C.c_int, P], "mhdg_sync": [P], "mhdg_device_info": [C.c_char_p, C.c_int],

@@ -29,6 +29,11 @@ struct mhdg_ctx
     bool have_corr = false;
     double *x0 = nullptr, *bvec = nullptr, *rvec = nullptr, *wvec = nullptr, *invdiag = nullptr, *delta = nullptr, *tmp = nullptr;
     double *scal = nullptr, *hcol = nullptr, *ydev = nullptr;
+    // ImplicitInduction flags (ImplicitInduction.cpp:11) and the parts apply_op evaluates
+    bool op_res = false, op_hall = false, ideal_cfg = false, skip_implicit = false;
+    double *u_ideal = nullptr, *ide = nullptr, *Ef_i = nullptr; // implicit ideal part (set_ideal_state)
+    int8_t *imask = nullptr;
+    const double *dte_cur = nullptr;                              // per-edge pseudo-time step of the current solve
     std::vector<double *> V;
     double **Vptrs = nullptr;
     int restart = 2;

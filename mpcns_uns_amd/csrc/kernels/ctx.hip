@@ -143,6 +143,9 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
 {
     cfg = c;
     make_params(c, P);
+    op_hall = P.hall != 0;
+    op_res = P.resistive != 0;
+    ideal_cfg = c.implicit_ideal != 0;
     hip_check(hipStreamCreate(&stream), "hipStreamCreate");
     Tables t = build_tables(m, c, kLightMax);
     if (t.max_valence > kHeavyMax) throw std::runtime_error("edge valence exceeds the supported maximum");
@@ -229,19 +232,27 @@ void mhdg_ctx::apply_op(const double *phi_in, const double *v, const double *inv
     const double *phi = phi_in;
     if (!phi)
     {
-        LAUNCH("d1", k_d1, dim3(nblk(nf)), dim3(256), 0, stream, d, v, inv, dir, phi_w);
+        LAUNCH("d1", k_d1, dim3(nblk(nf)), dim3(256), 0, stream, d, v, inv, dir, dte_cur, phi_w);
         phi = phi_w;
+        if (dte_cur) dt = 1.0; // d1 already carries dt_e: y = x + F (ImplicitInduction.cpp:196)
     }
-    if (P.hall)
+    const bool ideal_part = ideal_cfg && u_ideal;
+    if (op_hall || ideal_part) LAUNCH("bcell", k_bcell, dim3(nblk(nc)), dim3(256), 0, stream, d, phi, bcell);
+    if (op_hall) LAUNCH("hall_g", k_hall_g, dim3(nblk(nc)), dim3(256), 0, stream, d, bcell, alpha, B0, gcell);
+    if (ideal_part)
     {
-        LAUNCH("bcell", k_bcell, dim3(nblk(nc)), dim3(256), 0, stream, d, phi, bcell);
-        LAUNCH("hall_g", k_hall_g, dim3(nblk(nc)), dim3(256), 0, stream, d, bcell, alpha, B0, gcell);
+        // add_ideal_emf of (u, b, phi) with b as total and induced field (ImplicitInduction.cpp:150-157)
+        SpC none{};
+        LAUNCH("face", k_face, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 0, 0, none, none, none, static_cast<double *>(nullptr), u_ideal,
+               bcell, bcell, phi, static_cast<const double *>(nullptr), Ef_i);
+        launch_edge_emf(Ef_i, u_ideal, phi, P.cuct, nullptr, ide, 0);
     }
-    LAUNCH("edge_apply", k_edge_apply, dim3(nblk(ne)), dim3(256), 0, stream, d, P.resistive, P.hall, phi, gcell, v, inv, dir, dt, mode,
-                       mask, b, out);
+    const double *idp = ideal_part ? ide : nullptr;
+    LAUNCH("edge_apply", k_edge_apply, dim3(nblk(ne)), dim3(256), 0, stream, d, int32_t(op_res), int32_t(op_hall), phi, gcell, idp,
+           imask, v, inv, dir, dt, mode, mask, b, out);
     if (n_heavy)
-        LAUNCH("edge_apply_heavy", k_edge_apply_heavy, dim3(n_heavy), dim3(256), 0, stream, d, P.resistive, P.hall,
-               static_cast<const int32_t *>(heavy), phi, gcell, v, inv, dir, dt, mode, mask, b, out);
+        LAUNCH("edge_apply_heavy", k_edge_apply_heavy, dim3(n_heavy), dim3(256), 0, stream, d, int32_t(op_res), int32_t(op_hall),
+               static_cast<const int32_t *>(heavy), phi, gcell, idp, imask, v, inv, dir, dt, mode, mask, b, out);
     check_launch("apply_op");
 }
 
@@ -252,8 +263,8 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
     const int m = restart;
     const int max_it = cfg.ind_max_it;
     const double rtol = cfg.ind_rtol;
-    LAUNCH("solve_init", k_solve_init, dim3(nblk(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt, ne,
-                       x0, invdiag);
+    LAUNCH("solve_init", k_solve_init, dim3(nblk(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt,
+           dte_cur, ne, x0, invdiag);
     apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp); // F(Phi^n)
     LAUNCH("axpy", k_add, dim3(nblk(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
     apply_op(nullptr, x0, nullptr, dir, dt, 1, 0, bvec, rvec); // r = b - A x0
@@ -351,7 +362,7 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     *iters = 0;
     *res = 0.0;
     const double *Euse = E_expl;
-    if (P.hall || P.resistive)
+    if ((op_hall || op_res || ideal_cfg) && !skip_implicit) // ImplicitInduction::active()
     {
         hip_check(hipMemcpyAsync(B0, Btot, 3 * size_t(nc) * 8, hipMemcpyDeviceToDevice, stream), "copy");
         if (precision == 0) *iters = solve64(Phi, E_expl, dir_mercury, gdt, E_last, res);
@@ -619,16 +630,57 @@ int mhdg_ind_apply_F(mhdg_ctx *c, const double *phi, double *out)
         c->d2h(out, c->tmp, size_t(c->ne) * 8);
     })
 }
-int mhdg_ind_solve(mhdg_ctx *c, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, int precision, double *E,
-                   int32_t *iterations, double *rel_res)
+int mhdg_ind_flags(mhdg_ctx *c, int32_t *active, int32_t *has_ideal, int32_t *has_hall)
+{
+    MHDG_TRY(c, {
+        if (active) *active = c->op_res || c->op_hall || c->ideal_cfg;
+        if (has_ideal) *has_ideal = c->ideal_cfg;
+        if (has_hall) *has_hall = c->op_hall;
+    })
+}
+int mhdg_ind_set_ideal_state(mhdg_ctx *c, const double *u, const int8_t *mask)
+{
+    MHDG_TRY(c, {
+        if (!c->u_ideal)
+        {
+            c->u_ideal = static_cast<double *>(c->alloc(3 * size_t(c->nc) * 8));
+            c->ide = static_cast<double *>(c->alloc(size_t(c->ne) * 8));
+            c->Ef_i = static_cast<double *>(c->alloc(3 * size_t(c->nf) * 8));
+            c->imask = static_cast<int8_t *>(c->alloc(size_t(c->ne)));
+        }
+        hip_check(hipMemcpyAsync(c->u_ideal, u, 3 * size_t(c->nc) * 8, hipMemcpyHostToDevice, c->stream), "H2D");
+        hip_check(hipMemcpyAsync(c->imask, mask, size_t(c->ne), hipMemcpyHostToDevice, c->stream), "H2D");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    })
+}
+int mhdg_ind_ideal_linear(mhdg_ctx *c, const double *phi, double *out)
+{
+    MHDG_TRY(c, {
+        const double *p = c->stage_in(0, phi, c->nf);
+        const bool h = c->op_hall, r = c->op_res;
+        c->op_hall = false;
+        c->op_res = false;
+        try { c->apply_op(p, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, c->tmp); }
+        catch (...) { c->op_hall = h; c->op_res = r; throw; }
+        c->op_hall = h;
+        c->op_res = r;
+        c->d2h(out, c->tmp, size_t(c->ne) * 8);
+    })
+}
+int mhdg_ind_solve(mhdg_ctx *c, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, const double *dt_edge,
+                   int precision, double *E, int32_t *iterations, double *rel_res)
 {
     MHDG_TRY(c, {
         const double *p = c->stage_in(0, Phi_n, c->nf), *ee = c->stage_in(1, E_expl, c->ne);
         int8_t *dd = reinterpret_cast<int8_t *>(c->stage(2, (size_t(c->ne) + 7) / 8));
         hip_check(hipMemcpyAsync(dd, dir, size_t(c->ne), hipMemcpyHostToDevice, c->stream), "H2D");
         double *eo = c->stage(3, c->ne);
+        c->dte_cur = dt_edge ? c->stage_in(4, dt_edge, c->ne) : nullptr;
         double rr = 0;
-        const int it = precision == 0 ? c->solve64(p, ee, dd, dt, eo, &rr) : c->solve_mixed(p, ee, dd, dt, eo, &rr);
+        int it = 0;
+        try { it = precision == 0 ? c->solve64(p, ee, dd, dt, eo, &rr) : c->solve_mixed(p, ee, dd, dt, eo, &rr); }
+        catch (...) { c->dte_cur = nullptr; throw; }
+        c->dte_cur = nullptr;
         c->d2h(E, eo, size_t(c->ne) * 8);
         *iterations = it;
         if (rel_res) *rel_res = rr;
@@ -662,6 +714,31 @@ int mhdg_set_background(mhdg_ctx *c, const double *PA, const double *BA)
 int mhdg_step(mhdg_ctx *c, double dt_max, int precision, double *dt_used, int32_t *iterations, double *rel_res)
 {
     MHDG_TRY(c, {
+        int it = 0;
+        double rr = 0;
+        const double dt = c->step(dt_max, precision, &it, &rr);
+        if (dt_used) *dt_used = dt;
+        if (iterations) *iterations = it;
+        if (rel_res) *rel_res = rr;
+    })
+}
+int mhdg_step_explicit(mhdg_ctx *c, double dt_max, double *dt_used)
+{
+    MHDG_TRY(c, {
+        int it = 0;
+        double rr = 0;
+        c->skip_implicit = true;
+        double dt = 0;
+        try { dt = c->step(dt_max, 0, &it, &rr); }
+        catch (...) { c->skip_implicit = false; throw; }
+        c->skip_implicit = false;
+        if (dt_used) *dt_used = dt;
+    })
+}
+int mhdg_step_hall(mhdg_ctx *c, double dt_max, int precision, double *dt_used, int32_t *iterations, double *rel_res)
+{
+    MHDG_TRY(c, {
+        if (!(c->op_hall || c->op_res)) throw std::runtime_error("step_hall: the configuration has no implicit Hall/resistive part");
         int it = 0;
         double rr = 0;
         const double dt = c->step(dt_max, precision, &it, &rr);

@@ -16,18 +16,19 @@ template <class T> __global__ void k_dot(const T *a, const T *b, int32_t n, RedS
 template <class T> __global__ void k_mgs(T *w, const T *Vprev, const T *Vi, int32_t n, double *h, int32_t i, int32_t norm, RedScratch rs);
 template <class T> __global__ void k_scale_div(const T *w, const double *h, int32_t n, T *out);
 __global__ void k_div_scalar(const double *r, const double *beta, int32_t n, double *out);
-__global__ void k_d1(DevMesh d, const double *v, const double *invdiag, const int8_t *dir, double *phi);
+__global__ void k_d1(DevMesh d, const double *v, const double *invdiag, const int8_t *dir, const double *dte, double *phi);
 __global__ void k_bcell(DevMesh d, const double *phi, double *b);
 __global__ void k_hall_g(DevMesh d, const double *b, const double *alpha, const double *B0, double *g);
-__global__ void k_edge_apply(DevMesh d, int32_t resistive, int32_t hall, const double *phi, const double *g, const double *v,
-                             const double *invdiag, const int8_t *dir, double dt, int32_t mode, int32_t mask, const double *bvec, double *out);
+__global__ void k_edge_apply(DevMesh d, int32_t resistive, int32_t hall, const double *phi, const double *g, const double *ide,
+                             const int8_t *imask, const double *v, const double *invdiag, const int8_t *dir, double dt, int32_t mode,
+                             int32_t mask, const double *bvec, double *out);
 __global__ void k_edge_apply_heavy(DevMesh d, int32_t resistive, int32_t hall, const int32_t *heavy, const double *phi, const double *g,
-                                   const double *v, const double *invdiag, const int8_t *dir, double dt, int32_t mode, int32_t mask,
-                                   const double *bvec, double *out);
+                                   const double *ide, const int8_t *imask, const double *v, const double *invdiag, const int8_t *dir,
+                                   double dt, int32_t mode, int32_t mask, const double *bvec, double *out);
 __global__ void k_gmres_update(const double *const *V, const double *y, int32_t j, const double *invdiag, const int8_t *dir, int32_t n,
                                double *delta);
 __global__ void k_solve_init(const double *E_expl, const double *corr, const int8_t *dir, const double *m_e, const double *d2, double dt,
-                             int32_t n, double *x0, double *invdiag);
+                             const double *dte, int32_t n, double *x0, double *invdiag);
 __global__ void k_add(const double *a, const double *b, int32_t n, double *out);
 __global__ void k_solve_final(const double *x0, const double *delta, const double *E_expl, const int8_t *dir, int32_t n, double *E,
                               double *corr);

@@ -106,8 +106,10 @@ __global__ void __launch_bounds__(256) k_div_scalar(const double *r, const doubl
 }
 
 // ------------------------------------------------------------------------------------ operator
-// Phi'_f = sum_k f2e_k x_e, x = dir ? 0 : v * invdiag (right Jacobi preconditioner) or x = v.
-__global__ void __launch_bounds__(256) k_d1(DevMesh d, const double *v, const double *invdiag, const int8_t *dir, double *phi)
+// Phi'_f = sum_k f2e_k x_e, x = dir ? 0 : v * invdiag (right Jacobi preconditioner) or x = v; with a per-edge
+// pseudo-time step dte (nullable) the terms are (f2e_k dte_e) x_e (ImplicitInduction.cpp:187-194).
+__global__ void __launch_bounds__(256) k_d1(DevMesh d, const double *v, const double *invdiag, const int8_t *dir, const double *dte,
+                                            double *phi)
 {
     const int32_t f = gid();
     if (f >= d.nf) return;
@@ -116,7 +118,7 @@ __global__ void __launch_bounds__(256) k_d1(DevMesh d, const double *v, const do
     {
         const int32_t q = d.f2e[k], e = pid(q);
         const double x = invdiag ? (dir[e] ? 0.0 : v[e] * invdiag[e]) : v[e];
-        s += psg(q) * x;
+        s += dte ? psg(q) * dte[e] * x : psg(q) * x;
     }
     phi[f] = s;
 }
@@ -161,12 +163,13 @@ __global__ void __launch_bounds__(256) k_hall_g(DevMesh d, const double *b, cons
 }
 
 // Edge assembly of apply_F (ImplicitInduction.cpp:139-182): part0 = sum_f f2e star2 Phi (faces ascending),
-// part1 = sum_c g_c . (w_ec t_e) (cells ascending), F = m part0 + part1 + 0.
+// part1 = sum_c g_c . (w_ec t_e) (cells ascending), part2 = imask ? ide : 0 (implicit ideal part, ide nullable),
+// F = m part0 + part1 + part2.
 // mode 0: out = F.  mode 1 (apply_A): out = x + dt F with x = dir ? 0 : v*invdiag (invdiag non-null) or v;
 // with `mask` set, Dirichlet rows of the result are zeroed (Arnoldi vector, ImplicitInduction.cpp:340).
 __global__ void __launch_bounds__(256) k_edge_apply(DevMesh d, int32_t resistive, int32_t hall, const double *phi, const double *g,
-                                                    const double *v, const double *invdiag, const int8_t *dir, double dt, int32_t mode,
-                                                    int32_t mask, const double *bvec, double *out)
+                                                    const double *ide, const int8_t *imask, const double *v, const double *invdiag,
+                                                    const int8_t *dir, double dt, int32_t mode, int32_t mask, const double *bvec, double *out)
 {
     const int32_t e = gid();
     if (e >= d.ne) return;
@@ -189,7 +192,8 @@ __global__ void __launch_bounds__(256) k_edge_apply(DevMesh d, int32_t resistive
             p1 += dot(L3(g, d.ec_cell[k]), wt);
         }
     }
-    const double F = d.m_e[e] * p0 + p1 + 0.0;
+    const double p2 = ide ? (imask[e] ? ide[e] : 0.0) : 0.0;
+    const double F = d.m_e[e] * p0 + p1 + p2;
     double r;
     if (mode == 0) r = F;
     else
@@ -205,8 +209,9 @@ __global__ void __launch_bounds__(256) k_edge_apply(DevMesh d, int32_t resistive
 // k_edge_apply for the high-valence edges, one block per edge: terms in parallel into LDS, sums sequential
 // in the reference's order (faces ascending, cells ascending).
 __global__ void __launch_bounds__(256) k_edge_apply_heavy(DevMesh d, int32_t resistive, int32_t hall, const int32_t *heavy, const double *phi,
-                                                          const double *g, const double *v, const double *invdiag, const int8_t *dir,
-                                                          double dt, int32_t mode, int32_t mask, const double *bvec, double *out)
+                                                          const double *g, const double *ide, const int8_t *imask, const double *v,
+                                                          const double *invdiag, const int8_t *dir, double dt, int32_t mode, int32_t mask,
+                                                          const double *bvec, double *out)
 {
     __shared__ double t0[kHeavyMax], t1[2 * kHeavyMax];
     const int32_t e = heavy[blockIdx.x];
@@ -233,7 +238,8 @@ __global__ void __launch_bounds__(256) k_edge_apply_heavy(DevMesh d, int32_t res
     double p0 = 0.0, p1 = 0.0;
     if (resistive) for (int32_t i = 0; i < nf; ++i) p0 += t0[i];
     if (hall) for (int32_t i = 0; i < ncl; ++i) p1 += t1[i];
-    const double F = d.m_e[e] * p0 + p1 + 0.0;
+    const double p2 = ide ? (imask[e] ? ide[e] : 0.0) : 0.0;
+    const double F = d.m_e[e] * p0 + p1 + p2;
     double r;
     if (mode == 0) r = F;
     else
@@ -259,14 +265,14 @@ __global__ void __launch_bounds__(256) k_gmres_update(const double *const *V, co
 
 // elementwise helpers of solve()
 __global__ void __launch_bounds__(256) k_solve_init(const double *E_expl, const double *corr, const int8_t *dir, const double *m_e,
-                                                    const double *d2, double dt, int32_t n, double *x0, double *invdiag)
+                                                    const double *d2, double dt, const double *dte, int32_t n, double *x0, double *invdiag)
 {
     const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     double x = E_expl[e];
     if (corr && !dir[e]) x += corr[e];
     x0[e] = x;
-    invdiag[e] = 1.0 / (1.0 + dt * m_e[e] * d2[e]);
+    invdiag[e] = 1.0 / (1.0 + (dte ? dte[e] : dt) * m_e[e] * d2[e]);
 }
 __global__ void __launch_bounds__(256) k_add(const double *a, const double *b, int32_t n, double *out)
 {

@@ -17,7 +17,7 @@ struct DevMesh32
 
 constexpr int kDotChunk = 16; // basis vectors per multi-dot pass
 
-__global__ void k_d1_f32(DevMesh d, const float *v, const float *invdiag, const int8_t *dir, float *phi);
+__global__ void k_d1_f32(DevMesh d, const float *v, const float *invdiag, const int8_t *dir, const float *dte, float *phi);
 __global__ void k_bcell_f32(DevMesh d, DevMesh32 s, const float *phi, float *b);
 __global__ void k_hall_g_f32(DevMesh d, DevMesh32 s, const float *b, float *g);
 __global__ void k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi, const float *g, const float *v,
@@ -37,7 +37,7 @@ __global__ void k_alpha_f32(const double *alpha, const double *B0, int32_t nc, f
 struct MixedSolver
 {
     DevMesh32 s{};
-    float *phi = nullptr, *b = nullptr, *g = nullptr, *w = nullptr;
+    float *phi = nullptr, *b = nullptr, *g = nullptr, *w = nullptr, *dte32 = nullptr;
     std::vector<float *> V;
     float **Vptrs = nullptr;
     double *h = nullptr, *part = nullptr;

@@ -27,7 +27,8 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 } // namespace
 
-__global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, const float *v, const float *invdiag, const int8_t *dir, float *phi)
+__global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, const float *v, const float *invdiag, const int8_t *dir, const float *dte,
+                                                float *phi)
 {
     const int32_t f = gid();
     if (f >= d.nf) return;
@@ -36,7 +37,7 @@ __global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, const float *v, const
     {
         const int32_t q = d.f2e[k], e = pid(q);
         const float x = dir[e] ? 0.0f : v[e] * invdiag[e];
-        s += fsg(q) * x;
+        s += fsg(q) * (dte ? dte[e] * x : x);
     }
     phi[f] = s;
 }
@@ -323,6 +324,16 @@ using namespace mhdg;
 
 namespace
 {
+#ifndef MHDG_INNER_RTOL
+#define MHDG_INNER_RTOL 1e-6
+#endif
+#ifndef MHDG_DGKS
+#define MHDG_DGKS 0.1
+#endif
+// inner FP32 GMRES stops at this fraction of the outer FP64 residual
+constexpr double kInnerRtol = MHDG_INNER_RTOL;
+// DGKS test: re-orthogonalise when the projection removed more than 1 - kDgks of the vector's norm
+constexpr double kDgks = MHDG_DGKS;
 inline int nb(int64_t n) { return static_cast<int>((n + 255) / 256); }
 void chk(hipError_t e, const char *w)
 {
@@ -382,10 +393,18 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
     const int m = M.m;
     const int max_it = cfg.ind_max_it;
     const double rtol = cfg.ind_rtol;
-    const double inner_rtol = 1e-4;
+    const double inner_rtol = kInnerRtol;
+    if (ideal_cfg && u_ideal) throw std::runtime_error("mixed-precision solve: the implicit ideal part has no FP32 operator; use precision 0");
+    const float *dte32 = nullptr;
+    if (dte_cur)
+    {
+        if (!M.dte32) M.dte32 = static_cast<float *>(alloc(size_t(ne) * sizeof(float)));
+        LAUNCH("f64_to_f32", k_f64_to_f32, dim3(nb(ne)), dim3(256), 0, stream, dte_cur, int64_t(ne), M.dte32);
+        dte32 = M.dte32;
+    }
     // frozen state and Jacobi diagonal in FP32
     LAUNCH("alpha_f32", k_alpha_f32, dim3(nb(nc)), dim3(256), 0, stream, alpha, B0, nc, M.s.alpha, M.s.B0);
-    LAUNCH("solve_init", k_solve_init, dim3(nb(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt, ne, x0,
+    LAUNCH("solve_init", k_solve_init, dim3(nb(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt, dte_cur, ne, x0,
            invdiag);
     LAUNCH("f64_to_f32", k_f64_to_f32, dim3(nb(ne)), dim3(256), 0, stream, invdiag, int64_t(ne), const_cast<float *>(M.s.invdiag));
     apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp);
@@ -393,7 +412,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
     chk(hipMemsetAsync(delta, 0, size_t(ne) * 8, stream), "memset");
     const double bb = dot(bvec, bvec, 0, 0);
     const double bnorm = std::sqrt(std::max(bb, 1e-300));
-    const float dtf = static_cast<float>(dt);
+    const float dtf = dte32 ? 1.0f : static_cast<float>(dt); // with dt_e, d1 carries the step
     int total = 0;
     double beta = 0;
     std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
@@ -414,16 +433,16 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
         for (; j < m && total < max_it; ++j, ++total)
         {
             // w = A P V_j in FP32
-            LAUNCH("d1_f32", k_d1_f32, dim3(nb(nf)), dim3(256), 0, stream, d, M.V[j], M.s.invdiag, dir, M.phi);
-            if (P.hall)
+            LAUNCH("d1_f32", k_d1_f32, dim3(nb(nf)), dim3(256), 0, stream, d, M.V[j], M.s.invdiag, dir, dte32, M.phi);
+            if (op_hall)
             {
                 LAUNCH("bcell_f32", k_bcell_f32, dim3(nb(nc)), dim3(256), 0, stream, d, M.s, M.phi, M.b);
                 LAUNCH("hall_g_f32", k_hall_g_f32, dim3(nb(nc)), dim3(256), 0, stream, d, M.s, M.b, M.g);
             }
-            LAUNCH("edge_apply_f32", k_edge_apply_f32, dim3(nb(ne)), dim3(256), 0, stream, d, M.s, P.resistive, P.hall, M.phi, M.g, M.V[j],
+            LAUNCH("edge_apply_f32", k_edge_apply_f32, dim3(nb(ne)), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall), M.phi, M.g, M.V[j],
                    dir, dtf, M.w);
             if (n_heavy)
-                LAUNCH("edge_apply_f32_heavy", k_edge_apply_f32_heavy, dim3(n_heavy), dim3(256), 0, stream, d, M.s, P.resistive, P.hall,
+                LAUNCH("edge_apply_f32_heavy", k_edge_apply_f32_heavy, dim3(n_heavy), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall),
                        static_cast<const int32_t *>(heavy), M.phi, M.g, M.V[j], dir, dtf, M.w);
             // classical Gram-Schmidt against V_0..V_j
             for (int i0 = 0; i0 <= j; i0 += kDotChunk)
@@ -432,7 +451,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
             LAUNCH("multiaxpy", k_multiaxpy_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, j + 1, M.h, M.w, ne, rs, M.h + j + 1);
             d2h(hc.data(), M.h, sizeof(double) * 64);
             double hn = hc[j + 1];
-            if (hn < 0.7071 * std::sqrt(std::max(hc[63], 0.0)))
+            if (hn < kDgks * std::sqrt(std::max(hc[63], 0.0)))
             {
                 // DGKS: one more projection
                 for (int i0 = 0; i0 <= j; i0 += kDotChunk)

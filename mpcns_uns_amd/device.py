@@ -101,12 +101,26 @@ class Device:
         self._ok(lib().mhdg_ind_apply_F(self._h, ptr(f64(phi)), ptr(out)))
         return out
 
-    def ind_solve(self, Phi_n, E_expl, dirichlet, dt, precision=0):
+    def ind_flags(self):
+        """(active, has_ideal, has_hall) of ImplicitInduction (ImplicitInduction.h:31-33)."""
+        a, i, h = C.c_int32(), C.c_int32(), C.c_int32()
+        self._ok(lib().mhdg_ind_flags(self._h, C.byref(a), C.byref(i), C.byref(h)))
+        return bool(a.value), bool(i.value), bool(h.value)
+
+    def ind_set_ideal_state(self, u, mask):
+        self._ok(lib().mhdg_ind_set_ideal_state(self._h, ptr(f64(u)), ptr(np.ascontiguousarray(mask, dtype=np.int8))))
+
+    def ind_ideal_linear(self, phi):
+        out = np.zeros(self.ne)
+        self._ok(lib().mhdg_ind_ideal_linear(self._h, ptr(f64(phi)), ptr(out)))
+        return out
+
+    def ind_solve(self, Phi_n, E_expl, dirichlet, dt, precision=0, dt_edge=None):
         E = np.zeros(self.ne)
         it, rr = C.c_int32(), C.c_double()
         d = np.ascontiguousarray(dirichlet, dtype=np.int8)
-        self._ok(lib().mhdg_ind_solve(self._h, ptr(f64(Phi_n)), ptr(f64(E_expl)), ptr(d), dt, precision, ptr(E),
-                                      C.byref(it), C.byref(rr)))
+        self._ok(lib().mhdg_ind_solve(self._h, ptr(f64(Phi_n)), ptr(f64(E_expl)), ptr(d), dt,
+                                      ptr(None if dt_edge is None else f64(dt_edge)), precision, ptr(E), C.byref(it), C.byref(rr)))
         return E, it.value, rr.value
 
     # ------------------------------------------------------------------ resident state
@@ -124,6 +138,16 @@ class Device:
     def step(self, dt_max=1e9, precision=0):
         dt, it, rr = C.c_double(), C.c_int32(), C.c_double()
         self._ok(lib().mhdg_step(self._h, dt_max, precision, C.byref(dt), C.byref(it), C.byref(rr)))
+        return dt.value, it.value, rr.value
+
+    def step_explicit(self, dt_max=1e9):
+        dt = C.c_double()
+        self._ok(lib().mhdg_step_explicit(self._h, dt_max, C.byref(dt)))
+        return dt.value
+
+    def step_hall(self, dt_max=1e9, precision=0):
+        dt, it, rr = C.c_double(), C.c_int32(), C.c_double()
+        self._ok(lib().mhdg_step_hall(self._h, dt_max, precision, C.byref(dt), C.byref(it), C.byref(rr)))
         return dt.value, it.value, rr.value
 
     def derived(self, name):

@@ -149,8 +149,10 @@ struct orc_ctx
     std::vector<double> sinv;
     std::vector<char> svalid;
     // ImplicitInduction (ImplicitInduction.h:61-93)
-    bool resistive = false, hall = false;
+    bool resistive = false, hall = false, ideal = false;
     std::vector<double> m_, taper_, alpha_, B0_, recon_, d2_, corr_;
+    std::vector<double> u_ideal;      // set_ideal_state (ImplicitInduction.cpp:113-117), empty until set
+    std::vector<char> ideal_mask;
     std::vector<int64_t> ce_ptr;
     std::vector<int32_t> ce_edge;
     std::vector<V3> ce_wt;
@@ -222,6 +224,7 @@ void build_induction(orc_ctx &o)
     const int32_t ne = o.ne, nc = o.nc;
     o.resistive = cfg.resistive && cfg.inver_Rem > 0;
     o.hall = cfg.hall && cfg.hall_coef > 0;
+    o.ideal = cfg.implicit_ideal != 0;
     o.m_.assign(ne, 0.0);
     if (o.resistive)
         for (int32_t e = 0; e < ne; ++e)
@@ -690,8 +693,7 @@ void apply_F(orc_ctx &o, const double *phi, double *out, std::vector<double> &pa
             const double v = o.star2[f] * phi[f];
             for (int64_t k = o.f2e.ptr[f]; k < o.f2e.ptr[f + 1]; ++k) part[3 * size_t(o.f2e.idx[k])] += o.f2e.sgn[k] * v;
         }
-    if (o.hall)
-    {
+    if (o.hall || o.ideal)
 #pragma omp parallel for schedule(static)
         for (int32_t c = 0; c < nc; ++c)
         {
@@ -703,6 +705,15 @@ void apply_F(orc_ctx &o, const double *phi, double *out, std::vector<double> &pa
             }
             bcell[3 * size_t(c)] = b.x; bcell[3 * size_t(c) + 1] = b.y; bcell[3 * size_t(c) + 2] = b.z;
         }
+    if (o.ideal && !o.u_ideal.empty())
+    {
+        // ImplicitInduction.cpp:150-157: ideal EMF of (u, b, phi) with b as total and induced field
+        std::vector<double> ide(ne, 0.0);
+        ideal_emf(o, o.u_ideal.data(), bcell.data(), bcell.data(), phi, phi, o.cfg.cuct, ide.data());
+        for (int32_t e = 0; e < ne; ++e) part[3 * size_t(e) + 2] = o.ideal_mask[e] ? ide[e] : 0.0;
+    }
+    if (o.hall)
+    {
         for (int32_t c = 0; c < nc; ++c)
         {
             if (o.alpha_[c] == 0.0) continue;
@@ -733,17 +744,19 @@ struct IndWork
     explicit IndWork(const orc_ctx &o) : part(3 * size_t(o.ne)), bcell(3 * size_t(o.nc)), phi(o.nf), tmp(o.ne) {}
 };
 
-// Common/induction/ImplicitInduction.cpp:185-198
-void apply_A(orc_ctx &o, IndWork &wk, const std::vector<double> &x, std::vector<double> &y, double dt)
+// Common/induction/ImplicitInduction.cpp:185-198 (dte: per-edge pseudo-time step, null for the scalar dt)
+void apply_A(orc_ctx &o, IndWork &wk, const std::vector<double> &x, std::vector<double> &y, double dt, const double *dte)
 {
     for (int32_t f = 0; f < o.nf; ++f)
     {
         double s = 0;
-        for (int64_t k = o.f2e.ptr[f]; k < o.f2e.ptr[f + 1]; ++k) s += o.f2e.sgn[k] * x[o.f2e.idx[k]];
+        if (dte) for (int64_t k = o.f2e.ptr[f]; k < o.f2e.ptr[f + 1]; ++k) s += o.f2e.sgn[k] * dte[o.f2e.idx[k]] * x[o.f2e.idx[k]];
+        else for (int64_t k = o.f2e.ptr[f]; k < o.f2e.ptr[f + 1]; ++k) s += o.f2e.sgn[k] * x[o.f2e.idx[k]];
         wk.phi[f] = s;
     }
     apply_F(o, wk.phi.data(), wk.tmp.data(), wk.part, wk.bcell);
-    for (size_t e = 0; e < y.size(); ++e) y[e] = x[e] + dt * wk.tmp[e];
+    if (dte) for (size_t e = 0; e < y.size(); ++e) y[e] = x[e] + wk.tmp[e];
+    else for (size_t e = 0; e < y.size(); ++e) y[e] = x[e] + dt * wk.tmp[e];
 }
 
 double dot_all(const std::vector<double> &a, const std::vector<double> &b)
@@ -754,7 +767,8 @@ double dot_all(const std::vector<double> &a, const std::vector<double> &b)
 }
 
 // Common/induction/ImplicitInduction.cpp:299-390 (Jacobi right preconditioner, MGS Arnoldi)
-int ind_solve(orc_ctx &o, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res)
+int ind_solve(orc_ctx &o, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res,
+              const double *dte = nullptr)
 {
     const int32_t ne = o.ne;
     using Vecd = std::vector<double>;
@@ -767,10 +781,10 @@ int ind_solve(orc_ctx &o, const double *Phi_n, const double *E_expl, const int8_
     Vecd Fn(ne);
     apply_F(o, Phi_n, Fn.data(), wk.part, wk.bcell);
     for (int32_t e = 0; e < ne; ++e) b[e] = E_expl[e] + Fn[e];
-    apply_A(o, wk, x0, Ax, dt);
+    apply_A(o, wk, x0, Ax, dt, dte);
     for (int32_t e = 0; e < ne; ++e) r[e] = dir[e] ? 0.0 : b[e] - Ax[e];
     Vecd inv_diag(ne);
-    for (int32_t e = 0; e < ne; ++e) inv_diag[e] = 1.0 / (1.0 + dt * o.m_[e] * o.d2_[e]);
+    for (int32_t e = 0; e < ne; ++e) inv_diag[e] = 1.0 / (1.0 + (dte ? dte[e] : dt) * o.m_[e] * o.d2_[e]);
     const double bnorm = std::sqrt(std::max(dot_all(b, b), 1e-300));
     double beta = std::sqrt(dot_all(r, r));
     int total = 0;
@@ -788,7 +802,7 @@ int ind_solve(orc_ctx &o, const double *Phi_n, const double *E_expl, const int8_
         for (; j < m && total < max_it; ++j, ++total)
         {
             for (int32_t e = 0; e < ne; ++e) z[e] = dir[e] ? 0.0 : V[j][e] * inv_diag[e];
-            apply_A(o, wk, z, w, dt);
+            apply_A(o, wk, z, w, dt, dte);
             for (int32_t e = 0; e < ne; ++e) if (dir[e]) w[e] = 0.0;
             for (int i = 0; i <= j; ++i)
             {
@@ -823,7 +837,7 @@ int ind_solve(orc_ctx &o, const double *Phi_n, const double *E_expl, const int8_
         }
         Vecd xt(ne);
         for (int32_t e = 0; e < ne; ++e) xt[e] = x0[e] + delta[e];
-        apply_A(o, wk, xt, Ax, dt);
+        apply_A(o, wk, xt, Ax, dt, dte);
         for (int32_t e = 0; e < ne; ++e) r[e] = dir[e] ? 0.0 : b[e] - Ax[e];
         beta = std::sqrt(dot_all(r, r));
     }
@@ -923,7 +937,7 @@ double step(orc_ctx &o, double dt_max, int *iters, double *res)
     std::vector<double> Eout = o.E;
     *iters = 0;
     *res = 0;
-    if (o.resistive || o.hall)
+    if (o.resistive || o.hall || o.ideal) // ImplicitInduction::active()
     {
         std::vector<double> ne_eff(nc, 0.0);
         for (int32_t i = 0; i < nc; ++i)
@@ -1027,9 +1041,28 @@ int orc_ind_apply_F(orc_ctx *o, const double *phi, double *out)
     apply_F(*o, phi, out, wk.part, wk.bcell);
     return 0;
 }
-int orc_ind_solve(orc_ctx *o, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, int32_t *iters, double *rel_res)
+int orc_ind_solve(orc_ctx *o, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, const double *dte, double *E,
+                  int32_t *iters, double *rel_res)
 {
-    *iters = ind_solve(*o, Phi_n, E_expl, dir, dt, E, rel_res);
+    *iters = ind_solve(*o, Phi_n, E_expl, dir, dt, E, rel_res, dte);
+    return 0;
+}
+// ImplicitInduction::set_ideal_state / ideal_linear (ImplicitInduction.cpp:113-125)
+int orc_ind_set_ideal_state(orc_ctx *o, const double *u, const int8_t *mask)
+{
+    o->u_ideal.assign(u, u + 3 * size_t(o->nc));
+    o->ideal_mask.assign(mask, mask + o->ne);
+    return 0;
+}
+int orc_ind_ideal_linear(orc_ctx *o, const double *phi, double *out)
+{
+    const bool h = o->hall, r = o->resistive;
+    o->hall = false;
+    o->resistive = false;
+    IndWork wk(*o);
+    apply_F(*o, phi, out, wk.part, wk.bcell);
+    o->hall = h;
+    o->resistive = r;
     return 0;
 }
 int orc_set_state(orc_ctx *o, const double *UH, const double *UNa, const double *Phi)

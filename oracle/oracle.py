@@ -57,7 +57,8 @@ class Oracle:
                 "orc_div_b": [_P, _P, _P, _P, _P, _P], "orc_local_dt": [_P, _P, _P, _P, _P],
                 "orc_add_sources": [_P, C.c_int, _P, _P, _P, _P, _D, _P],
                 "orc_ind_set_frozen": [_P, _P, _P], "orc_ind_apply_F": [_P, _P, _P],
-                "orc_ind_solve": [_P, _P, _P, _P, _D, _P, _P, _P],
+                "orc_ind_solve": [_P, _P, _P, _P, _D, _P, _P, _P, _P],
+                "orc_ind_set_ideal_state": [_P, _P, _P], "orc_ind_ideal_linear": [_P, _P, _P],
                 "orc_set_state": [_P, _P, _P, _P], "orc_get_state": [_P, _P, _P, _P],
                 "orc_set_background": [_P, _P, _P], "orc_step": [_P, _D, _P, _P, _P],
                 "orc_get_derived": [_P, C.c_int, _P],
@@ -140,12 +141,21 @@ class Oracle:
         self.lib().orc_ind_apply_F(self._h, _ptr(_f(phi)), _ptr(out))
         return out
 
-    def ind_solve(self, Phi_n, E_expl, dirichlet, dt, precision=0):
+    def ind_solve(self, Phi_n, E_expl, dirichlet, dt, dt_edge=None):
         E = np.zeros(self.ne)
         it, rr = C.c_int32(), C.c_double()
         d = np.ascontiguousarray(dirichlet, dtype=np.int8)
-        self.lib().orc_ind_solve(self._h, _ptr(_f(Phi_n)), _ptr(_f(E_expl)), _ptr(d), dt, _ptr(E), C.byref(it), C.byref(rr))
+        self.lib().orc_ind_solve(self._h, _ptr(_f(Phi_n)), _ptr(_f(E_expl)), _ptr(d), dt,
+                                 _ptr(None if dt_edge is None else _f(dt_edge)), _ptr(E), C.byref(it), C.byref(rr))
         return E, it.value, rr.value
+
+    def ind_set_ideal_state(self, u, mask):
+        self.lib().orc_ind_set_ideal_state(self._h, _ptr(_f(u)), _ptr(np.ascontiguousarray(mask, dtype=np.int8)))
+
+    def ind_ideal_linear(self, phi):
+        out = np.zeros(self.ne)
+        self.lib().orc_ind_ideal_linear(self._h, _ptr(_f(phi)), _ptr(out))
+        return out
 
     def set_state(self, U_H, U_Na, Phi):
         self.lib().orc_set_state(self._h, _ptr(_f(U_H)), _ptr(_f(U_Na)), _ptr(_f(Phi)))
@@ -198,7 +208,8 @@ class Reference:
                 "ref_div_b": [_P, _P, _P, _P, _P], "ref_local_dt": [_P, _P, _P, _P],
                 "ref_add_sources": [_P, C.c_int, _P, _P, _P, _P, _P, _D, _P],
                 "ref_ind_set_frozen": [_P, _P, _P], "ref_ind_apply_F": [_P, _P, _P],
-                "ref_ind_solve": [_P, _P, _P, _P, _D, _P, _P], "ref_solver_init": [_P],
+                "ref_ind_solve": [_P, _P, _P, _P, _D, _P, _P, _P], "ref_solver_init": [_P],
+                "ref_ind_set_ideal_state": [_P, _P, _P], "ref_ind_ideal_linear": [_P, _P, _P],
                 "ref_set_state": [_P, _P, _P, _P], "ref_get_state": [_P, _P, _P, _P], "ref_step": [_P, _D, _P],
                 "ref_get_derived": [_P, C.c_int, _P], "ref_last_induction_residual": [_P],
             }.items():
@@ -332,12 +343,21 @@ class Reference:
         self.lib().ref_ind_apply_F(self._h, _ptr(_f(phi)), _ptr(out))
         return out
 
-    def ind_solve(self, Phi_n, E_expl, dirichlet, dt):
+    def ind_solve(self, Phi_n, E_expl, dirichlet, dt, dt_edge=None):
         E = np.zeros(self.ne)
         rr = C.c_double()
         d = np.ascontiguousarray(dirichlet, dtype=np.int8)
-        it = self.lib().ref_ind_solve(self._h, _ptr(_f(Phi_n)), _ptr(_f(E_expl)), _ptr(d), dt, _ptr(E), C.byref(rr))
+        it = self.lib().ref_ind_solve(self._h, _ptr(_f(Phi_n)), _ptr(_f(E_expl)), _ptr(d), dt,
+                                      _ptr(None if dt_edge is None else _f(dt_edge)), _ptr(E), C.byref(rr))
         return E, it, rr.value
+
+    def ind_set_ideal_state(self, u, mask):
+        self.lib().ref_ind_set_ideal_state(self._h, _ptr(_f(u)), _ptr(np.ascontiguousarray(mask, dtype=np.int8)))
+
+    def ind_ideal_linear(self, phi):
+        out = np.zeros(self.ne)
+        self.lib().ref_ind_ideal_linear(self._h, _ptr(_f(phi)), _ptr(out))
+        return out
 
     def solver_init(self):
         if self.lib().ref_solver_init(self._h):

@@ -47,6 +47,7 @@ struct ref_ctx
     std::unique_ptr<mercury::Solver> solver;
     std::vector<char> active;
     std::vector<double> PhiAdd, BaddCell;
+    std::vector<double> u_ideal; // set_ideal_state keeps the pointer: the harness owns the array
     std::string err;
 };
 
@@ -73,6 +74,7 @@ Config make_config(const mhdg_config &c, int dipole, const double *m, const doub
     k.ind_rtol = c.ind_rtol; k.ind_max_it = c.ind_max_it; k.ind_restart = c.ind_restart;
     k.ind_ilu = false;
     k.ideal.cuct = c.cuct;
+    k.implicit_ideal = c.implicit_ideal != 0;
     k.fsH = {c.fs[0][0], c.fs[0][4]};
     k.fsNa = {c.fs[1][0], c.fs[1][4]};
     k.u_inf = {c.u_inf[0], c.u_inf[1], c.u_inf[2]};
@@ -232,11 +234,19 @@ void ref_add_sources(ref_ctx *r, int s, const double *WH, const double *WNa, con
 }
 void ref_ind_set_frozen(ref_ctx *r, const double *B0, const double *ne) { r->ind->set_frozen_state(B0, ne); }
 void ref_ind_apply_F(ref_ctx *r, const double *phi, double *out) { r->ind->apply_F(phi, out); }
-int ref_ind_solve(ref_ctx *r, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res)
+int ref_ind_solve(ref_ctx *r, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, const double *dte, double *E,
+                  double *rel_res)
 {
     std::vector<char> d(dir, dir + r->lm.mesh.cx.n_edges());
-    return r->ind->solve(Phi_n, E_expl, d, dt, E, rel_res);
+    return r->ind->solve(Phi_n, E_expl, d, dt, E, rel_res, dte);
 }
+void ref_ind_set_ideal_state(ref_ctx *r, const double *u, const int8_t *mask)
+{
+    r->u_ideal.assign(u, u + 3 * size_t(r->lm.mesh.cx.n_cells()));
+    std::vector<char> m(mask, mask + r->lm.mesh.cx.n_edges());
+    r->ind->set_ideal_state(r->u_ideal.data(), m);
+}
+void ref_ind_ideal_linear(ref_ctx *r, const double *phi, double *out) { r->ind->ideal_linear(phi, out); }
 
 // ---- full step through mercury::Solver (MercuryUns/driver/Solver.cpp:228-277)
 int ref_solver_init(ref_ctx *r)

@@ -182,3 +182,74 @@ def test_step_mixed_vs_fp64(small_hall):
         assert maxrel(y, x) < 1e-8
     _, rel, _ = b.div_b(b.get_state()[2], w["background"][0])
     assert rel < 1e-12
+
+
+def test_implicit_ideal_and_edge_dt():
+    """set_ideal_state / ideal_linear (bitwise: the same face and edge EMF kernels as the explicit stage) and the
+    per-edge pseudo-time solve, FP64 and mixed."""
+    from mpcns_uns_amd import Device, workloads
+    from oracle.oracle import Oracle
+
+    w = workloads.build(net=(12, 48, 3), hall=True, implicit_ideal=1)
+    m = w["mesh"]
+    dev, orc = Device(m, w["cfg"]), Oracle(m, w["cfg"])
+    assert dev.ind_flags() == (True, True, True)
+    rng = np.random.default_rng(12)
+    B0 = np.tile([0.1, -0.2, 0.3], m.n_cells) + 0.05 * rng.normal(size=3 * m.n_cells)
+    ne_eff = 1.0 + 0.1 * rng.uniform(size=m.n_cells)
+    u = rng.uniform(-1, 1, 3 * m.n_cells)
+    mask = (m.array("edge_mode") == 0).astype(np.int8)
+    phi = rng.normal(size=m.n_faces)
+    for x in (dev, orc):
+        x.ind_set_frozen(B0, ne_eff)
+        x.ind_set_ideal_state(u, mask)
+    L = orc.ind_ideal_linear(phi)
+    assert np.abs(L).max() > 0
+    assert np.array_equal(dev.ind_ideal_linear(phi), L)
+    assert np.array_equal(dev.ind_apply_F(phi), orc.ind_apply_F(phi))
+    dirichlet = (m.array("edge_mode") != 0).astype(np.int8)
+    E_expl = rng.normal(size=m.n_edges) * 1e-3
+    dte = 2e-3 * rng.uniform(0.5, 2.0, m.n_edges)
+    Ed, itd, rd = dev.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, dt_edge=dte)
+    Eo, ito, ro = orc.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, dt_edge=dte)
+    assert rd < 1e-8 and ro < 1e-8 and abs(itd - ito) <= 1
+    assert maxrel(Ed, Eo) < 1e-7
+    with pytest.raises(RuntimeError, match="ideal"):
+        dev.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, precision=1)
+    # without the ideal part the mixed solve supports dt_edge too
+    w2 = workloads.build(net=(12, 48, 3), hall=True)
+    a, b = Device(w2["mesh"], w2["cfg"]), Device(w2["mesh"], w2["cfg"])
+    for x in (a, b):
+        x.ind_set_frozen(B0, ne_eff)
+    E64, _, r64 = a.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, precision=0, dt_edge=dte)
+    E32, _, r32 = b.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, precision=1, dt_edge=dte)
+    assert r64 < 1e-8 and r32 < 1e-8 and maxrel(E32, E64) < 1e-6
+
+
+def test_fused_step_entries(small_hall):
+    """mhdg_step_explicit on a Hall configuration is the ideal step of the same configuration without the implicit
+    part (bitwise against the oracle with Hall and resistivity off); mhdg_step_hall is mhdg_step."""
+    from mpcns_uns_amd import Device
+    from oracle.oracle import Oracle
+
+    w = small_hall
+    cfg_off = type(w["cfg"]).from_buffer_copy(w["cfg"])
+    cfg_off.hall, cfg_off.resistive = 0, 0
+    dev = Device(w["mesh"], w["cfg"])
+    orc = Oracle(w["mesh"], cfg_off)
+    for x in (dev, orc):
+        x.set_background(*w["background"])
+        x.set_state(*w["state"])
+    assert dev.step_explicit() == orc.step()[0]
+    for a, b in zip(dev.get_state(), orc.get_state()):
+        assert np.array_equal(a, b)
+    ref = Device(w["mesh"], w["cfg"])
+    for x in (dev, ref):
+        x.set_background(*w["background"])
+        x.set_state(*w["state"])
+    assert dev.step_hall(precision=0) == ref.step(precision=0)
+    for a, b in zip(dev.get_state(), ref.get_state()):
+        assert np.array_equal(a, b)
+    off = Device(w["mesh"], cfg_off)
+    with pytest.raises(RuntimeError, match="step_hall"):
+        off.step_hall()

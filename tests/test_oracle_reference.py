@@ -16,11 +16,11 @@ GEOMETRY = ("edge_vec", "edge_mid", "edge_len", "face_area", "face_center", "fac
             "edge_dual_area", "ec_ptr", "ec_cell", "ec_weight", "ef_weight", "star1", "star2", "star3")
 
 
-def _shell(hall, wall_dirichlet=0):
+def _shell(hall, wall_dirichlet=0, **cfg_kw):
     from mpcns_uns_amd import workloads
     from oracle.oracle import reference_for_mesh
 
-    w = workloads.build(net=(12, 48, 3), hall=hall, wall_dirichlet=wall_dirichlet)
+    w = workloads.build(net=(12, 48, 3), hall=hall, wall_dirichlet=wall_dirichlet, **cfg_kw)
     ref = reference_for_mesh(w["mesh"], w["cfg"], dipole_m=workloads.DIPOLE_M)
     return w, ref
 
@@ -124,3 +124,35 @@ def test_full_steps_bitwise(hall):
             assert np.array_equal(a, b)
         for n in ("W_H", "W_Na", "B_ind", "B_tot", "J"):
             assert np.array_equal(ref.derived(n), orc.derived(n))
+
+
+def test_implicit_ideal_and_edge_dt_bitwise():
+    """ImplicitInduction with the implicit ideal part (set_ideal_state, ideal_linear; ImplicitInduction.cpp:113-157) and
+    the per-edge pseudo-time solve (dt_edge; ImplicitInduction.cpp:185-198, 299-318), oracle against reference."""
+    from oracle.oracle import Oracle
+
+    w, ref = _shell(True, implicit_ideal=1)
+    m = w["mesh"]
+    orc = Oracle(m, w["cfg"])
+    rng = np.random.default_rng(12)
+    B0 = np.tile([0.1, -0.2, 0.3], m.n_cells) + 0.05 * rng.normal(size=3 * m.n_cells)
+    ne_eff = 1.0 + 0.1 * rng.uniform(size=m.n_cells)
+    u = rng.uniform(-1, 1, 3 * m.n_cells)
+    mask = (m.array("edge_mode") == 0).astype(np.int8)
+    phi = rng.normal(size=m.n_faces)
+    for x in (orc, ref):
+        x.ind_set_frozen(B0, ne_eff)
+        assert not np.any(x.ind_ideal_linear(phi))  # no u yet: the ideal part is off
+        x.ind_set_ideal_state(u, mask)
+    L = ref.ind_ideal_linear(phi)
+    assert np.abs(L).max() > 0
+    assert np.array_equal(orc.ind_ideal_linear(phi), L)
+    assert np.array_equal(orc.ind_apply_F(phi), ref.ind_apply_F(phi))
+    dirichlet = (m.array("edge_mode") != 0).astype(np.int8)
+    E_expl = rng.normal(size=m.n_edges) * 1e-3
+    dte = 2e-3 * rng.uniform(0.5, 2.0, m.n_edges)
+    for dt_edge in (dte, None, dte):  # warm starts carry over between the solves in both
+        Eo, ito, ro = orc.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, dt_edge=dt_edge)
+        Er, itr, rr = ref.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, dt_edge=dt_edge)
+        assert ito == itr and ro == rr and np.array_equal(Eo, Er)
+        assert rr < w["cfg"].ind_rtol

@@ -99,6 +99,8 @@ int mhdg_cell_current(mhdg_ctx *ctx, const double *B, double *J);
  * ghost_state(); it replaces the GhostFn callback (Reconstruction.h:17-21). */
 int mhdg_limited_gradients(mhdg_ctx *ctx, const double *W, int species, const double fs[5], int limit,
                            const double *ghost, double *grad);
+/* mhd::Stencil::set_rel_limit (Common/fluid/Reconstruction.h:27): theta of the relative limiter. */
+int mhdg_set_recon_rel_limit(mhdg_ctx *ctx, double theta);
 /* mhd::fluid_flux_rhs (Common/fluid/FluxRHS.cpp:30-74): W 5C, grad 15C -> rhs 5C. */
 int mhdg_flux_rhs(mhdg_ctx *ctx, int species, const double fs[5], const double *W, const double *grad,
                   const double *ghost, double *rhs);

@@ -630,6 +630,14 @@ int mhdg_ind_apply_F(mhdg_ctx *c, const double *phi, double *out)
         c->d2h(out, c->tmp, size_t(c->ne) * 8);
     })
 }
+int mhdg_set_recon_rel_limit(mhdg_ctx *c, double theta)
+{
+    MHDG_TRY(c, {
+        if (!(theta >= 0)) throw std::runtime_error("recon_rel_limit must be >= 0");
+        c->P.rel_limit = theta;
+        c->cfg.recon_rel_limit = theta;
+    })
+}
 int mhdg_ind_flags(mhdg_ctx *c, int32_t *active, int32_t *has_ideal, int32_t *has_hall)
 {
     MHDG_TRY(c, {

@@ -14,6 +14,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "_build", "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libmpcns_ref.so")
+# the reference's drivers linked against the device library through the C++ shim (mpcns_uns_amd/shim)
+REF_GPU_LIB = os.path.join(os.path.dirname(REF_LIB), "libmpcns_ref_gpu.so")
 
 _P, _D = C.c_void_p, C.c_double
 
@@ -31,6 +33,10 @@ def _f(a):
 
 def have_ref():
     return os.path.exists(REF_LIB)
+
+
+def have_ref_on_device():
+    return os.path.exists(REF_GPU_LIB)
 
 
 class Oracle:
@@ -186,13 +192,14 @@ class Reference:
     """The compiled reference (oracle/_ref) on a mesh given as cells of outward loops."""
 
     _lib = None
+    _path = REF_LIB
 
     @classmethod
     def lib(cls):
         if cls._lib is None:
-            if not have_ref():
-                raise RuntimeError("reference library not built (make -C oracle ref)")
-            L = C.CDLL(REF_LIB)
+            if not os.path.exists(cls._path):
+                raise RuntimeError(f"{cls._path} not built (make -C oracle ref)")
+            L = C.CDLL(cls._path)
             L.ref_create.argtypes = [C.c_int32, _P, C.c_int64, _P, _P, _P, _P, C.c_int64, _P, _P, _P, C.c_int,
                                      _P, C.c_int, _P, _P, _D, C.POINTER(_P)]
             L.ref_destroy.argtypes = [_P]
@@ -392,7 +399,15 @@ def tet_cell_loops(tets):
     return out
 
 
-def reference_for_mesh(mesh, cfg, dipole_m=None, dipole_c=(0, 0, 0), soft=1e-3):
+class ReferenceOnDevice(Reference):
+    """The reference's drivers (mercury::Solver, harness) with the mhd_common calls of the hot path served by the
+    device library through the C++ shim (mpcns_uns_amd/shim/mhd_gpu_shim.cpp): the drop-in configuration."""
+
+    _lib = None
+    _path = REF_GPU_LIB
+
+
+def reference_for_mesh(mesh, cfg, dipole_m=None, dipole_c=(0, 0, 0), soft=1e-3, on_device=False):
     """Reference objects on the same tetrahedral mesh (same node order, cell order and loops)."""
     xyz = mesh.array("xyz")
     tets = mesh.array("tets")
@@ -404,4 +419,5 @@ def reference_for_mesh(mesh, cfg, dipole_m=None, dipole_c=(0, 0, 0), soft=1e-3):
         es = f2e_idx[f2e_ptr[f]:f2e_ptr[f + 1]]
         nodes = sorted(set(int(x) for x in e2n_idx[es].ravel()))
         tags.append((nodes, int(ftag[f])))
-    return Reference(xyz, tet_cell_loops(tets), cfg, tags=tags, dipole_m=dipole_m, dipole_c=dipole_c, soft=soft)
+    cls = ReferenceOnDevice if on_device else Reference
+    return cls(xyz, tet_cell_loops(tets), cfg, tags=tags, dipole_m=dipole_m, dipole_c=dipole_c, soft=soft)

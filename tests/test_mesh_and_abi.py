@@ -112,3 +112,20 @@ def test_create_reports_errors_without_device():
     d, cfg, h = MeshDesc(), default_config(), C.c_void_p()
     assert lib().mhdg_create(C.byref(d), C.byref(cfg), C.byref(h)) != 0
     assert lib().mhdg_last_error(None)
+
+
+def test_shim_compiles_against_reference_headers():
+    """The C++ drop-in shim keeps the reference's mhd_common signatures: it compiles against the reference's own
+    headers (where the reference sources are present)."""
+    import shutil
+    import subprocess
+
+    ref = "/root/reference"
+    if not os.path.isdir(os.path.join(ref, "Common")):
+        pytest.skip("reference sources not present")
+    if not shutil.which("g++"):
+        pytest.skip("no g++")
+    cmd = ["g++", "-std=c++20", "-fsyntax-only", f"-I{ref}/include", f"-I{ref}/Common", "-I/opt/conda/include",
+           "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "mpcns_uns_amd", "shim", "mhd_gpu_shim.cpp")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]

@@ -141,6 +141,20 @@ int mhdg_ind_solve(mhdg_ctx *ctx, const double *Phi_n, const double *E_expl, con
                    double dt, const double *dt_edge, int precision, double *E, int32_t *iterations,
                    double *rel_res);
 
+/* --------------------------------------------- nonlinear Hall induction: tangent-linear and Newton-Krylov */
+/* No counterpart in the reference (it freezes the Hall field at B_tot^n, ImplicitInduction.cpp:105-111).
+ * G(phi) is apply_F with the Hall field B0 = B_add + R phi of the iterate, R the recon map of
+ * ImplicitInduction.cpp:150-160 (quadratic in phi).  mhdg_ind_jvp gives G'(phi_lin) dphi (hand-written
+ * tangent kernel); mhdg_ind_newton solves E - E_expl - G(Phi_n - dt d1 E) = 0 on the non-Dirichlet edges by
+ * Newton's method, each step by FP64 GMRES (Jacobi, MGS) on J = I + dt G'(Phi_k) d1 with relative forcing
+ * `eta`, until |R| <= rtol |E_expl + G(Phi_n)|.  E0 (nullable, E values) is the initial iterate (default
+ * E_expl).  The frozen alpha comes from mhdg_ind_set_frozen; the background B_add from mhdg_set_background. */
+int mhdg_ind_apply_G(mhdg_ctx *ctx, const double *phi, double *out);
+int mhdg_ind_jvp(mhdg_ctx *ctx, const double *phi_lin, const double *dphi, double *out);
+int mhdg_ind_newton(mhdg_ctx *ctx, const double *Phi_n, const double *E_expl, const int8_t *dirichlet, double dt,
+                    const double *E0, double rtol, int max_newton, double eta, double *E, int32_t *newton_its,
+                    int32_t *gmres_its, double *rel_res);
+
 /* --------------------------------------------------------------- device-resident step (fused path) */
 /* State: U_H, U_Na (5C each, AoS), induced flux Phi (F); background PhiAdd (F) and BaddCell (3C). */
 int mhdg_set_state(mhdg_ctx *ctx, const double *U_H, const double *U_Na, const double *Phi);

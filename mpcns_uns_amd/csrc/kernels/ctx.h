@@ -34,6 +34,11 @@ struct mhdg_ctx
     double *u_ideal = nullptr, *ide = nullptr, *Ef_i = nullptr; // implicit ideal part (set_ideal_state)
     int8_t *imask = nullptr;
     const double *dte_cur = nullptr;                              // per-edge pseudo-time step of the current solve
+    // Newton-Krylov on the nonlinear Hall induction (tangent.hip)
+    double *lin_b = nullptr;           // cell field at the linearisation point: apply_op then applies the tangent operator
+    double rtol_override = 0;          // GMRES tolerance of a Newton step (0: cfg.ind_rtol)
+    struct NewtonWork { double *phik = nullptr, *G = nullptr, *R = nullptr, *rhs = nullptr, *delta = nullptr, *E = nullptr,
+                        *lin_b = nullptr, *B0 = nullptr, *zero_phi = nullptr, *corr_save = nullptr; } nt;
     std::vector<double *> V;
     double **Vptrs = nullptr;
     int restart = 2;
@@ -69,4 +74,9 @@ struct mhdg_ctx
     void mixed_build();
     int solve_mixed(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
     double step(double dt_max, int precision, int *iters, double *res);
+    void newton_alloc();
+    void apply_G(const double *phi, double *out);
+    void jvp(const double *phi_lin, const double *dphi, double *out);
+    int newton(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, const double *E0, double rtol, int max_newton,
+               double eta, double *E, int *gmres_total, double *rel_res);
 };

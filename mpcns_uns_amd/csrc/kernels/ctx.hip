@@ -238,7 +238,8 @@ void mhdg_ctx::apply_op(const double *phi_in, const double *v, const double *inv
     }
     const bool ideal_part = ideal_cfg && u_ideal;
     if (op_hall || ideal_part) LAUNCH("bcell", k_bcell, dim3(nblk(nc)), dim3(256), 0, stream, d, phi, bcell);
-    if (op_hall) LAUNCH("hall_g", k_hall_g, dim3(nblk(nc)), dim3(256), 0, stream, d, bcell, alpha, B0, gcell);
+    if (op_hall && lin_b) LAUNCH("hall_g_tangent", k_hall_g_tangent, dim3(nblk(nc)), dim3(256), 0, stream, d, lin_b, bcell, alpha, Badd, gcell);
+    else if (op_hall) LAUNCH("hall_g", k_hall_g, dim3(nblk(nc)), dim3(256), 0, stream, d, bcell, alpha, B0, gcell);
     if (ideal_part)
     {
         // add_ideal_emf of (u, b, phi) with b as total and induced field (ImplicitInduction.cpp:150-157)
@@ -262,7 +263,7 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
 {
     const int m = restart;
     const int max_it = cfg.ind_max_it;
-    const double rtol = cfg.ind_rtol;
+    const double rtol = rtol_override > 0 ? rtol_override : cfg.ind_rtol;
     LAUNCH("solve_init", k_solve_init, dim3(nblk(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt,
            dte_cur, ne, x0, invdiag);
     apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp); // F(Phi^n)
@@ -691,6 +692,42 @@ int mhdg_ind_solve(mhdg_ctx *c, const double *Phi_n, const double *E_expl, const
         c->dte_cur = nullptr;
         c->d2h(E, eo, size_t(c->ne) * 8);
         *iterations = it;
+        if (rel_res) *rel_res = rr;
+    })
+}
+int mhdg_ind_apply_G(mhdg_ctx *c, const double *phi, double *out)
+{
+    MHDG_TRY(c, {
+        const double *p = c->stage_in(0, phi, c->nf);
+        c->apply_G(p, c->tmp);
+        c->d2h(out, c->tmp, size_t(c->ne) * 8);
+    })
+}
+int mhdg_ind_jvp(mhdg_ctx *c, const double *phi_lin, const double *dphi, double *out)
+{
+    MHDG_TRY(c, {
+        const double *p = c->stage_in(0, phi_lin, c->nf), *q = c->stage_in(1, dphi, c->nf);
+        c->jvp(p, q, c->tmp);
+        c->d2h(out, c->tmp, size_t(c->ne) * 8);
+    })
+}
+int mhdg_ind_newton(mhdg_ctx *c, const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, const double *E0, double rtol,
+                    int max_newton, double eta, double *E, int32_t *newton_its, int32_t *gmres_its, double *rel_res)
+{
+    MHDG_TRY(c, {
+        if (c->ideal_cfg && c->u_ideal) throw std::runtime_error("newton: the implicit ideal part is not supported");
+        if (!(rtol > 0) || !(eta > 0) || max_newton < 0) throw std::runtime_error("newton: rtol, eta > 0 and max_newton >= 0 required");
+        const double *p = c->stage_in(0, Phi_n, c->nf), *ee = c->stage_in(1, E_expl, c->ne);
+        int8_t *dd = reinterpret_cast<int8_t *>(c->stage(2, (size_t(c->ne) + 7) / 8));
+        hip_check(hipMemcpyAsync(dd, dir, size_t(c->ne), hipMemcpyHostToDevice, c->stream), "H2D");
+        const double *e0 = E0 ? c->stage_in(4, E0, c->ne) : nullptr;
+        double *eo = c->stage(3, c->ne);
+        int gm = 0;
+        double rr = 0;
+        const int k = c->newton(p, ee, dd, dt, e0, rtol, max_newton, eta, eo, &gm, &rr);
+        c->d2h(E, eo, size_t(c->ne) * 8);
+        if (newton_its) *newton_its = k;
+        if (gmres_its) *gmres_its = gm;
         if (rel_res) *rel_res = rr;
     })
 }

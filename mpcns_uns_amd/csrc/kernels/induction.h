@@ -30,6 +30,11 @@ __global__ void k_gmres_update(const double *const *V, const double *y, int32_t 
 __global__ void k_solve_init(const double *E_expl, const double *corr, const int8_t *dir, const double *m_e, const double *d2, double dt,
                              const double *dte, int32_t n, double *x0, double *invdiag);
 __global__ void k_add(const double *a, const double *b, int32_t n, double *out);
+// tangent.hip
+__global__ void k_field_of_flux(const double *Badd, const double *b, int64_t n, double *B0);
+__global__ void k_hall_g_tangent(DevMesh d, const double *b, const double *db, const double *alpha, const double *Badd, double *g);
+__global__ void k_newton_residual(const double *E, const double *E_expl, const double *G, const int8_t *dir, int32_t n, double *R);
+__global__ void k_scaled(const double *x, double s, int32_t n, double *out);
 __global__ void k_solve_final(const double *x0, const double *delta, const double *E_expl, const int8_t *dir, int32_t n, double *E,
                               double *corr);
 } // namespace mhdg

@@ -123,6 +123,27 @@ class Device:
                                       ptr(None if dt_edge is None else f64(dt_edge)), precision, ptr(E), C.byref(it), C.byref(rr)))
         return E, it.value, rr.value
 
+    def ind_apply_G(self, phi):
+        """Induction operator with the Hall field B_add + R phi of phi itself (quadratic in phi)."""
+        out = np.zeros(self.ne)
+        self._ok(lib().mhdg_ind_apply_G(self._h, ptr(f64(phi)), ptr(out)))
+        return out
+
+    def ind_jvp(self, phi_lin, dphi):
+        """Tangent-linear product G'(phi_lin) dphi."""
+        out = np.zeros(self.ne)
+        self._ok(lib().mhdg_ind_jvp(self._h, ptr(f64(phi_lin)), ptr(f64(dphi)), ptr(out)))
+        return out
+
+    def ind_newton(self, Phi_n, E_expl, dirichlet, dt, E0=None, rtol=1e-8, max_newton=20, eta=1e-4):
+        """Newton-Krylov solve of the nonlinear Hall induction step: (E, newton steps, GMRES iterations, rel. residual)."""
+        E = np.zeros(self.ne)
+        k, g, rr = C.c_int32(), C.c_int32(), C.c_double()
+        d = np.ascontiguousarray(dirichlet, dtype=np.int8)
+        self._ok(lib().mhdg_ind_newton(self._h, ptr(f64(Phi_n)), ptr(f64(E_expl)), ptr(d), dt, ptr(None if E0 is None else f64(E0)),
+                                       rtol, max_newton, eta, ptr(E), C.byref(k), C.byref(g), C.byref(rr)))
+        return E, k.value, g.value, rr.value
+
     # ------------------------------------------------------------------ resident state
     def set_state(self, U_H, U_Na, Phi):
         self._ok(lib().mhdg_set_state(self._h, ptr(f64(U_H)), ptr(f64(U_Na)), ptr(f64(Phi))))

@@ -16,7 +16,8 @@ struct mhdg_ctx
     mhdg::Params P{};
     mhdg::DevMesh d{};
     int32_t nc = 0, nf = 0, ne = 0, n_heavy = 0;
-    size_t c2f_nnz = 0, ec_nnz = 0;
+    size_t c2f_nnz = 0, ec_nnz = 0, e2f_nnz = 0;
+    int32_t cdeg = 0, fdeg = 0; // uniform faces per cell / edges per face (0: mixed shapes)
     void *heavy = nullptr;
     std::vector<double> taper_h;
     int8_t *dir_mercury = nullptr;

@@ -153,6 +153,15 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
     n_heavy = static_cast<int32_t>(t.heavy.size());
     c2f_nnz = t.c2f.size();
     ec_nnz = t.ec_cell.size();
+    e2f_nnz = t.e2f.size();
+    auto uniform = [](const std::vector<int32_t> &ptr) {
+        const int32_t deg = ptr.size() > 1 ? ptr[1] - ptr[0] : 0;
+        for (size_t r = 1; r < ptr.size(); ++r)
+            if (ptr[r] - ptr[r - 1] != deg) return 0;
+        return deg;
+    };
+    cdeg = uniform(t.c2f_ptr);
+    fdeg = uniform(t.f2e_ptr);
     if (c.ind_restart > 60) throw std::runtime_error("ind_restart above 60 is not supported");
     const size_t E = ne, F = nf, C = nc;
     d.nc = nc; d.nf = nf; d.ne = ne;

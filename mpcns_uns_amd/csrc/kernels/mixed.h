@@ -11,24 +11,28 @@ namespace mhdg
 // FP32 copies of the metric arrays the operator reads
 struct DevMesh32
 {
-    const float *star2, *farea, *cvol, *recon, *ecw, *evec, *m_e, *invdiag;
+    const float *star2, *m_e, *recon, *ecw, *evec, *invdiag;
     float *alpha, *B0;
+    const float *aV;          // per (cell, face) entry: s A_f / V_c (3)
+    const int32_t *nbr;       // per (cell, face) entry: the cell across the face, -1 on the boundary
+    int32_t cdeg, fdeg;       // faces per cell / edges per face when uniform (tets: 4, 3), else 0 (ptr arrays)
 };
 
 constexpr int kDotChunk = 16; // basis vectors per multi-dot pass
 
-__global__ void k_d1_f32(DevMesh d, const float *v, const float *invdiag, const int8_t *dir, const float *dte, float *phi);
+__global__ void k_d1_f32(DevMesh d, DevMesh32 s, const float *pv, const float *dte, float *phi);
 __global__ void k_bcell_f32(DevMesh d, DevMesh32 s, const float *phi, float *b);
 __global__ void k_hall_g_f32(DevMesh d, DevMesh32 s, const float *b, float *g);
 __global__ void k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi, const float *g, const float *v,
-                                 const int8_t *dir, float dt, float *out);
+                                 const float *pv, const int8_t *dir, float dt, float *out);
 __global__ void k_edge_apply_f32_heavy(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const int32_t *heavy, const float *phi,
-                                       const float *g, const float *v, const int8_t *dir, float dt, float *out);
+                                       const float *g, const float *v, const float *pv, const int8_t *dir, float dt, float *out);
+__global__ void k_tables_f32(DevMesh d, float *aV, int32_t *nbr);
 __global__ void k_multidot_f32(const float *const *V, int32_t i0, int32_t nv, const float *w, int32_t n, double *part, int32_t with_ww,
                                RedScratch rs, double *out);
 __global__ void k_multiaxpy_f32(const float *const *V, int32_t nv, const double *h, float *w, int32_t n, RedScratch rs, double *nrm);
-__global__ void k_to_f32_scaled(const double *x, const double *s, int32_t n, float *out);
-__global__ void k_scale_f32(const float *w, const double *h, int32_t n, float *out);
+__global__ void k_to_f32_scaled(const double *x, const double *s, const float *invdiag, const int8_t *dir, int32_t n, float *out, float *pv);
+__global__ void k_scale_f32(const float *w, const double *h, const float *invdiag, const int8_t *dir, int32_t n, float *out, float *pv);
 __global__ void k_update_mixed(const float *const *V, const double *y, int32_t j, const float *invdiag, const int8_t *dir, int32_t n,
                                double *delta);
 __global__ void k_f64_to_f32(const double *x, int64_t n, float *out);
@@ -37,7 +41,7 @@ __global__ void k_alpha_f32(const double *alpha, const double *B0, int32_t nc, f
 struct MixedSolver
 {
     DevMesh32 s{};
-    float *phi = nullptr, *b = nullptr, *g = nullptr, *w = nullptr, *dte32 = nullptr;
+    float *phi = nullptr, *b = nullptr, *g = nullptr, *w = nullptr, *dte32 = nullptr, *pv = nullptr;
     std::vector<float *> V;
     float **Vptrs = nullptr;
     double *h = nullptr, *part = nullptr;

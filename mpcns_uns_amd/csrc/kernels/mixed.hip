@@ -27,27 +27,39 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 } // namespace
 
-__global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, const float *v, const float *invdiag, const int8_t *dir, const float *dte,
-                                                float *phi)
+// Row range of a CSR table: fixed degree (deg > 0, every row the same length) or the ptr array
+__device__ __forceinline__ void rows(int32_t deg, const int32_t *ptr, int32_t r, int32_t &k0, int32_t &k1)
+{
+    if (deg > 0) { k0 = r * deg; k1 = k0 + deg; }
+    else { k0 = ptr[r]; k1 = ptr[r + 1]; }
+}
+
+// Phi'_f = sum_k f2e_k pv_e with pv = P v (0 on Dirichlet edges), times dt_e with a per-edge step
+__global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, DevMesh32 s, const float *pv, const float *dte, float *phi)
 {
     const int32_t f = gid();
     if (f >= d.nf) return;
-    float s = 0.0f;
-    for (int32_t k = d.f2e_ptr[f]; k < d.f2e_ptr[f + 1]; ++k)
+    int32_t k0, k1;
+    rows(s.fdeg, d.f2e_ptr, f, k0, k1);
+    float sum = 0.0f;
+    for (int32_t k = k0; k < k1; ++k)
     {
         const int32_t q = d.f2e[k], e = pid(q);
-        const float x = dir[e] ? 0.0f : v[e] * invdiag[e];
-        s += fsg(q) * (dte ? dte[e] * x : x);
+        const float x = pv[e];
+        sum += fsg(q) * (dte ? dte[e] * x : x);
     }
-    phi[f] = s;
+    phi[f] = sum;
 }
 
+// b_c = sum_k recon_k Phi'_{f_k}
 __global__ void __launch_bounds__(256) k_bcell_f32(DevMesh d, DevMesh32 s, const float *phi, float *b)
 {
     const int32_t c = gid();
     if (c >= d.nc) return;
+    int32_t k0, k1;
+    rows(s.cdeg, d.c2f_ptr, c, k0, k1);
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+    for (int32_t k = k0; k < k1; ++k)
     {
         const float v = phi[pid(d.c2f[k])];
         bx += s.recon[3 * size_t(k)] * v; by += s.recon[3 * size_t(k) + 1] * v; bz += s.recon[3 * size_t(k) + 2] * v;
@@ -55,6 +67,8 @@ __global__ void __launch_bounds__(256) k_bcell_f32(DevMesh d, DevMesh32 s, const
     b[3 * size_t(c)] = bx; b[3 * size_t(c) + 1] = by; b[3 * size_t(c) + 2] = bz;
 }
 
+// g_c = alpha_c J_c x B0_c with J_c = sum_k (s A_f / V_c)_k x b_f, b_f the face average with the neighbour
+// (per-(cell, face) table: coalesced, no face-indexed gathers but the neighbour's b)
 __global__ void __launch_bounds__(256) k_hall_g_f32(DevMesh d, DevMesh32 s, const float *b, float *g)
 {
     const int32_t c = gid();
@@ -63,43 +77,42 @@ __global__ void __launch_bounds__(256) k_hall_g_f32(DevMesh d, DevMesh32 s, cons
     F3 out{0.0f, 0.0f, 0.0f};
     if (a != 0.0f)
     {
-        F3 sum{0.0f, 0.0f, 0.0f};
+        int32_t k0, k1;
+        rows(s.cdeg, d.c2f_ptr, c, k0, k1);
         const F3 bc = f3(b, c);
-        for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+        F3 J{0.0f, 0.0f, 0.0f};
+        for (int32_t k = k0; k < k1; ++k)
         {
-            const int32_t v = d.c2f[k], f = pid(v);
+            const int32_t nb = s.nbr[k];
             F3 bf = bc;
-            const int32_t l = d.fL[f], r = d.fR[f];
-            if (l >= 0 && r >= 0)
+            if (nb >= 0)
             {
-                const F3 bo = f3(b, l == c ? r : l);
+                const F3 bo = f3(b, nb);
                 bf = {0.5f * (bc.x + bo.x), 0.5f * (bc.y + bo.y), 0.5f * (bc.z + bo.z)};
             }
-            const F3 A = f3(s.farea, f);
-            const float sg = fsg(v);
-            const F3 c3 = fcross({sg * A.x, sg * A.y, sg * A.z}, bf);
-            sum.x += c3.x; sum.y += c3.y; sum.z += c3.z;
+            const F3 c3 = fcross(f3(s.aV, k), bf);
+            J.x += c3.x; J.y += c3.y; J.z += c3.z;
         }
-        const float iv = 1.0f / s.cvol[c];
-        const F3 J{iv * sum.x, iv * sum.y, iv * sum.z};
         const F3 q = fcross(J, f3(s.B0, c));
         out = {a * q.x, a * q.y, a * q.z};
     }
     g[3 * size_t(c)] = out.x; g[3 * size_t(c) + 1] = out.y; g[3 * size_t(c) + 2] = out.z;
 }
 
-// out = dir ? 0 : (A P - I) v = (x - v) + dt F(phi) with x = v * invdiag (the shifted operator: same Krylov
-// space as A P, and the O(1) identity part never enters the FP32 vector)
+// out = dir ? 0 : (A P - I) v = (pv - v) + dt F(Phi') with F = m_e sum_f s star2_f Phi'_f + sum_c w_ec g_c . t_e
+// (the shifted operator: same Krylov space as A P, and the O(1) identity part never enters the FP32 vector)
 __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi,
-                                                        const float *g, const float *v, const int8_t *dir, float dt, float *out)
+                                                        const float *g, const float *v, const float *pv, const int8_t *dir, float dt,
+                                                        float *out)
 {
     const int32_t e = gid();
     if (e >= d.ne) return;
     if (dir[e]) { out[e] = 0.0f; return; }
-    if (d.e2f_ptr[e + 1] - d.e2f_ptr[e] > kLightMax) return; // k_edge_apply_f32_heavy
+    const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
+    if (f1 - f0 > kLightMax) return; // k_edge_apply_f32_heavy
     float p0 = 0.0f, p1 = 0.0f;
     if (resistive)
-        for (int32_t k = d.e2f_ptr[e]; k < d.e2f_ptr[e + 1]; ++k)
+        for (int32_t k = f0; k < f1; ++k)
         {
             const int32_t q = d.e2f[k], f = pid(q);
             p0 += fsg(q) * (s.star2[f] * phi[f]);
@@ -107,19 +120,15 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, 
     if (hall)
     {
         const F3 t = f3(s.evec, e);
-        for (int32_t k = d.ec_ptr[e]; k < d.ec_ptr[e + 1]; ++k)
-        {
-            const float w = s.ecw[k];
-            p1 += w * fdot(f3(g, d.ec_cell[k]), t);
-        }
+        for (int32_t k = d.ec_ptr[e]; k < d.ec_ptr[e + 1]; ++k) p1 += s.ecw[k] * fdot(f3(g, d.ec_cell[k]), t);
     }
-    out[e] = (v[e] * s.invdiag[e] - v[e]) + dt * (s.m_e[e] * p0 + p1);
+    out[e] = (pv[e] - v[e]) + dt * (s.m_e[e] * p0 + p1);
 }
 
 // The same for the high-valence edges, one block per edge (tree sums: order-free in FP32)
 __global__ void __launch_bounds__(256) k_edge_apply_f32_heavy(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const int32_t *heavy,
-                                                              const float *phi, const float *g, const float *v, const int8_t *dir, float dt,
-                                                              float *out)
+                                                              const float *phi, const float *g, const float *v, const float *pv,
+                                                              const int8_t *dir, float dt, float *out)
 {
     __shared__ float sh[2][4];
     const int32_t e = heavy[blockIdx.x];
@@ -141,7 +150,27 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32_heavy(DevMesh d, DevMesh
     if (threadIdx.x == 0)
     {
         const float P0 = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3], P1 = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
-        out[e] = dir[e] ? 0.0f : (v[e] * s.invdiag[e] - v[e]) + dt * (s.m_e[e] * P0 + P1);
+        out[e] = dir[e] ? 0.0f : (pv[e] - v[e]) + dt * (s.m_e[e] * P0 + P1);
+    }
+}
+
+// Operator table of the FP32 path: per (cell, face) s A_f / V_c and the neighbour across the face
+__global__ void __launch_bounds__(256) k_tables_f32(DevMesh d, float *aV, int32_t *nbr)
+{
+    const int32_t c = gid();
+    if (c < d.nc)
+    {
+        const double iv = 1.0 / d.cvol[c];
+        for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+        {
+            const int32_t v = d.c2f[k], f = pid(v);
+            const double sg = psg(v) * iv;
+            aV[3 * size_t(k)] = static_cast<float>(sg * d.farea[3 * size_t(f)]);
+            aV[3 * size_t(k) + 1] = static_cast<float>(sg * d.farea[3 * size_t(f) + 1]);
+            aV[3 * size_t(k) + 2] = static_cast<float>(sg * d.farea[3 * size_t(f) + 2]);
+            const int32_t l = d.fL[f], r = d.fR[f];
+            nbr[k] = (l >= 0 && r >= 0) ? (l == c ? r : l) : -1;
+        }
     }
 }
 
@@ -286,16 +315,26 @@ __global__ void __launch_bounds__(256) k_multiaxpy_f32(const float *const *V, in
     }
 }
 
-__global__ void __launch_bounds__(256) k_to_f32_scaled(const double *x, const double *s, int32_t n, float *out)
+// V_0 = r / beta in FP32 and its preconditioned copy pv = dir ? 0 : V_0 invdiag
+__global__ void __launch_bounds__(256) k_to_f32_scaled(const double *x, const double *s, const float *invdiag, const int8_t *dir, int32_t n,
+                                                       float *out, float *pv)
 {
     const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) out[e] = static_cast<float>(x[e] / s[0]);
+    if (e >= n) return;
+    const float v = static_cast<float>(x[e] / s[0]);
+    out[e] = v;
+    pv[e] = dir[e] ? 0.0f : v * invdiag[e];
 }
-__global__ void __launch_bounds__(256) k_scale_f32(const float *w, const double *h, int32_t n, float *out)
+// V_{j+1} = w / h and pv = dir ? 0 : V_{j+1} invdiag
+__global__ void __launch_bounds__(256) k_scale_f32(const float *w, const double *h, const float *invdiag, const int8_t *dir, int32_t n,
+                                                   float *out, float *pv)
 {
     const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     const double hv = h[0];
-    if (e < n && hv > 1e-300) out[e] = static_cast<float>(static_cast<double>(w[e]) / hv);
+    if (e >= n || !(hv > 1e-300)) return;
+    const float v = static_cast<float>(static_cast<double>(w[e]) / hv);
+    out[e] = v;
+    pv[e] = dir[e] ? 0.0f : v * invdiag[e];
 }
 __global__ void __launch_bounds__(256) k_update_mixed(const float *const *V, const double *y, int32_t j, const float *invdiag,
                                                       const int8_t *dir, int32_t n, double *delta)
@@ -362,12 +401,18 @@ void mhdg_ctx::mixed_build()
     };
     const size_t nc2f = c2f_nnz, nec = ec_nnz;
     M.s.star2 = f32(d.star2, F);
-    M.s.farea = f32(d.farea, 3 * F);
-    M.s.cvol = f32(d.cvol, C);
+    M.s.m_e = f32(d.m_e, E);
     M.s.recon = f32(d.recon, 3 * nc2f);
     M.s.ecw = f32(d.ecw, nec);
     M.s.evec = f32(d.evec, 3 * E);
-    M.s.m_e = f32(d.m_e, E);
+    {
+        float *aV = static_cast<float *>(alloc(3 * nc2f * sizeof(float)));
+        int32_t *nbr = static_cast<int32_t *>(alloc(nc2f * sizeof(int32_t)));
+        hipLaunchKernelGGL(k_tables_f32, dim3(nb(nc)), dim3(256), 0, stream, d, aV, nbr);
+        M.s.aV = aV; M.s.nbr = nbr;
+        M.s.cdeg = cdeg; M.s.fdeg = fdeg;
+    }
+    M.pv = static_cast<float *>(alloc(E * sizeof(float)));
     M.s.invdiag = static_cast<float *>(alloc(E * sizeof(float)));
     M.s.alpha = static_cast<float *>(alloc(C * sizeof(float)));
     M.s.B0 = static_cast<float *>(alloc(3 * C * sizeof(float)));
@@ -425,7 +470,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
         apply_op(nullptr, tmp, nullptr, dir, dt, 1, 0, bvec, rvec);
         beta = dot(rvec, rvec, 1, 1);
         if (!(beta > rtol * bnorm) || total >= max_it) break;
-        LAUNCH("scale", k_to_f32_scaled, dim3(nb(ne)), dim3(256), 0, stream, rvec, scal + 1, ne, M.V[0]);
+        LAUNCH("scale", k_to_f32_scaled, dim3(nb(ne)), dim3(256), 0, stream, rvec, scal + 1, M.s.invdiag, dir, ne, M.V[0], M.pv);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         const double stop = std::max(inner_rtol * beta, 0.25 * rtol * bnorm);
@@ -433,17 +478,17 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
         for (; j < m && total < max_it; ++j, ++total)
         {
             // w = A P V_j in FP32
-            LAUNCH("d1_f32", k_d1_f32, dim3(nb(nf)), dim3(256), 0, stream, d, M.V[j], M.s.invdiag, dir, dte32, M.phi);
+            LAUNCH("d1_f32", k_d1_f32, dim3(nb(nf)), dim3(256), 0, stream, d, M.s, M.pv, dte32, M.phi);
             if (op_hall)
             {
                 LAUNCH("bcell_f32", k_bcell_f32, dim3(nb(nc)), dim3(256), 0, stream, d, M.s, M.phi, M.b);
                 LAUNCH("hall_g_f32", k_hall_g_f32, dim3(nb(nc)), dim3(256), 0, stream, d, M.s, M.b, M.g);
             }
             LAUNCH("edge_apply_f32", k_edge_apply_f32, dim3(nb(ne)), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall), M.phi, M.g, M.V[j],
-                   dir, dtf, M.w);
+                   M.pv, dir, dtf, M.w);
             if (n_heavy)
                 LAUNCH("edge_apply_f32_heavy", k_edge_apply_f32_heavy, dim3(n_heavy), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall),
-                       static_cast<const int32_t *>(heavy), M.phi, M.g, M.V[j], dir, dtf, M.w);
+                       static_cast<const int32_t *>(heavy), M.phi, M.g, M.V[j], M.pv, dir, dtf, M.w);
             // classical Gram-Schmidt against V_0..V_j
             for (int i0 = 0; i0 <= j; i0 += kDotChunk)
                 LAUNCH("multidot", k_multidot_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, i0, std::min(kDotChunk, j + 1 - i0), M.w, ne,
@@ -463,7 +508,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
                 hn = c2[j + 1];
                 chk(hipMemcpyAsync(M.h + j + 1, &hn, 8, hipMemcpyHostToDevice, stream), "H2D");
             }
-            LAUNCH("scale", k_scale_f32, dim3(nb(ne)), dim3(256), 0, stream, M.w, M.h + j + 1, ne, M.V[j + 1]);
+            LAUNCH("scale", k_scale_f32, dim3(nb(ne)), dim3(256), 0, stream, M.w, M.h + j + 1, M.s.invdiag, dir, ne, M.V[j + 1], M.pv);
             for (int i = 0; i <= j; ++i) H[i][j] = hc[i];
             H[j][j] += 1.0; // Arnoldi ran on A P - I
             H[j + 1][j] = hn;

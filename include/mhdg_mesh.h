@@ -63,6 +63,34 @@ int mhdg_mesh_cell_vector(const mhdg_mesh *m, const double *flux, double *b);
 int mhdg_mesh_seeded_state(const mhdg_mesh *m, const mhdg_config *cfg, double noise, uint64_t seed,
                            const double *BaddCell, double *U_H, double *U_Na, double *Phi);
 
+/* ------------------------------------------------------------------------ UNSBIN mesh and field files */
+/* The reference's container (include/uns/io/BinContainer.h:1-6, docs/FileFormats.md:3-9).  Complex files as
+ * uns::io::write_local_mesh / read_local_mesh (src/uns/io/MeshIO.cpp:38-115) for one rank holding the whole
+ * mesh: the stored incidences, numbering and orientation are used as they are, geometry is recomputed, and
+ * the stored topology hash is verified.  A part file of a several-rank partition is refused. */
+int mhdg_mesh_read_unsbin(const char *path, mhdg_mesh **out);
+int mhdg_mesh_write_unsbin(const mhdg_mesh *m, const char *path);
+/* uns::topology_hash (src/uns/complex/CellComplex.cpp:56-70). */
+int mhdg_mesh_topology_hash(const mhdg_mesh *m, uint64_t *hash);
+/* Face-tag names, newline-joined, index = tag id (0 reserved): the `tag_names` record. */
+int mhdg_mesh_tag_names(const mhdg_mesh *m, char *buf, int buflen);
+int mhdg_mesh_set_tag_names(mhdg_mesh *m, const char *newline_joined);
+
+/* Field files as uns::io::write_fields / read_fields (MeshIO.cpp:117-151): entity-numbered arrays
+ * (dim 0 node, 1 edge, 2 face, 3 cell; ncomp values per entity) and the topology hash of their complex. */
+typedef struct mhdg_field {
+    const char *name;
+    int32_t dim, ncomp;
+    int64_t count;          /* number of doubles */
+    const double *values;
+} mhdg_field;
+int mhdg_fields_write(const char *path, uint64_t hash, const mhdg_field *fields, int32_t n_fields);
+typedef struct mhdg_fields mhdg_fields;
+int mhdg_fields_read(const char *path, mhdg_fields **out, uint64_t *hash, int32_t *n_fields);
+/* Field i of a file read by mhdg_fields_read; the pointers stay valid until mhdg_fields_free. */
+int mhdg_fields_get(const mhdg_fields *f, int32_t i, mhdg_field *field);
+void mhdg_fields_free(mhdg_fields *f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,15 @@ def lib():
         L.mhdg_mesh_background.argtypes = [P, P, C.c_int, P, P, D, P, P]
         L.mhdg_mesh_cell_vector.argtypes = [P, P, P]
         L.mhdg_mesh_seeded_state.argtypes = [P, C.POINTER(Config), D, C.c_uint64, P, P, P, P]
+        L.mhdg_mesh_read_unsbin.argtypes = [C.c_char_p, C.POINTER(P)]
+        L.mhdg_mesh_write_unsbin.argtypes = [P, C.c_char_p]
+        L.mhdg_mesh_topology_hash.argtypes = [P, C.POINTER(C.c_uint64)]
+        L.mhdg_mesh_tag_names.argtypes = [P, C.c_char_p, C.c_int]
+        L.mhdg_mesh_set_tag_names.argtypes = [P, C.c_char_p]
+        L.mhdg_fields_write.argtypes = [C.c_char_p, C.c_uint64, P, I32]
+        L.mhdg_fields_read.argtypes = [C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64), C.POINTER(I32)]
+        L.mhdg_fields_get.argtypes = [P, I32, P]
+        L.mhdg_fields_free.argtypes = [P]
         for name, args in {
             "mhdg_cell_vector": [P, P, P], "mhdg_cell_current": [P, P, P],
             "mhdg_limited_gradients": [P, P, C.c_int, P, C.c_int, P, P],

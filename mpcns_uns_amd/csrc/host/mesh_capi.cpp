@@ -1,6 +1,8 @@
 // Host C-ABI for mesh setup (include/mhdg_mesh.h).
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -9,6 +11,7 @@
 #include "complex.h"
 #include "geometry.h"
 #include "shell.h"
+#include "unsbin.h"
 #include "vec3.h"
 
 struct mhdg_mesh
@@ -19,6 +22,11 @@ struct mhdg_mesh
     std::vector<int32_t> tets;
     std::vector<int8_t> face_kind, edge_mode;
     bool have_bc = false;
+    std::vector<std::string> tag_names;
+};
+struct mhdg_fields
+{
+    std::vector<mhdg::unsbin::Field> f;
 };
 
 namespace
@@ -35,6 +43,93 @@ mhdg_mesh *finish_mesh(mhdg_mesh *m)
 extern "C" {
 const char *mhdg_mesh_error(void) { return g_err.c_str(); }
 
+int mhdg_mesh_read_unsbin(const char *path, mhdg_mesh **out)
+{
+    try
+    {
+        auto m = std::make_unique<mhdg_mesh>();
+        mhdg::unsbin::read_complex(path, m->cx, m->xyz, m->tag_names);
+        *out = finish_mesh(m.release());
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+int mhdg_mesh_write_unsbin(const mhdg_mesh *m, const char *path)
+{
+    try
+    {
+        mhdg::unsbin::write_complex(path, m->cx, m->xyz, m->tag_names);
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+int mhdg_mesh_topology_hash(const mhdg_mesh *m, uint64_t *hash)
+{
+    *hash = mhdg::unsbin::topology_hash(m->cx);
+    return 0;
+}
+int mhdg_mesh_tag_names(const mhdg_mesh *m, char *buf, int len)
+{
+    std::string s;
+    for (const auto &t : m->tag_names) s += t + "\n";
+    if (len <= 0) return 1;
+    std::snprintf(buf, static_cast<size_t>(len), "%s", s.c_str());
+    return static_cast<int>(s.size()) < len ? 0 : 1;
+}
+int mhdg_mesh_set_tag_names(mhdg_mesh *m, const char *joined)
+{
+    m->tag_names.clear();
+    std::string cur;
+    for (const char *p = joined; *p; ++p)
+    {
+        if (*p == '\n') { m->tag_names.push_back(cur); cur.clear(); }
+        else cur += *p;
+    }
+    if (!cur.empty()) m->tag_names.push_back(cur);
+    return 0;
+}
+int mhdg_fields_write(const char *path, uint64_t hash, const mhdg_field *fields, int32_t n)
+{
+    try
+    {
+        std::vector<mhdg::unsbin::Field> v(n);
+        for (int32_t i = 0; i < n; ++i)
+        {
+            v[i].name = fields[i].name;
+            v[i].dim = fields[i].dim;
+            v[i].ncomp = fields[i].ncomp;
+            v[i].values.assign(fields[i].values, fields[i].values + fields[i].count);
+        }
+        mhdg::unsbin::write_fields(path, hash, v);
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+int mhdg_fields_read(const char *path, mhdg_fields **out, uint64_t *hash, int32_t *n)
+{
+    try
+    {
+        auto f = std::make_unique<mhdg_fields>();
+        f->f = mhdg::unsbin::read_fields(path, hash);
+        *n = static_cast<int32_t>(f->f.size());
+        *out = f.release();
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+int mhdg_fields_get(const mhdg_fields *f, int32_t i, mhdg_field *out)
+{
+    if (i < 0 || i >= static_cast<int32_t>(f->f.size())) { g_err = "field index out of range"; return 1; }
+    const auto &x = f->f[i];
+    out->name = x.name.c_str();
+    out->dim = x.dim;
+    out->ncomp = x.ncomp;
+    out->count = static_cast<int64_t>(x.values.size());
+    out->values = x.values.data();
+    return 0;
+}
+void mhdg_fields_free(mhdg_fields *f) { delete f; }
+
 int mhdg_mesh_shell(int n_lat, int n_lon, int n_r, double r_in, double r_out, double ratio, double jitter, uint64_t seed, mhdg_mesh **out)
 {
     try
@@ -48,6 +143,7 @@ int mhdg_mesh_shell(int n_lat, int n_lon, int n_r, double r_in, double r_out, do
         const int32_t nn = static_cast<int32_t>(m->xyz.size() / 3);
         m->cx = mhdg::build_tet_complex(nn, m->tets.data(), static_cast<int64_t>(m->tets.size() / 4), sm.tags.data(),
                                         static_cast<int64_t>(sm.tags.size() / 4));
+        m->tag_names = {"", "inner", "outer"}; // tag ids 1 (r = r_in) and 2 (r = r_out)
         *out = finish_mesh(m);
         return 0;
     }

@@ -148,6 +148,17 @@ class Device:
     def set_state(self, U_H, U_Na, Phi):
         self._ok(lib().mhdg_set_state(self._h, ptr(f64(U_H)), ptr(f64(U_Na)), ptr(f64(Phi))))
 
+    def set_state_from_fields(self, path):
+        """Resident state from a restart file in the layout of mercury::Solver::write_global_fields
+        (MercuryUns/driver/Solver.cpp:721-745): U_H, U_Na (cell, 5) and B (face flux)."""
+        from .mesh import read_fields
+
+        _, f = read_fields(path)
+        for name, dim, nc, n in (("U_H", 3, 5, self.nc), ("U_Na", 3, 5, self.nc), ("B", 2, 1, self.nf)):
+            if name not in f or f[name][0] != dim or f[name][1] != nc or f[name][2].size != nc * n:
+                raise RuntimeError(f"{path}: no {name} record of {n} entities x {nc} on this mesh")
+        self.set_state(f["U_H"][2], f["U_Na"][2], f["B"][2])
+
     def get_state(self):
         UH, UNa, Phi = np.zeros(5 * self.nc), np.zeros(5 * self.nc), np.zeros(self.nf)
         self._ok(lib().mhdg_get_state(self._h, ptr(UH), ptr(UNa), ptr(Phi)))

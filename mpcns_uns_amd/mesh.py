@@ -63,6 +63,36 @@ class Mesh:
         m._loops = (cp, lp, nd, tp, tn, ti, mat)
         return m
 
+    @classmethod
+    def read_unsbin(cls, path):
+        """Complex file of the reference (uns::io::read_local_mesh) written for one rank: stored numbering and
+        orientation kept, geometry recomputed, topology hash verified."""
+        h = C.c_void_p()
+        if lib().mhdg_mesh_read_unsbin(str(path).encode(), C.byref(h)):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+        return cls(h)
+
+    def write_unsbin(self, path):
+        """Complex file readable by the reference (uns::io::write_local_mesh layout, one rank)."""
+        if lib().mhdg_mesh_write_unsbin(self._h, str(path).encode()):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+
+    @property
+    def topology_hash(self):
+        h = C.c_uint64()
+        lib().mhdg_mesh_topology_hash(self._h, C.byref(h))
+        return h.value
+
+    @property
+    def tag_names(self):
+        buf = C.create_string_buffer(1 << 16)
+        lib().mhdg_mesh_tag_names(self._h, buf, len(buf))
+        return buf.value.decode().split("\n")[:-1]
+
+    @tag_names.setter
+    def tag_names(self, names):
+        lib().mhdg_mesh_set_tag_names(self._h, "".join(n + "\n" for n in names).encode())
+
     # ---------------------------------------------------------------- arrays
     def array(self, name):
         p, n, es = C.c_void_p(), C.c_int64(), C.c_int32()
@@ -140,3 +170,39 @@ def default_config(**kw):
         else:
             setattr(c, k, v)
     return c
+
+
+# --------------------------------------------------------------------------- field files
+DIMS = {"node": 0, "edge": 1, "face": 2, "cell": 3}
+
+
+class _Field(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("dim", C.c_int32), ("ncomp", C.c_int32), ("count", C.c_int64), ("values", C.c_void_p)]
+
+
+def write_fields(path, fields, topology_hash=0):
+    """UNSBIN field file (uns::io::write_fields): fields = [(name, dim, ncomp, values)], dim 0 node .. 3 cell."""
+    keep, recs = [], (_Field * len(fields))()
+    for i, (name, dim, ncomp, values) in enumerate(fields):
+        v = f64(values)
+        keep.append(v)
+        recs[i] = _Field(name.encode(), DIMS.get(dim, dim), int(ncomp), v.size, v.ctypes.data)
+    if lib().mhdg_fields_write(str(path).encode(), C.c_uint64(topology_hash), recs, len(fields)):
+        raise RuntimeError(lib().mhdg_mesh_error().decode())
+
+
+def read_fields(path):
+    """(topology hash, {name: (dim, ncomp, values)}) of a UNSBIN field file (uns::io::read_fields)."""
+    h, n, hs = C.c_void_p(), C.c_int32(), C.c_uint64()
+    if lib().mhdg_fields_read(str(path).encode(), C.byref(h), C.byref(hs), C.byref(n)):
+        raise RuntimeError(lib().mhdg_mesh_error().decode())
+    out = {}
+    try:
+        for i in range(n.value):
+            f = _Field()
+            lib().mhdg_fields_get(h, i, C.byref(f))
+            vals = np.ctypeslib.as_array(C.cast(f.values, C.POINTER(C.c_double)), (f.count,)).copy() if f.count else np.zeros(0)
+            out[f.name.decode()] = (f.dim, f.ncomp, vals)
+    finally:
+        lib().mhdg_fields_free(h)
+    return hs.value, out

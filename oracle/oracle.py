@@ -219,9 +219,13 @@ class Reference:
                 "ref_ind_set_ideal_state": [_P, _P, _P], "ref_ind_ideal_linear": [_P, _P, _P],
                 "ref_set_state": [_P, _P, _P, _P], "ref_get_state": [_P, _P, _P, _P], "ref_step": [_P, _D, _P],
                 "ref_get_derived": [_P, C.c_int, _P], "ref_last_induction_residual": [_P],
+                "ref_write_mesh": [_P, C.c_char_p], "ref_read_mesh": [C.c_char_p, _P, _P], "ref_topology_hash": [_P],
+                "ref_write_fields": [C.c_char_p, C.c_uint64, C.c_int, _P, _P, _P, _P, _P],
+                "ref_read_field": [C.c_char_p, C.c_char_p, _P, _P, _P, _P, _P],
             }.items():
                 getattr(L, name).argtypes = args
             L.ref_ind_solve.restype = C.c_int
+            L.ref_topology_hash.restype = C.c_uint64
             cls._lib = L
         return cls._lib
 

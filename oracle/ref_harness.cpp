@@ -29,6 +29,7 @@
 #include "uns/complex/Validate.h"
 #include "uns/geometry/Geometry.h"
 #include "uns/geometry/Reconstruct.h"
+#include "uns/io/MeshIO.h"
 
 #include "../include/mhdg.h"
 
@@ -289,4 +290,61 @@ void ref_get_derived(ref_ctx *r, int which, double *out)
     }
 }
 double ref_last_induction_residual(ref_ctx *r) { return r->solver->last_induction_residual(); }
+
+// uns::io (src/uns/io/MeshIO.cpp): the reference's complex and field files
+int ref_write_mesh(ref_ctx *r, const char *path)
+{
+    try { uns::io::write_local_mesh(path, r->lm); return 0; }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_write_mesh: %s\n", e.what()); return 1; }
+}
+// reads a complex file (the reader verifies the stored hash); counts of nodes, edges, faces, cells and the hash
+int ref_read_mesh(const char *path, int64_t counts[4], uint64_t *hash)
+{
+    try
+    {
+        const LocalMesh lm = uns::io::read_local_mesh(path);
+        for (int d = 0; d < 4; ++d) counts[d] = lm.mesh.cx.count(static_cast<Dim>(d));
+        *hash = topology_hash(lm.mesh.cx);
+        return 0;
+    }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_read_mesh: %s\n", e.what()); return 1; }
+}
+uint64_t ref_topology_hash(ref_ctx *r) { return topology_hash(r->lm.mesh.cx); }
+int ref_write_fields(const char *path, uint64_t hash, int n, const char *const *names, const int32_t *dims, const int32_t *ncomps,
+                     const double *const *values, const int64_t *counts)
+{
+    try
+    {
+        std::vector<uns::io::FieldRecord> f(n);
+        for (int i = 0; i < n; ++i)
+        {
+            f[i].name = names[i];
+            f[i].dim = static_cast<Dim>(dims[i]);
+            f[i].ncomp = ncomps[i];
+            f[i].values.assign(values[i], values[i] + counts[i]);
+        }
+        uns::io::write_fields(path, hash, f);
+        return 0;
+    }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_write_fields: %s\n", e.what()); return 1; }
+}
+// the values of field `name` (count in/out: capacity / size), its dim and ncomp, and the file's hash
+int ref_read_field(const char *path, const char *name, double *out, int64_t *count, int32_t *dim, int32_t *ncomp, uint64_t *hash)
+{
+    try
+    {
+        for (const auto &f : uns::io::read_fields(path, hash))
+            if (f.name == name)
+            {
+                if (static_cast<int64_t>(f.values.size()) > *count) return 2;
+                std::memcpy(out, f.values.data(), f.values.size() * 8);
+                *count = static_cast<int64_t>(f.values.size());
+                *dim = static_cast<int32_t>(f.dim);
+                *ncomp = f.ncomp;
+                return 0;
+            }
+        return 3;
+    }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_read_field: %s\n", e.what()); return 1; }
+}
 }

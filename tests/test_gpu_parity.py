@@ -253,3 +253,18 @@ def test_fused_step_entries(small_hall):
     off = Device(w["mesh"], cfg_off)
     with pytest.raises(RuntimeError, match="step_hall"):
         off.step_hall()
+
+
+def test_restart_from_field_file(small_explicit, tmp_path):
+    """A state written as a UNSBIN restart file (U_H, U_Na, B) and loaded on the device steps exactly as the arrays."""
+    from mpcns_uns_amd import write_fields
+
+    w = small_explicit
+    UH, UNa, Phi = w["state"]
+    p = tmp_path / "fields_final.unsb"
+    write_fields(p, [("U_H", "cell", 5, UH), ("U_Na", "cell", 5, UNa), ("B", "face", 1, Phi), ("_time", "node", 1, [0.0])])
+    dev, orc = _fresh(w)
+    dev.set_state_from_fields(p)
+    assert dev.step() == orc.step()
+    for a, b in zip(dev.get_state(), orc.get_state()):
+        assert np.array_equal(a, b)

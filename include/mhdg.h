@@ -155,6 +155,23 @@ int mhdg_ind_newton(mhdg_ctx *ctx, const double *Phi_n, const double *E_expl, co
                     const double *E0, double rtol, int max_newton, double eta, double *E, int32_t *newton_its,
                     int32_t *gmres_its, double *rel_res);
 
+/* ------------------------------------------------- tangent-linear of the explicit step (forward mode) */
+/* No counterpart in the reference.  The explicit-stage kernels are templates on their real type; with a dual
+ * number (value, derivative) they propagate one tangent direction exactly through the explicit stage of
+ * step_first_order (prepare, CFL step, limited gradients, Rusanov fluxes, sources, forward Euler + floors,
+ * UCT EMF with the boundary EMF, CT), branches taken on the values.
+ * mhdg_shell_energy: J = sum over fluid cells with r0 <= |x_c| <= r1 of |b_c + BaddCell_c|^2 V_c, b the
+ * reconstructed cell field of the resident Phi.
+ * mhdg_explicit_tangent: from the resident state, seeds dU_H, dU_Na (5C), dPhi (F), dPhiAdd (F), dBadd (3C),
+ * dfs (2 x 5: free-stream states), dE_inf (3: imposed inflow EMF), every one nullable = 0, runs n_steps
+ * explicit steps on (value, tangent) pairs and returns J and dJ of the final state and, when non-null, the
+ * tangents of the final U_H, U_Na, Phi.  The resident state is not changed; edge valence at most 512. */
+int mhdg_shell_energy(mhdg_ctx *ctx, double r0, double r1, double *J);
+int mhdg_explicit_tangent(mhdg_ctx *ctx, int n_steps, double dt_max, const double *dU_H, const double *dU_Na,
+                          const double *dPhi, const double *dPhiAdd, const double *dBadd, const double *dfs,
+                          const double *dE_inf, double r0, double r1, double *J, double *dJ, double *dU_H_out,
+                          double *dU_Na_out, double *dPhi_out);
+
 /* --------------------------------------------------------------- device-resident step (fused path) */
 /* State: U_H, U_Na (5C each, AoS), induced flux Phi (F); background PhiAdd (F) and BaddCell (3C). */
 int mhdg_set_state(mhdg_ctx *ctx, const double *U_H, const double *U_Na, const double *Phi);

@@ -95,7 +95,9 @@ def lib():
             "mhdg_ind_solve": [P, P, P, P, D, P, C.c_int, P, P, P],
             "mhdg_ind_flags": [P, P, P, P], "mhdg_ind_set_ideal_state": [P, P, P], "mhdg_ind_ideal_linear": [P, P, P],
             "mhdg_step_explicit": [P, D, P], "mhdg_ind_apply_G": [P, P, P], "mhdg_ind_jvp": [P, P, P, P],
-            "mhdg_ind_newton": [P, P, P, P, D, P, D, C.c_int, D, P, P, P, P], "mhdg_set_recon_rel_limit": [P, D], "mhdg_step_hall": [P, D, C.c_int, P, P, P],
+            "mhdg_ind_newton": [P, P, P, P, D, P, D, C.c_int, D, P, P, P, P],
+            "mhdg_shell_energy": [P, D, D, P],
+            "mhdg_explicit_tangent": [P, C.c_int, D, P, P, P, P, P, P, P, D, D, P, P, P, P, P], "mhdg_set_recon_rel_limit": [P, D], "mhdg_step_hall": [P, D, C.c_int, P, P, P],
             "mhdg_set_state": [P, P, P, P], "mhdg_get_state": [P, P, P, P],
             "mhdg_set_background": [P, P, P], "mhdg_step": [P, D, C.c_int, P, P, P],
             "mhdg_get_derived": [P, C.c_int, P], "mhdg_sync": [P], "mhdg_device_info": [C.c_char_p, C.c_int],

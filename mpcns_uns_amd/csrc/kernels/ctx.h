@@ -38,6 +38,11 @@ struct mhdg_ctx
     // Newton-Krylov on the nonlinear Hall induction (tangent.hip)
     double *lin_b = nullptr;           // cell field at the linearisation point: apply_op then applies the tangent operator
     double rtol_override = 0;          // GMRES tolerance of a Newton step (0: cfg.ind_rtol)
+    // tangent-linear of the explicit step (Dual state, tangent.hip)
+    struct TangentWork { mhdg::Dual *U[2]{}, *Phi = nullptr, *PhiAdd = nullptr, *Badd = nullptr, *W[2]{}, *grad[2]{}, *Bind = nullptr,
+                         *Btot = nullptr, *uplus = nullptr, *J = nullptr, *alpha = nullptr, *FA = nullptr, *Ef = nullptr, *E = nullptr,
+                         *dt_part = nullptr, *dt_dev = nullptr, *part = nullptr; } tw;
+    double *energy_part = nullptr;
     struct NewtonWork { double *phik = nullptr, *G = nullptr, *R = nullptr, *rhs = nullptr, *delta = nullptr, *E = nullptr,
                         *lin_b = nullptr, *B0 = nullptr, *zero_phi = nullptr, *corr_save = nullptr; } nt;
     std::vector<double *> V;
@@ -76,6 +81,9 @@ struct mhdg_ctx
     int solve_mixed(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
     double step(double dt_max, int precision, int *iters, double *res);
     void newton_alloc();
+    void tangent_alloc();
+    void explicit_tangent(int n_steps, double dt_max, const mhdg::Params &Pt);
+    template <class R> void shell_energy(const R *phi, const R *badd, double r0, double r1, R *out);
     void apply_G(const double *phi, double *out);
     void jvp(const double *phi_lin, const double *dphi, double *out);
     int newton(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, const double *E0, double rtol, int max_newton,

@@ -217,9 +217,9 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
 void mhdg_ctx::launch_edge_emf(const double *Efp, const double *u, const double *phi_ind, double cuct, const double *Ein, double *Eout,
                                int bc)
 {
-    LAUNCH("edge_emf", k_edge_emf, dim3(nblk(ne)), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct, kLightMax, Ein, Eout, bc);
+    LAUNCH("edge_emf", k_edge_emf<double>, dim3(nblk(ne)), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct, kLightMax, Ein, Eout, bc);
     if (n_heavy)
-        LAUNCH("edge_emf_heavy", k_edge_emf_heavy, dim3(n_heavy), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct,
+        LAUNCH("edge_emf_heavy", k_edge_emf_heavy<double>, dim3(n_heavy), dim3(256), 0, stream, d, P, Efp, u, phi_ind, cuct,
                            static_cast<const int32_t *>(heavy), Ein, Eout, bc);
     check_launch("edge emf");
 }
@@ -253,7 +253,7 @@ void mhdg_ctx::apply_op(const double *phi_in, const double *v, const double *inv
     {
         // add_ideal_emf of (u, b, phi) with b as total and induced field (ImplicitInduction.cpp:150-157)
         SpC none{};
-        LAUNCH("face", k_face, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 0, 0, none, none, none, static_cast<double *>(nullptr), u_ideal,
+        LAUNCH("face", k_face<double>, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 0, 0, none, none, none, static_cast<double *>(nullptr), u_ideal,
                bcell, bcell, phi, static_cast<const double *>(nullptr), Ef_i);
         launch_edge_emf(Ef_i, u_ideal, phi, P.cuct, nullptr, ide, 0);
     }
@@ -338,9 +338,9 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
 double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
 {
     const int nb = nblk(nc);
-    LAUNCH("prepare", k_prepare, dim3(nb), dim3(256), 0, stream, d, P, U[0], U[1], Phi, Badd, W[0], W[1], Bind, Btot, uplus, alpha,
+    LAUNCH("prepare", k_prepare<double>, dim3(nb), dim3(256), 0, stream, d, P, U[0], U[1], Phi, Badd, W[0], W[1], Bind, Btot, uplus, alpha,
                        dt_part);
-    LAUNCH("dt_final", k_dt_final, dim3(1), dim3(256), 0, stream, dt_part, nb, dt_max, dt_dev);
+    LAUNCH("dt_final", k_dt_final<double>, dim3(1), dim3(256), 0, stream, dt_part, nb, dt_max, dt_dev);
     double *dth = static_cast<double *>(host_pin);
     hip_check(hipMemcpyAsync(dth, dt_dev, 8, hipMemcpyDeviceToHost, stream), "D2H");
     hipEvent_t ev;
@@ -353,20 +353,20 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     gradsC.p[0] = grad[0]; gradsC.p[1] = grad[1];
     Us.p[0] = U[0]; Us.p[1] = U[1];
     if (P.second_order)
-        LAUNCH("grad_current", k_grad_current, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, Ws, none, grads, 1, Bind, J);
+        LAUNCH("grad_current", k_grad_current<double>, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, Ws, none, grads, 1, Bind, J);
     else
     {
         hip_check(hipMemsetAsync(grad[0], 0, 15 * size_t(nc) * 8, stream), "memset");
         hip_check(hipMemsetAsync(grad[1], 0, 15 * size_t(nc) * 8, stream), "memset");
-        LAUNCH("grad_current", k_grad_current, dim3(nb), dim3(256), 0, stream, d, P, 0, 0, Ws, none, grads, 1, Bind, J);
+        LAUNCH("grad_current", k_grad_current<double>, dim3(nb), dim3(256), 0, stream, d, P, 0, 0, Ws, none, grads, 1, Bind, J);
     }
-    LAUNCH("face", k_face, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 2, 0, Ws, gradsC, none, FA, uplus, Btot, Bind, Phi, PhiAdd, Ef);
+    LAUNCH("face", k_face<double>, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 2, 0, Ws, gradsC, none, FA, uplus, Btot, Bind, Phi, PhiAdd, Ef);
     check_launch("explicit stage");
     hip_check(hipEventSynchronize(ev), "event sync");
     (void)hipEventDestroy(ev);
     const double gdt = dth[0];
     if (!std::isfinite(gdt) || gdt <= 0) throw std::runtime_error("invalid time step");
-    LAUNCH("cell_update", k_cell_update, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, static_cast<const double *>(FA), none, W[0], W[1], Btot,
+    LAUNCH("cell_update", k_cell_update<double>, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, static_cast<const double *>(FA), none, W[0], W[1], Btot,
                        J, dt_dev, 0.0, 1, 1, Us, noneW);
     launch_edge_emf(Ef, uplus, Phi, P.cuct, nullptr, E_expl, 1);
     *iters = 0;
@@ -381,7 +381,7 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     }
     else
         hip_check(hipMemcpyAsync(E_last, E_expl, size_t(ne) * 8, hipMemcpyDeviceToDevice, stream), "copy");
-    LAUNCH("faraday", k_faraday, dim3(nblk(nf)), dim3(256), 0, stream, d, Euse, nullptr, dt_dev, 0.0, Phi);
+    LAUNCH("faraday", k_faraday<double>, dim3(nblk(nf)), dim3(256), 0, stream, d, Euse, nullptr, dt_dev, 0.0, Phi);
     check_launch("faraday");
     if (prof_on) prof_flush();
     return gdt;
@@ -485,7 +485,7 @@ int mhdg_cell_current(mhdg_ctx *c, const double *B, double *J)
         double *o = c->stage(1, 3 * size_t(c->nc));
         SpC none{};
         SpW nw{};
-        hipLaunchKernelGGL(k_grad_current, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, c->P, 0, 0, none, none, nw, 0, b, o);
+        hipLaunchKernelGGL(k_grad_current<double>, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, c->P, 0, 0, none, none, nw, 0, b, o);
         c->check_launch("cell_current");
         c->d2h(J, o, 3 * size_t(c->nc) * 8);
     })
@@ -501,7 +501,7 @@ int mhdg_limited_gradients(mhdg_ctx *c, const double *W, int species, const doub
         Ws.p[0] = c->stage_in(0, W, 5 * size_t(c->nc));
         if (ghost) gh.p[0] = c->stage_in(2, ghost, 5 * size_t(c->nf));
         gr.p[0] = c->stage(1, 15 * size_t(c->nc));
-        hipLaunchKernelGGL(k_grad_current, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, P, 1, species, Ws, gh, gr, limit,
+        hipLaunchKernelGGL(k_grad_current<double>, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, P, 1, species, Ws, gh, gr, limit,
                            static_cast<const double *>(nullptr), static_cast<double *>(nullptr));
         c->check_launch("limited_gradients");
         c->d2h(grad, gr.p[0], 15 * size_t(c->nc) * 8);
@@ -520,10 +520,10 @@ int mhdg_flux_rhs(mhdg_ctx *c, int species, const double fs[5], const double *W,
         if (ghost) gh.p[0] = c->stage_in(2, ghost, 5 * size_t(c->nf));
         double *fa = c->stage(3, 5 * size_t(c->nf));
         ro.p[0] = c->stage(4, 5 * size_t(c->nc));
-        hipLaunchKernelGGL(k_face, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, P, 1, species, Ws, Gs, gh, fa,
+        hipLaunchKernelGGL(k_face<double>, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, P, 1, species, Ws, Gs, gh, fa,
                            static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
                            static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), static_cast<double *>(nullptr));
-        hipLaunchKernelGGL(k_cell_update, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, P, 1, species, static_cast<const double *>(fa),
+        hipLaunchKernelGGL(k_cell_update<double>, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, P, 1, species, static_cast<const double *>(fa),
                            none, static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
                            static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
                            0.0, 0, 0, nw, ro);
@@ -540,7 +540,7 @@ int mhdg_ideal_emf(mhdg_ctx *c, const double *u, const double *Bt, const double 
         double *dE = c->stage_in(5, E, c->ne);
         double *ef = c->stage(6, 3 * size_t(c->nf));
         SpC none{};
-        hipLaunchKernelGGL(k_face, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, c->P, 0, 0, none, none, none,
+        hipLaunchKernelGGL(k_face<double>, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, c->P, 0, 0, none, none, none,
                            static_cast<double *>(nullptr), du, dbt, dbi, dpt, static_cast<const double *>(nullptr), ef);
         c->launch_edge_emf(ef, du, dpi, cuct, dE, dE, 0);
         c->d2h(E, dE, size_t(c->ne) * 8);
@@ -551,7 +551,7 @@ int mhdg_faraday(mhdg_ctx *c, const double *E, double *Phi, double dt)
     MHDG_TRY(c, {
         const double *dE = c->stage_in(0, E, c->ne);
         double *dP = c->stage_in(1, Phi, c->nf);
-        hipLaunchKernelGGL(k_faraday, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, dE, static_cast<const double *>(nullptr),
+        hipLaunchKernelGGL(k_faraday<double>, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, dE, static_cast<const double *>(nullptr),
                            static_cast<const double *>(nullptr), dt, dP);
         c->check_launch("faraday");
         c->d2h(Phi, dP, size_t(c->nf) * 8);
@@ -562,7 +562,7 @@ int mhdg_faraday_edge_dt(mhdg_ctx *c, const double *E, const double *dte, double
     MHDG_TRY(c, {
         const double *dE = c->stage_in(0, E, c->ne), *dT = c->stage_in(2, dte, c->ne);
         double *dP = c->stage_in(1, Phi, c->nf);
-        hipLaunchKernelGGL(k_faraday, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, dE, dT, static_cast<const double *>(nullptr), 0.0, dP);
+        hipLaunchKernelGGL(k_faraday<double>, dim3(nblk(c->nf)), dim3(256), 0, c->stream, c->d, dE, dT, static_cast<const double *>(nullptr), 0.0, dP);
         c->check_launch("faraday_edge_dt");
         c->d2h(Phi, dP, size_t(c->nf) * 8);
     })
@@ -598,7 +598,7 @@ int mhdg_local_dt(mhdg_ctx *c, const double *WH, const double *WNa, const double
         const double *bt = c->stage_in(2, Bt, 3 * size_t(c->nc));
         double *part = c->stage(3, nb), *o = c->stage(4, 1);
         hipLaunchKernelGGL(k_local_dt, dim3(nb), dim3(256), 0, c->stream, c->d, c->P, a, b, bt, part);
-        hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, static_cast<const double *>(part), nb, __builtin_inf(), o);
+        hipLaunchKernelGGL(k_dt_final<double>, dim3(1), dim3(256), 0, c->stream, static_cast<const double *>(part), nb, __builtin_inf(), o);
         c->check_launch("local_dt");
         c->d2h(dt, o, 8);
     })
@@ -615,7 +615,7 @@ int mhdg_add_sources(mhdg_ctx *c, int species, const double *WH, const double *W
         SpW ro{}, nw{};
         ri.p[0] = r;
         ro.p[0] = r;
-        hipLaunchKernelGGL(k_cell_update, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, c->P, 1, species,
+        hipLaunchKernelGGL(k_cell_update<double>, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, c->P, 1, species,
                            static_cast<const double *>(nullptr), ri, a, b, bt, j, static_cast<const double *>(nullptr), dt, 1, 0, nw, ro);
         c->check_launch("add_sources");
         c->d2h(rhs, r, 5 * C * 8);
@@ -738,6 +738,50 @@ int mhdg_ind_newton(mhdg_ctx *c, const double *Phi_n, const double *E_expl, cons
         if (newton_its) *newton_its = k;
         if (gmres_its) *gmres_its = gm;
         if (rel_res) *rel_res = rr;
+    })
+}
+int mhdg_shell_energy(mhdg_ctx *c, double r0, double r1, double *J)
+{
+    MHDG_TRY(c, {
+        c->tangent_alloc();
+        c->shell_energy<double>(c->Phi, c->Badd, r0, r1, J);
+    })
+}
+int mhdg_explicit_tangent(mhdg_ctx *c, int n_steps, double dt_max, const double *dU_H, const double *dU_Na, const double *dPhi,
+                          const double *dPhiAdd, const double *dBadd, const double *dfs, const double *dE_inf, double r0, double r1,
+                          double *J, double *dJ, double *dU_H_out, double *dU_Na_out, double *dPhi_out)
+{
+    MHDG_TRY(c, {
+        if (n_steps < 0) throw std::runtime_error("explicit_tangent: n_steps >= 0");
+        c->tangent_alloc();
+        const size_t C = c->nc, F = c->nf;
+        auto seed = [&](const double *val, const double *host_d, size_t n, int slot, Dual *out) {
+            const double *dd = host_d ? c->stage_in(slot, host_d, n) : nullptr;
+            hipLaunchKernelGGL(k_seed, dim3(nblk(int64_t(n))), dim3(256), 0, c->stream, val, dd, int64_t(n), out);
+        };
+        seed(c->U[0], dU_H, 5 * C, 0, c->tw.U[0]);
+        seed(c->U[1], dU_Na, 5 * C, 1, c->tw.U[1]);
+        seed(c->Phi, dPhi, F, 2, c->tw.Phi);
+        seed(c->PhiAdd, dPhiAdd, F, 3, c->tw.PhiAdd);
+        seed(c->Badd, dBadd, 3 * C, 4, c->tw.Badd);
+        Params Pt = c->P;
+        for (int s = 0; s < 2; ++s)
+            for (int m = 0; m < 5; ++m) Pt.dfs[s][m] = dfs ? dfs[5 * s + m] : 0.0;
+        for (int k = 0; k < 3; ++k) Pt.dE_inf[k] = dE_inf ? dE_inf[k] : 0.0;
+        c->explicit_tangent(n_steps, dt_max, Pt);
+        Dual e;
+        c->shell_energy<Dual>(c->tw.Phi, c->tw.Badd, r0, r1, &e);
+        if (J) *J = e.v;
+        if (dJ) *dJ = e.d;
+        auto out = [&](const Dual *x, size_t n, double *host) {
+            if (!host) return;
+            double *tmpd = c->stage(5, n);
+            hipLaunchKernelGGL(k_tangent_of, dim3(nblk(int64_t(n))), dim3(256), 0, c->stream, x, int64_t(n), tmpd);
+            c->d2h(host, tmpd, n * 8);
+        };
+        out(c->tw.U[0], 5 * C, dU_H_out);
+        out(c->tw.U[1], 5 * C, dU_Na_out);
+        out(c->tw.Phi, F, dPhi_out);
     })
 }
 int mhdg_set_state(mhdg_ctx *c, const double *UH, const double *UNa, const double *Ph)

@@ -8,7 +8,7 @@
 
 #include <cstdint>
 
-#include "../host/vec3.h"
+#include "dual.h"
 
 namespace mhdg
 {
@@ -22,7 +22,8 @@ __device__ __forceinline__ double mx(double a, double b) { return (a < b) ? b : 
 enum : int8_t { kInterior = 0, kWall = 1, kInflow = 2, kOutflow = 3, kSkip = 4 };
 constexpr int32_t kNoStencil = INT32_MIN;
 
-struct Prim { double rho, u, v, w, p; };
+template <class R> struct PrimT { R rho, u, v, w, p; };
+using Prim = PrimT<double>;
 
 struct DevMesh
 {
@@ -55,35 +56,40 @@ struct Params
     int32_t second_order, lorentz, hall, resistive;
     double fs[kMaxSpecies][5];
     double E_inf[3];
+    // tangent seeds of the free-stream states and of the imposed inflow EMF (used by the R = Dual kernels)
+    double dfs[kMaxSpecies][5];
+    double dE_inf[3];
 };
 
 // Per-species array pointers passed by value (kernel arguments)
-struct SpC { const double *p[kMaxSpecies]; };
-struct SpW { double *p[kMaxSpecies]; };
+template <class R> struct SpCT { const R *p[kMaxSpecies]; };
+template <class R> struct SpWT { R *p[kMaxSpecies]; };
+using SpC = SpCT<double>;
+using SpW = SpWT<double>;
 
 __device__ __forceinline__ V3 L3(const double *p, long i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 
 // Common/physics/Euler.h:13-49
-__device__ __forceinline__ Prim to_prim(const double *U, double gamma, double rho_floor, double p_floor)
+template <class R> __device__ __forceinline__ PrimT<R> to_prim(const R *U, double gamma, double rho_floor, double p_floor)
 {
-    Prim W;
+    PrimT<R> W;
     W.rho = mx(U[0], rho_floor);
     W.u = U[1] / W.rho; W.v = U[2] / W.rho; W.w = U[3] / W.rho;
-    const double ke = 0.5 * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w);
+    const R ke = 0.5 * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w);
     W.p = mx((gamma - 1.0) * (U[4] - ke), p_floor);
     return W;
 }
-__device__ __forceinline__ void to_cons(const Prim &W, double gamma, double *U)
+template <class R> __device__ __forceinline__ void to_cons(const PrimT<R> &W, double gamma, R *U)
 {
     U[0] = W.rho; U[1] = W.rho * W.u; U[2] = W.rho * W.v; U[3] = W.rho * W.w;
     U[4] = W.p / (gamma - 1.0) + 0.5 * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w);
 }
 
 // Common/physics/Scales.cpp:50-62
-struct NumInfo { double nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa; };
-__device__ __forceinline__ NumInfo numinfo(const Params &P, double rhoH, double rhoNa)
+template <class R> struct NumInfoT { R nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa; };
+template <class R> __device__ __forceinline__ NumInfoT<R> numinfo(const Params &P, R rhoH, R rhoNa)
 {
-    NumInfo n;
+    NumInfoT<R> n;
     n.nH = mx(rhoH, 0.0) / P.M_H * P.M_ref;
     n.nNa = mx(rhoNa, 0.0) / P.M_Na * P.M_ref;
     n.ne = n.nH + n.nNa;

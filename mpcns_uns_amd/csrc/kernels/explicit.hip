@@ -9,28 +9,28 @@ namespace mhdg
 namespace
 {
 // src/uns/geometry/Reconstruct.cpp:11-45 for one cell
-__device__ V3 cell_vector(const DevMesh &d, int32_t c, const double *flux)
+template <class R> __device__ V3T<R> cell_vector(const DevMesh &d, int32_t c, const R *flux)
 {
-    double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, r[3] = {0, 0, 0};
+    R M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, r[3] = {0, 0, 0};
     for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
     {
         const int32_t v = d.c2f[k], f = pid(v);
-        const double A = d.famag[f];
+        const R A = d.famag[f];
         if (A <= 0) continue;
-        const V3 n = (psg(v) / A) * L3(d.farea, f);
-        const double nv[3] = {n.x, n.y, n.z};
-        const double phi = psg(v) * flux[f];
+        const V3T<R> n = (psg(v) / A) * L3(d.farea, f);
+        const R nv[3] = {n.x, n.y, n.z};
+        const R phi = psg(v) * flux[f];
         for (int i = 0; i < 3; ++i)
         {
             r[i] += nv[i] * phi;
             for (int j = 0; j < 3; ++j) M[i][j] += A * nv[i] * nv[j];
         }
     }
-    const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+    const R det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
                        M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
-    const double sc = mx(mx(mx(M[0][0], M[1][1]), M[2][2]), 1e-300);
-    if (fabs(det) < 1e-12 * sc * sc * sc) return V3();
-    double I[3][3];
+    const R sc = mx(mx(mx(M[0][0], M[1][1]), M[2][2]), 1e-300);
+    if (fabs(det) < 1e-12 * sc * sc * sc) return V3T<R>();
+    R I[3][3];
     I[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / det;
     I[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) / det;
     I[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) / det;
@@ -45,25 +45,25 @@ __device__ V3 cell_vector(const DevMesh &d, int32_t c, const double *flux)
 }
 
 // MercuryUns/fluid/MercurySources.cpp:104-145 for one fluid cell (+inf if no face restricts it)
-__device__ double cell_dt(const DevMesh &d, const Params &P, int32_t c, const double *wH, const double *wNa, const V3 &b)
+template <class R> __device__ R cell_dt(const DevMesh &d, const Params &P, int32_t c, const R *wH, const R *wNa, const V3T<R> &b)
 {
-    double dt = __builtin_inf();
-    const double B2 = dot(b, b);
-    const double rho_tot = mx(wH[0] + wNa[0], P.rho_floor_dt);
+    R dt = __builtin_inf();
+    const R B2 = dot(b, b);
+    const R rho_tot = mx(wH[0] + wNa[0], P.rho_floor_dt);
     for (int s = 0; s < 2; ++s)
     {
-        const double *w = s == 0 ? wH : wNa;
-        const double cs2 = P.gamma * w[4] / mx(w[0], 1e-30);
-        double denom = 0;
+        const R *w = s == 0 ? wH : wNa;
+        const R cs2 = P.gamma * w[4] / mx(w[0], 1e-30);
+        R denom = 0;
         for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
         {
             const int32_t f = pid(d.c2f[k]);
-            const double A = d.famag[f];
-            const V3 n = (1.0 / A) * L3(d.farea, f);
-            const double vA2 = P.inver_MA2 * B2 / rho_tot;
-            const double vAn2 = P.inver_MA2 * dot(b, n) * dot(b, n) / rho_tot;
-            const double term = cs2 + vA2;
-            const double cf = sqrt(0.5 * (term + sqrt(mx(0.0, term * term - 4 * cs2 * vAn2))));
+            const R A = d.famag[f];
+            const V3T<R> n = (1.0 / A) * L3(d.farea, f);
+            const R vA2 = P.inver_MA2 * B2 / rho_tot;
+            const R vAn2 = P.inver_MA2 * dot(b, n) * dot(b, n) / rho_tot;
+            const R term = cs2 + vA2;
+            const R cf = sqrt(0.5 * (term + sqrt(mx(0.0, term * term - 4 * cs2 * vAn2))));
             denom += (fabs(w[1] * n.x + w[2] * n.y + w[3] * n.z) + cf) * A;
         }
         if (denom > 0) dt = mn(dt, P.cfl * d.cvol[c] / denom);
@@ -71,7 +71,7 @@ __device__ double cell_dt(const DevMesh &d, const Params &P, int32_t c, const do
     return dt;
 }
 
-__device__ __forceinline__ double block_min(double v, double *sh)
+template <class R> __device__ __forceinline__ R block_min(R v, R *sh)
 {
     const int t = threadIdx.x;
     sh[t] = v;
@@ -87,45 +87,45 @@ __device__ __forceinline__ double block_min(double v, double *sh)
 
 // Solver::prepare (MercuryUns/driver/Solver.cpp:94-125) + cell_time_steps + n_e,eff / alpha of the
 // frozen Hall state (Solver.cpp:257-260, ImplicitInduction.cpp:105-111).  One thread per cell.
-__global__ void __launch_bounds__(256) k_prepare(DevMesh d, Params P, const double *U0, const double *U1, const double *Phi,
-                                                 const double *Badd, double *W0, double *W1, double *Bind, double *Btot,
-                                                 double *uplus, double *alpha, double *dt_part)
+template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, Params P, const R *U0, const R *U1, const R *Phi,
+                                                 const R *Badd, R *W0, R *W1, R *Bind, R *Btot,
+                                                 R *uplus, R *alpha, R *dt_part)
 {
-    __shared__ double sh[256];
+    __shared__ R sh[256];
     const int32_t c = gid();
-    double dtc = __builtin_inf();
+    R dtc = __builtin_inf();
     if (c < d.nc)
     {
-        const Prim a = to_prim(U0 + 5 * size_t(c), P.gamma, P.rho_floor, P.p_floor);
-        const Prim b = to_prim(U1 + 5 * size_t(c), P.gamma, P.rho_floor, P.p_floor);
-        const double wa[5] = {a.rho, a.u, a.v, a.w, a.p}, wb[5] = {b.rho, b.u, b.v, b.w, b.p};
+        const PrimT<R> a = to_prim(U0 + 5 * size_t(c), P.gamma, P.rho_floor, P.p_floor);
+        const PrimT<R> b = to_prim(U1 + 5 * size_t(c), P.gamma, P.rho_floor, P.p_floor);
+        const R wa[5] = {a.rho, a.u, a.v, a.w, a.p}, wb[5] = {b.rho, b.u, b.v, b.w, b.p};
         for (int m = 0; m < 5; ++m) { W0[5 * size_t(c) + m] = wa[m]; W1[5 * size_t(c) + m] = wb[m]; }
-        const V3 bi = cell_vector(d, c, Phi);
-        const V3 ba = L3(Badd, c);
-        const V3 bt(bi.x + ba.x, bi.y + ba.y, bi.z + ba.z);
+        const V3T<R> bi = cell_vector(d, c, Phi);
+        const V3T<R> ba = L3(Badd, c);
+        const V3T<R> bt(bi.x + ba.x, bi.y + ba.y, bi.z + ba.z);
         st3(Bind, c, bi);
         st3(Btot, c, bt);
-        V3 up;
-        double al = 0.0;
+        V3T<R> up;
+        R al = 0.0;
         if (d.mat[c] == 0)
         {
-            const NumInfo ni = numinfo(P, wa[0], wb[0]);
-            up = V3(ni.chiH * wa[1] + ni.chiNa * wb[1], ni.chiH * wa[2] + ni.chiNa * wb[2], ni.chiH * wa[3] + ni.chiNa * wb[3]);
+            const NumInfoT<R> ni = numinfo(P, wa[0], wb[0]);
+            up = V3T<R>(ni.chiH * wa[1] + ni.chiNa * wb[1], ni.chiH * wa[2] + ni.chiNa * wb[2], ni.chiH * wa[3] + ni.chiNa * wb[3]);
             al = (P.hall && ni.ne_eff > 0) ? d.taper[c] * P.hall_coef / ni.ne_eff : 0.0;
             dtc = cell_dt(d, P, c, wa, wb, bt);
         }
         st3(uplus, c, up);
         alpha[c] = al;
     }
-    const double m = block_min(dtc, sh);
+    const R m = block_min(dtc, sh);
     if (threadIdx.x == 0) dt_part[blockIdx.x] = m;
 }
 
 // Final min of the block minima; out[0] = min(dt, dt_max).
-__global__ void __launch_bounds__(256) k_dt_final(const double *part, int32_t n, double dt_max, double *out)
+template <class R> __global__ void __launch_bounds__(256) k_dt_final(const R *part, int32_t n, double dt_max, R *out)
 {
-    __shared__ double sh[256];
-    double v = __builtin_inf();
+    __shared__ R sh[256];
+    R v = __builtin_inf();
     for (int32_t i = threadIdx.x; i < n; i += blockDim.x) v = mn(v, part[i]);
     v = block_min(v, sh);
     if (threadIdx.x == 0) out[0] = mn(v, dt_max);
@@ -135,10 +135,10 @@ __global__ void __launch_bounds__(256) k_dt_final(const double *part, int32_t n,
 // (Common/induction/Current.cpp:9-31).  ghost: optional per-face table of 5 primitives per species.
 // Each block stages its cells' 15-value gradient rows in LDS and writes them as one contiguous range (the
 // per-thread 120-byte rows written directly left partially written cache lines behind: 2.2x write traffic).
-__global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpC Ws, SpC ghosts, SpW grads,
-                                                      int32_t limit, const double *Bind, double *J)
+template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads,
+                                                      int32_t limit, const R *Bind, R *J)
 {
-    __shared__ double stage[256 * 15];
+    __shared__ R stage[256 * 15];
     const int32_t c = gid();
     const int32_t c_first = blockIdx.x * blockDim.x;
     const int32_t n_here = min(int32_t(blockDim.x), d.nc - c_first);
@@ -146,38 +146,39 @@ __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32
     const int32_t k0 = act ? d.c2f_ptr[c] : 0, k1 = act ? d.c2f_ptr[c + 1] : 0;
     if (act && J)
     {
-        V3 sum;
-        const V3 bc = L3(Bind, c);
+        V3T<R> sum;
+        const V3T<R> bc = L3(Bind, c);
         for (int32_t k = k0; k < k1; ++k)
         {
             const int32_t v = d.c2f[k], f = pid(v);
-            V3 bf = bc;
+            V3T<R> bf = bc;
             const int32_t l = d.fL[f], r = d.fR[f];
             if (l >= 0 && r >= 0) bf = 0.5 * (bc + L3(Bind, l == c ? r : l));
             sum += cross(psg(v) * L3(d.farea, f), bf);
         }
-        const double invV = 1.0 / d.cvol[c];
-        st3(J, c, V3(sum.x * invV, sum.y * invV, sum.z * invV));
+        const R invV = 1.0 / d.cvol[c];
+        st3(J, c, V3T<R>(sum.x * invV, sum.y * invV, sum.z * invV));
     }
-    const V3 xc = act ? L3(d.ccen, c) : V3();
+    const V3T<R> xc = act ? L3(d.ccen, c) : V3T<R>();
     for (int32_t s = 0; s < ns; ++s)
     {
         const int32_t sp = s0 + s;
-        double *out = stage + 15 * threadIdx.x;
+        R *out = stage + 15 * threadIdx.x;
         if (!act || !d.svalid[c])
             for (int q = 0; q < 15; ++q) out[q] = 0.0;
         else
         {
-        const double *W = Ws.p[s];
-        const double *gh = ghosts.p[s];
-        const double *fs = P.fs[sp];
-        double wc[5], r[5][3], lo[5], hi[5];
+        const R *W = Ws.p[s];
+        const R *gh = ghosts.p[s];
+        R fs[5];
+        for (int m = 0; m < 5; ++m) fs[m] = seeded<R>(P.fs[sp][m], P.dfs[sp][m]);
+        R wc[5], r[5][3], lo[5], hi[5];
         for (int m = 0; m < 5; ++m) { wc[m] = W[5 * size_t(c) + m]; lo[m] = hi[m] = wc[m]; r[m][0] = r[m][1] = r[m][2] = 0.0; }
         for (int32_t k = k0; k < k1; ++k)
         {
             const int32_t nb = d.snb[k];
             if (nb == kNoStencil) continue;
-            double wn[5];
+            R wn[5];
             if (nb >= 0)
                 for (int m = 0; m < 5; ++m) wn[m] = W[5 * size_t(nb) + m];
             else
@@ -195,8 +196,8 @@ __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32
                         if (sp == 1) { wn[0] = 1e-12; wn[1] = 0; wn[2] = 0; wn[3] = 0; wn[4] = 1e-12; }
                         else
                         {
-                            const V3 n = (psg(d.c2f[k]) / d.famag[f]) * L3(d.farea, f);
-                            const double un = wc[1] * n.x + wc[2] * n.y + wc[3] * n.z;
+                            const V3T<R> n = (psg(d.c2f[k]) / d.famag[f]) * L3(d.farea, f);
+                            const R un = wc[1] * n.x + wc[2] * n.y + wc[3] * n.z;
                             wn[1] -= 2 * un * n.x; wn[2] -= 2 * un * n.y; wn[3] -= 2 * un * n.z;
                         }
                     }
@@ -204,36 +205,36 @@ __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32
                         for (int m = 0; m < 5; ++m) wn[m] = fs[m];
                 }
             }
-            const double w = d.sw[k];
-            const V3 dd = L3(d.sd, k);
+            const R w = d.sw[k];
+            const V3T<R> dd = L3(d.sd, k);
             for (int m = 0; m < 5; ++m)
             {
-                const double dq = wn[m] - wc[m];
+                const R dq = wn[m] - wc[m];
                 r[m][0] += w * dd.x * dq; r[m][1] += w * dd.y * dq; r[m][2] += w * dd.z * dq;
                 lo[m] = mn(lo[m], wn[m]);
                 hi[m] = mx(hi[m], wn[m]);
             }
         }
         const double *I6 = d.sinv + 6 * size_t(c);
-        const double I[9] = {I6[0], I6[1], I6[2], I6[1], I6[3], I6[4], I6[2], I6[4], I6[5]};
+        const R I[9] = {I6[0], I6[1], I6[2], I6[1], I6[3], I6[4], I6[2], I6[4], I6[5]};
         for (int m = 0; m < 5; ++m)
         {
-            const double g[3] = {I[0] * r[m][0] + I[1] * r[m][1] + I[2] * r[m][2], I[3] * r[m][0] + I[4] * r[m][1] + I[5] * r[m][2],
+            const R g[3] = {I[0] * r[m][0] + I[1] * r[m][1] + I[2] * r[m][2], I[3] * r[m][0] + I[4] * r[m][1] + I[5] * r[m][2],
                                  I[6] * r[m][0] + I[7] * r[m][1] + I[8] * r[m][2]};
-            double psi = 1.0;
+            R psi = 1.0;
             if (limit)
                 for (int32_t k = k0; k < k1; ++k)
                 {
-                    const V3 dx = L3(d.fcen, pid(d.c2f[k])) - xc;
-                    const double dl = g[0] * dx.x + g[1] * dx.y + g[2] * dx.z;
+                    const V3T<R> dx = L3(d.fcen, pid(d.c2f[k])) - xc;
+                    const R dl = g[0] * dx.x + g[1] * dx.y + g[2] * dx.z;
                     if (dl > 1e-300) psi = mn(psi, (hi[m] - wc[m]) / dl);
                     else if (dl < -1e-300) psi = mn(psi, (lo[m] - wc[m]) / dl);
                 }
             if (limit && P.rel_limit > 0 && (m == 0 || m == 4))
                 for (int32_t k = k0; k < k1; ++k)
                 {
-                    const V3 dx = L3(d.fcen, pid(d.c2f[k])) - xc;
-                    const double dl = fabs(g[0] * dx.x + g[1] * dx.y + g[2] * dx.z);
+                    const V3T<R> dx = L3(d.fcen, pid(d.c2f[k])) - xc;
+                    const R dl = fabs(g[0] * dx.x + g[1] * dx.y + g[2] * dx.z);
                     if (dl > 1e-300) psi = mn(psi, P.rel_limit * wc[m] / dl);
                 }
             psi = mx(0.0, mn(1.0, psi));
@@ -241,7 +242,7 @@ __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32
         }
         }
         __syncthreads();
-        double *dst = grads.p[s] + 15 * size_t(c_first);
+        R *dst = grads.p[s] + 15 * size_t(c_first);
         for (int32_t i = threadIdx.x; i < 15 * n_here; i += blockDim.x) dst[i] = stage[i];
         __syncthreads();
     }
@@ -250,50 +251,50 @@ __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32
 namespace
 {
 // Common/fluid/FluxRHS.cpp:16-27
-__device__ __forceinline__ Prim face_state(const double *W, const double *grad, int32_t c, const V3 &dx)
+template <class R> __device__ __forceinline__ PrimT<R> face_state(const R *W, const R *grad, int32_t c, const V3T<R> &dx)
 {
-    const double *w = W + 5 * size_t(c);
-    const double *g = grad + 15 * size_t(c);
-    Prim r;
+    const R *w = W + 5 * size_t(c);
+    const R *g = grad + 15 * size_t(c);
+    PrimT<R> r;
     r.rho = w[0] + g[0] * dx.x + g[1] * dx.y + g[2] * dx.z;
     r.u = w[1] + g[3] * dx.x + g[4] * dx.y + g[5] * dx.z;
     r.v = w[2] + g[6] * dx.x + g[7] * dx.y + g[8] * dx.z;
     r.w = w[3] + g[9] * dx.x + g[10] * dx.y + g[11] * dx.z;
     r.p = w[4] + g[12] * dx.x + g[13] * dx.y + g[14] * dx.z;
     if (r.rho > 0 && r.p > 0) return r;
-    return Prim{w[0], w[1], w[2], w[3], w[4]};
+    return PrimT<R>{w[0], w[1], w[2], w[3], w[4]};
 }
 
 // Common/physics/Euler.h:29-49
-__device__ void rusanov(const Prim &L, const Prim &R, const double n[3], double gamma, double *F)
+template <class R> __device__ void rusanov(const PrimT<R> &L, const PrimT<R> &Rt, const R n[3], double gamma, R *F)
 {
-    double UL[5], UR[5], FL[5], FR[5];
+    R UL[5], UR[5], FL[5], FR[5];
     to_cons(L, gamma, UL);
-    to_cons(R, gamma, UR);
-    const double unL = L.u * n[0] + L.v * n[1] + L.w * n[2];
+    to_cons(Rt, gamma, UR);
+    const R unL = L.u * n[0] + L.v * n[1] + L.w * n[2];
     FL[0] = L.rho * unL; FL[1] = L.rho * unL * L.u + L.p * n[0]; FL[2] = L.rho * unL * L.v + L.p * n[1];
     FL[3] = L.rho * unL * L.w + L.p * n[2]; FL[4] = unL * (UL[4] + L.p);
-    const double unR = R.u * n[0] + R.v * n[1] + R.w * n[2];
-    FR[0] = R.rho * unR; FR[1] = R.rho * unR * R.u + R.p * n[0]; FR[2] = R.rho * unR * R.v + R.p * n[1];
-    FR[3] = R.rho * unR * R.w + R.p * n[2]; FR[4] = unR * (UR[4] + R.p);
-    const double sL = fabs(L.u * n[0] + L.v * n[1] + L.w * n[2]) + sqrt(gamma * L.p / L.rho);
-    const double sR = fabs(R.u * n[0] + R.v * n[1] + R.w * n[2]) + sqrt(gamma * R.p / R.rho);
-    const double s = mx(sL, sR);
+    const R unR = Rt.u * n[0] + Rt.v * n[1] + Rt.w * n[2];
+    FR[0] = Rt.rho * unR; FR[1] = Rt.rho * unR * Rt.u + Rt.p * n[0]; FR[2] = Rt.rho * unR * Rt.v + Rt.p * n[1];
+    FR[3] = Rt.rho * unR * Rt.w + Rt.p * n[2]; FR[4] = unR * (UR[4] + Rt.p);
+    const R sL = fabs(L.u * n[0] + L.v * n[1] + L.w * n[2]) + sqrt(gamma * L.p / L.rho);
+    const R sR = fabs(Rt.u * n[0] + Rt.v * n[1] + Rt.w * n[2]) + sqrt(gamma * Rt.p / Rt.rho);
+    const R s = mx(sL, sR);
     for (int m = 0; m < 5; ++m) F[m] = 0.5 * (FL[m] + FR[m]) - 0.5 * s * (UR[m] - UL[m]);
 }
 
 // ghost_state (Common/fluid/Reconstruction.cpp:9-24)
-__device__ __forceinline__ Prim ghost_state(int8_t kind, int species, const Prim &in, const V3 &n, const double *fs)
+template <class R> __device__ __forceinline__ PrimT<R> ghost_state(int8_t kind, int species, const PrimT<R> &in, const V3T<R> &n, const R *fs)
 {
     if (kind == kWall)
     {
-        if (species == 1) return Prim{1e-12, 0, 0, 0, 1e-12};
-        const double un = in.u * n.x + in.v * n.y + in.w * n.z;
-        Prim g = in;
+        if (species == 1) return PrimT<R>{1e-12, 0, 0, 0, 1e-12};
+        const R un = in.u * n.x + in.v * n.y + in.w * n.z;
+        PrimT<R> g = in;
         g.u -= 2 * un * n.x; g.v -= 2 * un * n.y; g.w -= 2 * un * n.z;
         return g;
     }
-    if (kind == kInflow) return Prim{fs[0], fs[1], fs[2], fs[3], fs[4]};
+    if (kind == kInflow) return PrimT<R>{fs[0], fs[1], fs[2], fs[3], fs[4]};
     return in;
 }
 } // namespace
@@ -301,44 +302,44 @@ __device__ __forceinline__ Prim ghost_state(int8_t kind, int species, const Prim
 // Face kernel: fluid_flux_rhs's face body (Common/fluid/FluxRHS.cpp:36-73) for `ns` species, storing
 // F*|A| per face and species (FA, 5*ns per face, zero for skipped faces), and optionally the ideal EMF
 // face candidate E_f (Common/induction/IdealEMF.cpp:32-55, 3 per face).
-__global__ void __launch_bounds__(256) k_face(DevMesh d, Params P, int32_t ns, int32_t s0, SpC Ws, SpC grads, SpC ghosts, double *FA,
-                                              const double *uplus, const double *Btot, const double *Bind, const double *Phi,
-                                              const double *PhiAdd, double *Ef)
+template <class R> __global__ void __launch_bounds__(256) k_face(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> grads, SpCT<R> ghosts, R *FA,
+                                              const R *uplus, const R *Btot, const R *Bind, const R *Phi,
+                                              const R *PhiAdd, R *Ef)
 {
     // up to two species the block's flux rows are staged in LDS and written as one contiguous range
-    __shared__ double stage[256 * 10];
+    __shared__ R stage[256 * 10];
     const int32_t f = gid();
     const bool act = f < d.nf;
     const bool staged = FA && ns <= 2;
     const int32_t l = act ? d.fL[f] : -1, r = act ? d.fR[f] : -1;
-    const double A = act ? d.famag[f] : 1.0;
-    const V3 n = act ? (1.0 / A) * L3(d.farea, f) : V3();
+    const R A = act ? d.famag[f] : 1.0;
+    const V3T<R> n = act ? (1.0 / A) * L3(d.farea, f) : V3T<R>();
     if (FA && act)
     {
         const int8_t kind = d.kind[f];
         const bool any = (l >= 0 && d.mat[l] == 0) || (r >= 0 && d.mat[r] == 0);
-        double *out = staged ? stage + 5 * ns * threadIdx.x : FA + size_t(5 * ns) * f;
+        R *out = staged ? stage + 5 * ns * threadIdx.x : FA + size_t(5 * ns) * f;
         if (kind == kSkip || !any)
             for (int q = 0; q < 5 * ns; ++q) out[q] = 0.0;
         else
         {
-            const double nv[3] = {n.x, n.y, n.z};
-            const V3 xf = L3(d.fcen, f);
+            const R nv[3] = {n.x, n.y, n.z};
+            const V3T<R> xf = L3(d.fcen, f);
             for (int32_t s = 0; s < ns; ++s)
             {
                 const int32_t sp = s0 + s;
-                const double *W = Ws.p[s], *G = grads.p[s];
-                Prim L, R;
+                const R *W = Ws.p[s], *G = grads.p[s];
+                PrimT<R> L, Rt;
                 if (l >= 0 && r >= 0 && kind == kInterior)
                 {
                     L = face_state(W, G, l, xf - L3(d.ccen, l));
-                    R = face_state(W, G, r, xf - L3(d.ccen, r));
+                    Rt = face_state(W, G, r, xf - L3(d.ccen, r));
                 }
                 else
                 {
                     // the single fluid cell; f2c order puts the lower cell id first (FluxRHS.cpp:57-62)
                     int32_t c;
-                    double sg;
+                    R sg;
                     if (l >= 0 && r >= 0)
                     {
                         const int32_t first = l < r ? l : r, second = l < r ? r : l;
@@ -346,18 +347,23 @@ __global__ void __launch_bounds__(256) k_face(DevMesh d, Params P, int32_t ns, i
                     }
                     else c = l >= 0 ? l : r;
                     sg = c == l ? 1.0 : -1.0;
-                    const Prim in = face_state(W, G, c, xf - L3(d.ccen, c));
-                    Prim gh;
+                    const PrimT<R> in = face_state(W, G, c, xf - L3(d.ccen, c));
+                    PrimT<R> gh;
                     if (ghosts.p[s])
                     {
-                        const double *q = ghosts.p[s] + 5 * size_t(f);
-                        gh = Prim{q[0], q[1], q[2], q[3], q[4]};
+                        const R *q = ghosts.p[s] + 5 * size_t(f);
+                        gh = PrimT<R>{q[0], q[1], q[2], q[3], q[4]};
                     }
-                    else gh = ghost_state(kind, sp, in, sg * n, P.fs[sp]);
-                    if (sg > 0) { L = in; R = gh; } else { R = in; L = gh; }
+                    else
+                    {
+                        R fsv[5];
+                        for (int m = 0; m < 5; ++m) fsv[m] = seeded<R>(P.fs[sp][m], P.dfs[sp][m]);
+                        gh = ghost_state(kind, sp, in, sg * n, fsv);
+                    }
+                    if (sg > 0) { L = in; Rt = gh; } else { Rt = in; L = gh; }
                 }
-                double F[5];
-                rusanov(L, R, nv, P.gamma, F);
+                R F[5];
+                rusanov(L, Rt, nv, P.gamma, F);
                 for (int m = 0; m < 5; ++m) out[5 * s + m] = F[m] * A;
             }
         }
@@ -366,25 +372,25 @@ __global__ void __launch_bounds__(256) k_face(DevMesh d, Params P, int32_t ns, i
     {
         __syncthreads();
         const int32_t f_first = blockIdx.x * blockDim.x, n_here = min(int32_t(blockDim.x), d.nf - f_first);
-        double *dst = FA + size_t(5 * ns) * f_first;
+        R *dst = FA + size_t(5 * ns) * f_first;
         for (int32_t i = threadIdx.x; i < 5 * ns * n_here; i += blockDim.x) dst[i] = stage[i];
     }
     if (Ef && act)
     {
-        const double Bn_tot = (PhiAdd ? Phi[f] + PhiAdd[f] : Phi[f]) / A;
-        V3 F;
+        const R Bn_tot = (PhiAdd ? Phi[f] + PhiAdd[f] : Phi[f]) / A;
+        V3T<R> F;
         if (l >= 0 && r >= 0)
         {
-            const V3 uL = L3(uplus, l), uR = L3(uplus, r);
-            const V3 fLx = dot(uL, n) * L3(Btot, l) - Bn_tot * uL;
-            const V3 fRx = dot(uR, n) * L3(Btot, r) - Bn_tot * uR;
-            const double a = mx(fabs(dot(uL, n)), fabs(dot(uR, n)));
+            const V3T<R> uL = L3(uplus, l), uR = L3(uplus, r);
+            const V3T<R> fLx = dot(uL, n) * L3(Btot, l) - Bn_tot * uL;
+            const V3T<R> fRx = dot(uR, n) * L3(Btot, r) - Bn_tot * uR;
+            const R a = mx(fabs(dot(uL, n)), fabs(dot(uR, n)));
             F = 0.5 * (fLx + fRx) - (0.5 * a) * (L3(Bind, r) - L3(Bind, l));
         }
         else
         {
             const int32_t c = l >= 0 ? l : r;
-            const V3 u = L3(uplus, c);
+            const V3T<R> u = L3(uplus, c);
             F = dot(u, n) * L3(Btot, c) - Bn_tot * u;
         }
         st3(Ef, f, -1.0 * cross(n, F));
@@ -395,20 +401,20 @@ __global__ void __launch_bounds__(256) k_face(DevMesh d, Params P, int32_t ns, i
 // FluxRHS.cpp:71-72), Lorentz sources (MercurySources.cpp:20-63), and - when dt != 0 - the forward Euler
 // update with floors (Solver.cpp:246-252, 171-191).  rhs_out (nullable) receives the rhs.
 // FA nullable (no flux term), rhs_in nullable (start from zero); species of slot s is s0 + s.
-__global__ void __launch_bounds__(256) k_cell_update(DevMesh d, Params P, int32_t ns, int32_t s0, const double *FA, SpC rhs_in,
-                                                     const double *W0, const double *W1, const double *Btot, const double *J,
-                                                     const double *dt_dev, double dt_val, int32_t sources, int32_t update, SpW Us,
-                                                     SpW rhs_out)
+template <class R> __global__ void __launch_bounds__(256) k_cell_update(DevMesh d, Params P, int32_t ns, int32_t s0, const R *FA, SpCT<R> rhs_in,
+                                                     const R *W0, const R *W1, const R *Btot, const R *J,
+                                                     const R *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<R> Us,
+                                                     SpWT<R> rhs_out)
 {
     const int32_t c = gid();
     if (c >= d.nc) return;
     const bool fluid = d.mat[c] == 0;
-    const double dt = dt_dev ? dt_dev[0] : dt_val;
-    const double V = d.cvol[c];
+    const R dt = dt_dev ? dt_dev[0] : dt_val;
+    const R V = d.cvol[c];
     for (int32_t s = 0; s < ns; ++s)
     {
         const int32_t sp = s0 + s;
-        double rhs[5] = {0, 0, 0, 0, 0};
+        R rhs[5] = {0, 0, 0, 0, 0};
         if (rhs_in.p[s])
             for (int m = 0; m < 5; ++m) rhs[m] = rhs_in.p[s][5 * size_t(c) + m];
         if (fluid)
@@ -417,18 +423,18 @@ __global__ void __launch_bounds__(256) k_cell_update(DevMesh d, Params P, int32_
             for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
             {
                 const int32_t v = d.cfs[k], f = pid(v);
-                const double *fa = FA + size_t(5 * ns) * f + 5 * s;
+                const R *fa = FA + size_t(5 * ns) * f + 5 * s;
                 if (v >= 0) for (int m = 0; m < 5; ++m) rhs[m] -= fa[m] / V;
                 else for (int m = 0; m < 5; ++m) rhs[m] += fa[m] / V;
             }
             if (sources)
             {
-                const double *W = sp == 0 ? W0 : W1;
-                const NumInfo ni = numinfo(P, W0[5 * size_t(c)], W1[5 * size_t(c)]);
-                const double ws = sp == 0 ? ni.wH : ni.wNa;
-                const double rho = mx(W[5 * size_t(c)], 0.0);
-                const V3 u(W[5 * size_t(c) + 1], W[5 * size_t(c) + 2], W[5 * size_t(c) + 3]);
-                V3 S;
+                const R *W = sp == 0 ? W0 : W1;
+                const NumInfoT<R> ni = numinfo(P, W0[5 * size_t(c)], W1[5 * size_t(c)]);
+                const R ws = sp == 0 ? ni.wH : ni.wNa;
+                const R rho = mx(W[5 * size_t(c)], 0.0);
+                const V3T<R> u(W[5 * size_t(c) + 1], W[5 * size_t(c) + 2], W[5 * size_t(c) + 3]);
+                V3T<R> S;
                 if (P.lorentz) S += (P.inver_MA2 * ws) * cross(L3(J, c), L3(Btot, c));
                 rhs[1] += S.x; rhs[2] += S.y; rhs[3] += S.z;
                 rhs[4] += dot(u, S) + (rho > 1e-30 ? 0.5 * dt * dot(S, S) / rho : 0.0);
@@ -438,14 +444,14 @@ __global__ void __launch_bounds__(256) k_cell_update(DevMesh d, Params P, int32_
             for (int m = 0; m < 5; ++m) rhs_out.p[s][5 * size_t(c) + m] = rhs[m];
         if (update && fluid)
         {
-            double *u = Us.p[s] + 5 * size_t(c);
-            double q[5];
+            R *u = Us.p[s] + 5 * size_t(c);
+            R q[5];
             for (int m = 0; m < 5; ++m) q[m] = u[m] + dt * rhs[m];
-            const double ke = q[0] > 0 ? 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0] : 0.0;
-            if (!(q[0] >= P.rho_floor)) to_cons(Prim{P.rho_floor, 0, 0, 0, P.p_floor}, P.gamma, q);
+            const R ke = q[0] > 0 ? 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0] : 0.0;
+            if (!(q[0] >= P.rho_floor)) to_cons(PrimT<R>{P.rho_floor, 0, 0, 0, P.p_floor}, P.gamma, q);
             else if ((P.gamma - 1.0) * (q[4] - ke) < P.p_floor)
             {
-                const Prim w = to_prim(q, P.gamma, P.rho_floor, P.p_floor);
+                const PrimT<R> w = to_prim(q, P.gamma, P.rho_floor, P.p_floor);
                 to_cons(w, P.gamma, q);
             }
             for (int m = 0; m < 5; ++m) u[m] = q[m];
@@ -464,9 +470,9 @@ __device__ __forceinline__ void face_dirs(const DevMesh &d, int32_t f, int32_t e
     m = nv > 0 ? (1.0 / nv) * v : V3();
     nrm = (1.0 / d.famag[f]) * L3(d.farea, f);
 }
-__device__ __forceinline__ double edge_amax(const DevMesh &d, const double *u_cell, int32_t c0, int32_t c1, const V3 &m)
+template <class R> __device__ __forceinline__ R edge_amax(const DevMesh &d, const R *u_cell, int32_t c0, int32_t c1, const V3T<R> &m)
 {
-    double a = 0.0;
+    R a = 0.0;
     for (int32_t k = c0; k < c1; ++k) a = mx(a, fabs(dot(L3(u_cell, d.ec_cell[k]), m)));
     return a;
 }
@@ -478,11 +484,11 @@ __device__ __forceinline__ void plane_basis(const V3 &t, V3 &e1, V3 &e2)
     e2 = cross(t, e1);
 }
 // eps-regularised 2x2 solve of the omega correction (IdealEMF.cpp:110-118)
-__device__ __forceinline__ void uct_solve(double M11, double M12, double M22, double g1, double g2, double &l1, double &l2)
+template <class R> __device__ __forceinline__ void uct_solve(R M11, R M12, R M22, R g1, R g2, R &l1, R &l2)
 {
-    const double tr = M11 + M22;
-    const double eps = 1e-14 * tr;
-    const double det = (M11 + eps) * (M22 + eps) - M12 * M12;
+    const R tr = M11 + M22;
+    const R eps = 1e-14 * tr;
+    const R det = (M11 + eps) * (M22 + eps) - M12 * M12;
     l1 = 0;
     l2 = 0;
     if (tr > 0 && det > 0)
@@ -493,11 +499,13 @@ __device__ __forceinline__ void uct_solve(double M11, double M12, double M22, do
 }
 constexpr int kEmfFast = 6; // valence served from registers by k_edge_emf
 // boundary EMF of Solver::rhs (MercuryUns/driver/Solver.cpp:162-167)
-__device__ __forceinline__ double edge_bc(const DevMesh &d, const Params &P, int32_t e, double v)
+template <class R> __device__ __forceinline__ R edge_bc(const DevMesh &d, const Params &P, int32_t e, R v)
 {
     const int8_t mode = d.emode[e];
     if (mode == 1) return 0.0;
-    if (mode == 2) return dot(V3(P.E_inf[0], P.E_inf[1], P.E_inf[2]), L3(d.evec, e));
+    if (mode == 2)
+        return dot(V3T<R>(seeded<R>(P.E_inf[0], P.dE_inf[0]), seeded<R>(P.E_inf[1], P.dE_inf[1]), seeded<R>(P.E_inf[2], P.dE_inf[2])),
+                   L3(d.evec, e));
     return v;
 }
 } // namespace
@@ -505,15 +513,15 @@ __device__ __forceinline__ double edge_bc(const DevMesh &d, const Params &P, int
 // add_ideal_emf edge loop (Common/induction/IdealEMF.cpp:59-128), one thread per edge with valence at most
 // `vmax`; larger edges are left to k_edge_emf_heavy.  E_out[e] = E_in[e] + E_ideal (E_in nullable = 0);
 // apply_bc additionally applies the boundary EMF of Solver::rhs.
-__global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const double *Ef, const double *u_cell, const double *phi_ind,
-                                                  double cuct, int32_t vmax, const double *E_in, double *E_out, int32_t apply_bc)
+template <class R> __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const R *Ef, const R *u_cell, const R *phi_ind,
+                                                  double cuct, int32_t vmax, const R *E_in, R *E_out, int32_t apply_bc)
 {
     const int32_t e = gid();
     if (e >= d.ne) return;
     const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
     const int32_t nf = f1 - f0;
     if (nf > vmax) return;
-    double out = E_in ? E_in[e] : 0.0;
+    R out = E_in ? E_in[e] : 0.0;
     const int32_t c0 = d.ec_ptr[e], c1 = d.ec_ptr[e + 1];
     bool any_fluid = false;
     for (int32_t k = c0; k < c1 && !any_fluid; ++k) any_fluid = d.mat[d.ec_cell[k]] == 0;
@@ -522,21 +530,21 @@ __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const dou
         const V3 t = L3(d.evec, e);
         const double L = d.elen[e];
         const V3 th = (1.0 / L) * t;
-        double Ec = 0;
+        R Ec = 0;
         for (int32_t i = f0; i < f1; ++i) Ec += d.efw[i] * dot(L3(Ef, pid(d.e2f[i])), t);
         V3 e1, e2;
         plane_basis(th, e1, e2);
-        double M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0, Eu = 0;
+        R M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0, Eu = 0;
         const int32_t nc = c1 - c0;
         if (nf <= kEmfFast && nc <= kEmfFast)
         {
             // common valence: the edge's cell velocities and the per-face terms stay in registers, so the
             // second pass re-reads nothing (same operations in the same order as the general path)
-            V3 uc[kEmfFast];
+            V3T<R> uc[kEmfFast];
 #pragma unroll
             for (int k = 0; k < kEmfFast; ++k)
                 if (k < nc) uc[k] = L3(u_cell, d.ec_cell[c0 + k]);
-            double fqa[kEmfFast], fqb[kEmfFast], fw[kEmfFast], faef[kEmfFast], fsig[kEmfFast], fpf[kEmfFast];
+            R fqa[kEmfFast], fqb[kEmfFast], fw[kEmfFast], faef[kEmfFast], fsig[kEmfFast], fpf[kEmfFast];
 #pragma unroll
             for (int i = 0; i < kEmfFast; ++i)
                 if (i < nf)
@@ -544,13 +552,13 @@ __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const dou
                     const int32_t f = pid(d.e2f[f0 + i]);
                     V3 m, nrm;
                     face_dirs(d, f, e, th, m, nrm);
-                    double aef = 0.0;
+                    R aef = 0.0;
 #pragma unroll
                     for (int k = 0; k < kEmfFast; ++k)
                         if (k < nc) aef = mx(aef, fabs(dot(uc[k], m)));
-                    const double sig = dot(th, cross(m, nrm));
-                    const V3 qv = (aef * sig) * nrm;
-                    const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[f0 + i];
+                    const R sig = dot(th, cross(m, nrm));
+                    const V3T<R> qv = (aef * sig) * nrm;
+                    const R qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[f0 + i];
                     M11 += w * qa * qa;
                     M12 += w * qa * qb;
                     M22 += w * qb * qb;
@@ -558,13 +566,13 @@ __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const dou
                     g2 += w * qb;
                     fqa[i] = qa; fqb[i] = qb; fw[i] = w; faef[i] = aef; fsig[i] = sig; fpf[i] = phi_ind[f] / d.famag[f];
                 }
-            double l1, l2;
+            R l1, l2;
             uct_solve(M11, M12, M22, g1, g2, l1, l2);
 #pragma unroll
             for (int i = 0; i < kEmfFast; ++i)
                 if (i < nf)
                 {
-                    const double omega = fw[i] * (1.0 - (fqa[i] * l1 + fqb[i] * l2));
+                    const R omega = fw[i] * (1.0 - (fqa[i] * l1 + fqb[i] * l2));
                     Eu += omega * faef[i] * fsig[i] * fpf[i];
                 }
         }
@@ -575,27 +583,27 @@ __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const dou
                 const int32_t f = pid(d.e2f[i]);
                 V3 m, nrm;
                 face_dirs(d, f, e, th, m, nrm);
-                const double aef = edge_amax(d, u_cell, c0, c1, m);
-                const V3 qv = (aef * dot(th, cross(m, nrm))) * nrm;
-                const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
+                const R aef = edge_amax(d, u_cell, c0, c1, m);
+                const V3T<R> qv = (aef * dot(th, cross(m, nrm))) * nrm;
+                const R qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
                 M11 += w * qa * qa;
                 M12 += w * qa * qb;
                 M22 += w * qb * qb;
                 g1 += w * qa;
                 g2 += w * qb;
             }
-            double l1, l2;
+            R l1, l2;
             uct_solve(M11, M12, M22, g1, g2, l1, l2);
             for (int32_t i = f0; i < f1; ++i)
             {
                 const int32_t f = pid(d.e2f[i]);
                 V3 m, nrm;
                 face_dirs(d, f, e, th, m, nrm);
-                const double aef = edge_amax(d, u_cell, c0, c1, m);
-                const V3 qv = (aef * dot(th, cross(m, nrm))) * nrm;
-                const double qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
-                const double omega = w * (1.0 - (qa * l1 + qb * l2));
-                const double sig = dot(th, cross(m, nrm));
+                const R aef = edge_amax(d, u_cell, c0, c1, m);
+                const V3T<R> qv = (aef * dot(th, cross(m, nrm))) * nrm;
+                const R qa = dot(qv, e1), qb = dot(qv, e2), w = d.efw[i];
+                const R omega = w * (1.0 - (qa * l1 + qb * l2));
+                const R sig = dot(th, cross(m, nrm));
                 Eu += omega * aef * sig * (phi_ind[f] / d.famag[f]);
             }
         }
@@ -606,12 +614,13 @@ __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, Params P, const dou
 
 // The same for one high-valence edge per block (pole axes of a lat-lon net): per-face terms in parallel,
 // every sum sequential in the reference's order.
-__global__ void __launch_bounds__(256) k_edge_emf_heavy(DevMesh d, Params P, const double *Ef, const double *u_cell,
-                                                        const double *phi_ind, double cuct, const int32_t *heavy,
-                                                        const double *E_in, double *E_out, int32_t apply_bc)
+template <class R> __global__ void __launch_bounds__(256) k_edge_emf_heavy(DevMesh d, Params P, const R *Ef, const R *u_cell,
+                                                        const R *phi_ind, double cuct, const int32_t *heavy,
+                                                        const R *E_in, R *E_out, int32_t apply_bc)
 {
-    __shared__ double s_w[kHeavyMax], s_ec[kHeavyMax], s_qa[kHeavyMax], s_qb[kHeavyMax], s_sig[kHeavyMax], s_aef[kHeavyMax];
-    __shared__ double s_l[2];
+    constexpr int kHS = kHeavyShared<R>();
+    __shared__ R s_w[kHS], s_ec[kHS], s_qa[kHS], s_qb[kHS], s_sig[kHS], s_aef[kHS];
+    __shared__ R s_l[2];
     const int32_t e = heavy[blockIdx.x];
     const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1], nf = f1 - f0;
     const int32_t c0 = d.ec_ptr[e], c1 = d.ec_ptr[e + 1];
@@ -631,19 +640,19 @@ __global__ void __launch_bounds__(256) k_edge_emf_heavy(DevMesh d, Params P, con
             s_ec[i] = dot(L3(Ef, f), t);
             V3 m, nrm;
             face_dirs(d, f, e, th, m, nrm);
-            const double aef = edge_amax(d, u_cell, c0, c1, m);
-            const double sig = dot(th, cross(m, nrm));
-            const V3 qv = (aef * sig) * nrm;
+            const R aef = edge_amax(d, u_cell, c0, c1, m);
+            const R sig = dot(th, cross(m, nrm));
+            const V3T<R> qv = (aef * sig) * nrm;
             s_qa[i] = dot(qv, e1);
             s_qb[i] = dot(qv, e2);
             s_sig[i] = sig;
             s_aef[i] = aef;
         }
     __syncthreads();
-    double Ec = 0;
+    R Ec = 0;
     if (threadIdx.x == 0 && active)
     {
-        double M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0;
+        R M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0;
         for (int32_t i = 0; i < nf; ++i)
         {
             Ec += s_w[i] * s_ec[i];
@@ -653,10 +662,10 @@ __global__ void __launch_bounds__(256) k_edge_emf_heavy(DevMesh d, Params P, con
             g1 += s_w[i] * s_qa[i];
             g2 += s_w[i] * s_qb[i];
         }
-        const double tr = M11 + M22;
-        const double eps = 1e-14 * tr;
-        const double det = (M11 + eps) * (M22 + eps) - M12 * M12;
-        double l1 = 0, l2 = 0;
+        const R tr = M11 + M22;
+        const R eps = 1e-14 * tr;
+        const R det = (M11 + eps) * (M22 + eps) - M12 * M12;
+        R l1 = 0, l2 = 0;
         if (tr > 0 && det > 0)
         {
             l1 = ((M22 + eps) * g1 - M12 * g2) / det;
@@ -669,16 +678,16 @@ __global__ void __launch_bounds__(256) k_edge_emf_heavy(DevMesh d, Params P, con
         for (int32_t i = threadIdx.x; i < nf; i += blockDim.x)
         {
             const int32_t f = pid(d.e2f[f0 + i]);
-            const double omega = s_w[i] * (1.0 - (s_qa[i] * s_l[0] + s_qb[i] * s_l[1]));
+            const R omega = s_w[i] * (1.0 - (s_qa[i] * s_l[0] + s_qb[i] * s_l[1]));
             s_ec[i] = omega * s_aef[i] * s_sig[i] * (phi_ind[f] / d.famag[f]);
         }
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        double out = E_in ? E_in[e] : 0.0;
+        R out = E_in ? E_in[e] : 0.0;
         if (active)
         {
-            double Eu = 0;
+            R Eu = 0;
             for (int32_t i = 0; i < nf; ++i) Eu += s_ec[i];
             out += Ec + 2.0 * cuct * L * Eu;
         }
@@ -687,13 +696,13 @@ __global__ void __launch_bounds__(256) k_edge_emf_heavy(DevMesh d, Params P, con
 }
 
 // faraday_update / faraday_update_edge_dt (Common/induction/Faraday.cpp:15-36): B -= dt d1 E.
-__global__ void __launch_bounds__(256) k_faraday(DevMesh d, const double *E, const double *dte, const double *dt_dev, double dt_val,
-                                                 double *B)
+template <class R> __global__ void __launch_bounds__(256) k_faraday(DevMesh d, const R *E, const double *dte, const R *dt_dev, double dt_val,
+                                                 R *B)
 {
     const int32_t f = gid();
     if (f >= d.nf) return;
     if (d.fL[f] < 0 && d.fR[f] < 0) return;
-    double circ = 0.0;
+    R circ = 0.0;
     if (dte)
     {
         for (int32_t k = d.f2e_ptr[f]; k < d.f2e_ptr[f + 1]; ++k)
@@ -710,7 +719,7 @@ __global__ void __launch_bounds__(256) k_faraday(DevMesh d, const double *E, con
             const int32_t v = d.f2e[k];
             circ += psg(v) * E[pid(v)];
         }
-        const double dt = dt_dev ? dt_dev[0] : dt_val;
+        const R dt = dt_dev ? dt_dev[0] : dt_val;
         B[f] -= dt * circ;
     }
 }
@@ -769,4 +778,52 @@ __global__ void __launch_bounds__(256) k_local_dt(DevMesh d, Params P, const dou
     const double m = block_min(dtc, sh);
     if (threadIdx.x == 0) dt_part[blockIdx.x] = m;
 }
+
+// |B|^2 V summed over the fluid cells with r0 <= |x_c| <= r1, B = reconstructed cell field of phi + B_add:
+// per-block partial sums in a fixed tree order
+template <class R>
+__global__ void __launch_bounds__(256) k_shell_energy(DevMesh d, const R *phi, const R *badd, double r0, double r1, R *part)
+{
+    __shared__ R sh[256];
+    const int32_t c = gid();
+    R e = 0.0;
+    if (c < d.nc && d.mat[c] == 0)
+    {
+        const V3 x = L3(d.ccen, c);
+        const double r = norm(x);
+        if (r >= r0 && r <= r1)
+        {
+            const V3T<R> b = cell_vector(d, c, phi) + L3(badd, c);
+            e = dot(b, b) * d.cvol[c];
+        }
+    }
+    sh[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// the step (double) and its tangent-linear (Dual)
+template __global__ void k_shell_energy<double>(DevMesh, const double *, const double *, double, double, double *);
+template __global__ void k_shell_energy<Dual>(DevMesh, const Dual *, const Dual *, double, double, Dual *);
+template __global__ void k_prepare<double>(DevMesh d, Params P, const double *U0, const double *U1, const double *Phi, const double *Badd, double *W0, double *W1, double *Bind, double *Btot, double *uplus, double *alpha, double *dt_part);
+template __global__ void k_dt_final<double>(const double *part, int32_t n, double dt_max, double *out);
+template __global__ void k_grad_current<double>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> ghosts, SpWT<double> grads, int32_t limit, const double *Bind, double *J);
+template __global__ void k_face<double>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> grads, SpCT<double> ghosts, double *FA, const double *uplus, const double *Btot, const double *Bind, const double *Phi, const double *PhiAdd, double *Ef);
+template __global__ void k_cell_update<double>(DevMesh d, Params P, int32_t ns, int32_t s0, const double *FA, SpCT<double> rhs_in, const double *W0, const double *W1, const double *Btot, const double *J, const double *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<double> Us, SpWT<double> rhs_out);
+template __global__ void k_edge_emf<double>(DevMesh d, Params P, const double *Ef, const double *u_cell, const double *phi_ind, double cuct, int32_t vmax, const double *E_in, double *E_out, int32_t apply_bc);
+template __global__ void k_edge_emf_heavy<double>(DevMesh d, Params P, const double *Ef, const double *u_cell, const double *phi_ind, double cuct, const int32_t *heavy, const double *E_in, double *E_out, int32_t apply_bc);
+template __global__ void k_faraday<double>(DevMesh d, const double *E, const double *dte, const double *dt_dev, double dt_val, double *B);
+template __global__ void k_prepare<Dual>(DevMesh d, Params P, const Dual *U0, const Dual *U1, const Dual *Phi, const Dual *Badd, Dual *W0, Dual *W1, Dual *Bind, Dual *Btot, Dual *uplus, Dual *alpha, Dual *dt_part);
+template __global__ void k_dt_final<Dual>(const Dual *part, int32_t n, double dt_max, Dual *out);
+template __global__ void k_grad_current<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<Dual> Ws, SpCT<Dual> ghosts, SpWT<Dual> grads, int32_t limit, const Dual *Bind, Dual *J);
+template __global__ void k_face<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<Dual> Ws, SpCT<Dual> grads, SpCT<Dual> ghosts, Dual *FA, const Dual *uplus, const Dual *Btot, const Dual *Bind, const Dual *Phi, const Dual *PhiAdd, Dual *Ef);
+template __global__ void k_cell_update<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, const Dual *FA, SpCT<Dual> rhs_in, const Dual *W0, const Dual *W1, const Dual *Btot, const Dual *J, const Dual *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<Dual> Us, SpWT<Dual> rhs_out);
+template __global__ void k_edge_emf<Dual>(DevMesh d, Params P, const Dual *Ef, const Dual *u_cell, const Dual *phi_ind, double cuct, int32_t vmax, const Dual *E_in, Dual *E_out, int32_t apply_bc);
+template __global__ void k_edge_emf_heavy<Dual>(DevMesh d, Params P, const Dual *Ef, const Dual *u_cell, const Dual *phi_ind, double cuct, const int32_t *heavy, const Dual *E_in, Dual *E_out, int32_t apply_bc);
+template __global__ void k_faraday<Dual>(DevMesh d, const Dual *E, const double *dte, const Dual *dt_dev, double dt_val, Dual *B);
 } // namespace mhdg

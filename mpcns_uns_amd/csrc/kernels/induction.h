@@ -35,6 +35,8 @@ __global__ void k_field_of_flux(const double *Badd, const double *b, int64_t n, 
 __global__ void k_hall_g_tangent(DevMesh d, const double *b, const double *db, const double *alpha, const double *Badd, double *g);
 __global__ void k_newton_residual(const double *E, const double *E_expl, const double *G, const int8_t *dir, int32_t n, double *R);
 __global__ void k_scaled(const double *x, double s, int32_t n, double *out);
+__global__ void k_seed(const double *v, const double *dv, int64_t n, Dual *out);
+__global__ void k_tangent_of(const Dual *x, int64_t n, double *d);
 __global__ void k_solve_final(const double *x0, const double *delta, const double *E_expl, const int8_t *dir, int32_t n, double *E,
                               double *corr);
 } // namespace mhdg

@@ -6,6 +6,8 @@
 // quadratic in Phi, and the step R(E) = E - E_expl - G(Phi^n - dt d1 E) = 0 is solved by Newton's method with
 // matrix-free Jacobian products  J v = v + dt G'(Phi) d1 v,  whose Hall part is the tangent kernel below:
 //     g'_c = alpha_c ( J_c[db] x (B_add,c + b_c) + J_c[b] x db_c ).
+#include <type_traits>
+
 #include "ctx.h"
 
 namespace mhdg
@@ -159,7 +161,7 @@ int mhdg_ctx::newton(const double *Phi_n, const double *Eex, const int8_t *dir, 
     for (;; ++k)
     {
         chk(hipMemcpyAsync(nt.phik, Phi_n, size_t(nf) * 8, hipMemcpyDeviceToDevice, stream), "copy");
-        LAUNCH("faraday", k_faraday, dim3(nbk(nf)), dim3(256), 0, stream, d, nt.E, static_cast<const double *>(nullptr),
+        LAUNCH("faraday", k_faraday<double>, dim3(nbk(nf)), dim3(256), 0, stream, d, nt.E, static_cast<const double *>(nullptr),
                static_cast<const double *>(nullptr), dt, nt.phik);
         apply_G(nt.phik, nt.G);
         LAUNCH("newton_residual", k_newton_residual, dim3(nbk(ne)), dim3(256), 0, stream, nt.E, Eex, nt.G, dir, ne, nt.R);
@@ -182,3 +184,102 @@ int mhdg_ctx::newton(const double *Phi_n, const double *Eex, const int8_t *dir, 
     if (rel_res) *rel_res = rn / bnorm;
     return k;
 }
+
+// ------------------------------------------------------------------ tangent-linear of the explicit step
+namespace mhdg
+{
+// value + tangent of a double array: out[i] = {v[i], dv ? dv[i] : 0}
+__global__ void __launch_bounds__(256) k_seed(const double *v, const double *dv, int64_t n, Dual *out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = Dual(v[i], dv ? dv[i] : 0.0);
+}
+// split a Dual array into its tangent part
+__global__ void __launch_bounds__(256) k_tangent_of(const Dual *x, int64_t n, double *d)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = x[i].d;
+}
+} // namespace mhdg
+
+void mhdg_ctx::tangent_alloc()
+{
+    if (tw.U[0]) return;
+    if (n_heavy)
+    {
+        std::vector<int32_t> hv(n_heavy), ptr(ne + 1);
+        chk(hipMemcpy(hv.data(), heavy, sizeof(int32_t) * n_heavy, hipMemcpyDeviceToHost), "D2H");
+        chk(hipMemcpy(ptr.data(), d.e2f_ptr, sizeof(int32_t) * (ne + 1), hipMemcpyDeviceToHost), "D2H");
+        for (int32_t e : hv)
+            if (ptr[e + 1] - ptr[e] > kHeavyShared<Dual>())
+                throw std::runtime_error("tangent step: edge valence above " + std::to_string(kHeavyShared<Dual>()));
+    }
+    auto D = [&](size_t n) { return static_cast<Dual *>(alloc(n * sizeof(Dual))); };
+    const size_t C = nc, F = nf, E = ne;
+    tw.U[0] = D(5 * C); tw.U[1] = D(5 * C); tw.Phi = D(F); tw.PhiAdd = D(F); tw.Badd = D(3 * C);
+    tw.W[0] = D(5 * C); tw.W[1] = D(5 * C); tw.grad[0] = D(15 * C); tw.grad[1] = D(15 * C);
+    tw.Bind = D(3 * C); tw.Btot = D(3 * C); tw.uplus = D(3 * C); tw.J = D(3 * C); tw.alpha = D(C);
+    tw.FA = D(10 * F); tw.Ef = D(3 * F); tw.E = D(E);
+    tw.dt_part = D(size_t(nbk(nc)) + 1); tw.dt_dev = D(1); tw.part = D(size_t(nbk(nc)));
+    energy_part = static_cast<double *>(alloc(size_t(nbk(nc)) * 8));
+}
+
+// n steps of step_first_order's explicit stage (prepare, CFL, gradients, fluxes, sources, update, UCT EMF with
+// the boundary EMF, CT) on the Dual state tw, with the seeds of the free-stream states and of the inflow EMF in Pt
+void mhdg_ctx::explicit_tangent(int n_steps, double dt_max, const Params &Pt)
+{
+    const int nb = nbk(nc);
+    SpCT<Dual> Ws{}, none{}, gradsC{};
+    SpWT<Dual> grads{}, Us{}, noneW{};
+    Ws.p[0] = tw.W[0]; Ws.p[1] = tw.W[1];
+    grads.p[0] = tw.grad[0]; grads.p[1] = tw.grad[1];
+    gradsC.p[0] = tw.grad[0]; gradsC.p[1] = tw.grad[1];
+    Us.p[0] = tw.U[0]; Us.p[1] = tw.U[1];
+    for (int s = 0; s < n_steps; ++s)
+    {
+        LAUNCH("prepare_t", k_prepare<Dual>, dim3(nb), dim3(256), 0, stream, d, Pt, tw.U[0], tw.U[1], tw.Phi, tw.Badd, tw.W[0], tw.W[1],
+               tw.Bind, tw.Btot, tw.uplus, tw.alpha, tw.dt_part);
+        LAUNCH("dt_final_t", k_dt_final<Dual>, dim3(1), dim3(256), 0, stream, tw.dt_part, nb, dt_max, tw.dt_dev);
+        if (!Pt.second_order)
+        {
+            chk(hipMemsetAsync(tw.grad[0], 0, 15 * size_t(nc) * sizeof(Dual), stream), "memset");
+            chk(hipMemsetAsync(tw.grad[1], 0, 15 * size_t(nc) * sizeof(Dual), stream), "memset");
+        }
+        LAUNCH("grad_current_t", k_grad_current<Dual>, dim3(nb), dim3(256), 0, stream, d, Pt, Pt.second_order ? 2 : 0, 0, Ws, none, grads,
+               1, static_cast<const Dual *>(tw.Bind), tw.J);
+        LAUNCH("face_t", k_face<Dual>, dim3(nbk(nf)), dim3(256), 0, stream, d, Pt, 2, 0, Ws, gradsC, none, tw.FA,
+               static_cast<const Dual *>(tw.uplus), static_cast<const Dual *>(tw.Btot), static_cast<const Dual *>(tw.Bind),
+               static_cast<const Dual *>(tw.Phi), static_cast<const Dual *>(tw.PhiAdd), tw.Ef);
+        LAUNCH("cell_update_t", k_cell_update<Dual>, dim3(nb), dim3(256), 0, stream, d, Pt, 2, 0, static_cast<const Dual *>(tw.FA), none,
+               static_cast<const Dual *>(tw.W[0]), static_cast<const Dual *>(tw.W[1]), static_cast<const Dual *>(tw.Btot),
+               static_cast<const Dual *>(tw.J), static_cast<const Dual *>(tw.dt_dev), 0.0, 1, 1, Us, noneW);
+        LAUNCH("edge_emf_t", k_edge_emf<Dual>, dim3(nbk(ne)), dim3(256), 0, stream, d, Pt, static_cast<const Dual *>(tw.Ef),
+               static_cast<const Dual *>(tw.uplus), static_cast<const Dual *>(tw.Phi), Pt.cuct, kLightMax, static_cast<const Dual *>(nullptr),
+               tw.E, 1);
+        if (n_heavy)
+            LAUNCH("edge_emf_heavy_t", k_edge_emf_heavy<Dual>, dim3(n_heavy), dim3(256), 0, stream, d, Pt, static_cast<const Dual *>(tw.Ef),
+                   static_cast<const Dual *>(tw.uplus), static_cast<const Dual *>(tw.Phi), Pt.cuct, static_cast<const int32_t *>(heavy),
+                   static_cast<const Dual *>(nullptr), tw.E, 1);
+        LAUNCH("faraday_t", k_faraday<Dual>, dim3(nbk(nf)), dim3(256), 0, stream, d, static_cast<const Dual *>(tw.E),
+               static_cast<const double *>(nullptr), static_cast<const Dual *>(tw.dt_dev), 0.0, tw.Phi);
+        check_launch("explicit tangent step");
+    }
+}
+
+template <class R> void mhdg_ctx::shell_energy(const R *phi, const R *badd, double r0, double r1, R *out)
+{
+    const int nb = nbk(nc);
+    R *part = nullptr;
+    if constexpr (std::is_same<R, Dual>::value) part = tw.part;
+    else part = energy_part;
+    hipLaunchKernelGGL(k_shell_energy<R>, dim3(nb), dim3(256), 0, stream, d, phi, badd, r0, r1, part);
+    check_launch("shell energy");
+    std::vector<R> h(nb);
+    chk(hipMemcpyAsync(h.data(), part, sizeof(R) * nb, hipMemcpyDeviceToHost, stream), "D2H");
+    chk(hipStreamSynchronize(stream), "sync");
+    R s = 0.0;
+    for (int i = 0; i < nb; ++i) s += h[i];
+    *out = s;
+}
+template void mhdg_ctx::shell_energy<double>(const double *, const double *, double, double, double *);
+template void mhdg_ctx::shell_energy<Dual>(const Dual *, const Dual *, double, double, Dual *);

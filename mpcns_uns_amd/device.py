@@ -144,6 +144,24 @@ class Device:
                                        rtol, max_newton, eta, ptr(E), C.byref(k), C.byref(g), C.byref(rr)))
         return E, k.value, g.value, rr.value
 
+    def shell_energy(self, r0, r1):
+        """sum over fluid cells with r0 <= |x_c| <= r1 of |B_ind + B_add|^2 V of the resident state."""
+        J = C.c_double()
+        self._ok(lib().mhdg_shell_energy(self._h, r0, r1, C.byref(J)))
+        return J.value
+
+    def explicit_tangent(self, n_steps, r0, r1, dU_H=None, dU_Na=None, dPhi=None, dPhiAdd=None, dBadd=None, dfs=None, dE_inf=None,
+                         dt_max=1e9, states=False):
+        """Forward-mode derivative of n explicit steps from the resident state along the given seeds:
+        (J, dJ) of the shell energy of the final state, and the final state tangents when `states`."""
+        J, dJ = C.c_double(), C.c_double()
+        opt = lambda a: None if a is None else f64(a)
+        seeds = [opt(a) for a in (dU_H, dU_Na, dPhi, dPhiAdd, dBadd, dfs, dE_inf)]
+        outs = [np.zeros(5 * self.nc), np.zeros(5 * self.nc), np.zeros(self.nf)] if states else [None, None, None]
+        self._ok(lib().mhdg_explicit_tangent(self._h, n_steps, dt_max, *[ptr(a) for a in seeds], r0, r1, C.byref(J), C.byref(dJ),
+                                             *[ptr(a) for a in outs]))
+        return (J.value, dJ.value, *outs) if states else (J.value, dJ.value)
+
     # ------------------------------------------------------------------ resident state
     def set_state(self, U_H, U_Na, Phi):
         self._ok(lib().mhdg_set_state(self._h, ptr(f64(U_H)), ptr(f64(U_Na)), ptr(f64(Phi))))

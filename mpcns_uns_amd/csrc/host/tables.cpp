@@ -28,8 +28,49 @@ void pack_csr(const mhdg_incidence &a, int32_t rows, std::vector<int32_t> &ptr, 
 }
 } // namespace
 
+// A descriptor is checked completely on the host before anything reaches the device: every index a kernel
+// follows must stay inside its table.
+static void check_csr(const mhdg_incidence &t, int64_t rows, int64_t cols, const char *name, bool need_sgn = true)
+{
+    const std::string n(name);
+    if (!t.ptr || !t.idx || (need_sgn && !t.sgn)) throw std::runtime_error("mesh descriptor: " + n + " missing");
+    if (t.ptr[0] != 0) throw std::runtime_error("mesh descriptor: " + n + ".ptr[0] != 0");
+    for (int64_t r = 0; r < rows; ++r)
+        if (t.ptr[r + 1] < t.ptr[r]) throw std::runtime_error("mesh descriptor: " + n + ".ptr not monotonic");
+    if (t.ptr[rows] > int64_t(INT32_MAX)) throw std::runtime_error("mesh descriptor: " + n + " has more than 2^31 entries");
+    for (int64_t k = 0; k < t.ptr[rows]; ++k)
+    {
+        if (t.idx[k] < 0 || t.idx[k] >= cols) throw std::runtime_error("mesh descriptor: " + n + ".idx out of range");
+        if (need_sgn && t.sgn[k] != 1 && t.sgn[k] != -1) throw std::runtime_error("mesh descriptor: " + n + ".sgn not +-1");
+    }
+}
+void check_desc(const mhdg_mesh_desc &d)
+{
+    if (d.n_cells <= 0 || d.n_faces <= 0 || d.n_edges <= 0) throw std::runtime_error("mesh descriptor: empty mesh");
+    check_csr(d.c2f, d.n_cells, d.n_faces, "c2f");
+    check_csr(d.f2e, d.n_faces, d.n_edges, "f2e");
+    check_csr(d.e2f, d.n_edges, d.n_faces, "e2f");
+    check_csr(d.f2c, d.n_faces, d.n_cells, "f2c");
+    if (d.f2e.ptr[d.n_faces] != d.e2f.ptr[d.n_edges] || d.c2f.ptr[d.n_cells] != d.f2c.ptr[d.n_faces])
+        throw std::runtime_error("mesh descriptor: e2f / f2c are not the transposes of f2e / c2f");
+    const void *need[] = {d.cell_material, d.face_kind, d.edge_mode, d.face_area, d.face_center, d.face_area_mag, d.cell_center,
+                          d.cell_volume, d.edge_vec, d.edge_mid, d.edge_len, d.ec_ptr, d.ec_cell, d.ec_weight, d.ef_weight, d.star1, d.star2};
+    for (const void *p : need)
+        if (!p) throw std::runtime_error("mesh descriptor: a geometry or boundary array is missing");
+    if (d.ec_ptr[0] != 0) throw std::runtime_error("mesh descriptor: ec_ptr[0] != 0");
+    for (int64_t e = 0; e < d.n_edges; ++e)
+        if (d.ec_ptr[e + 1] < d.ec_ptr[e]) throw std::runtime_error("mesh descriptor: ec_ptr not monotonic");
+    for (int64_t k = 0; k < d.ec_ptr[d.n_edges]; ++k)
+        if (d.ec_cell[k] < 0 || d.ec_cell[k] >= d.n_cells) throw std::runtime_error("mesh descriptor: ec_cell out of range");
+    for (int64_t c = 0; c < d.n_cells; ++c)
+        if (d.cell_material[c] != 0 && d.cell_material[c] != 1) throw std::runtime_error("mesh descriptor: cell_material not 0/1");
+    for (int64_t f = 0; f < d.n_faces; ++f)
+        if (d.face_kind[f] < 0 || d.face_kind[f] > 4) throw std::runtime_error("mesh descriptor: face_kind not 0..4");
+}
+
 Tables build_tables(const mhdg_mesh_desc &d, const mhdg_config &cfg, int32_t light_max)
 {
+    check_desc(d);
     Tables t;
     t.nc = d.n_cells; t.nf = d.n_faces; t.ne = d.n_edges;
     const int32_t nc = t.nc, nf = t.nf, ne = t.ne;

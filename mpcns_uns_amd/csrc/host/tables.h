@@ -24,5 +24,7 @@ struct Tables
 };
 
 // Throws std::runtime_error on inconsistent input.
+// Checks every count, CSR pointer and index of a descriptor (throws std::runtime_error naming the defect).
+void check_desc(const mhdg_mesh_desc &d);
 Tables build_tables(const mhdg_mesh_desc &d, const mhdg_config &cfg, int32_t light_max);
 } // namespace mhdg

@@ -421,6 +421,8 @@ int mhdg_create(const mhdg_mesh_desc *mesh, const mhdg_config *cfg, mhdg_ctx **o
     auto *c = new mhdg_ctx();
     try
     {
+        if (!mesh || !cfg) throw std::runtime_error("mhdg_create: null descriptor");
+        check_desc(*mesh); // before any device call: a bad descriptor is reported as such, with or without a GPU
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw std::runtime_error("no HIP device available");
         c->build(*mesh, *cfg);

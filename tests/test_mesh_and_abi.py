@@ -129,3 +129,56 @@ def test_shim_compiles_against_reference_headers():
            "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "mpcns_uns_amd", "shim", "mhd_gpu_shim.cpp")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("defect, message", [
+    ("f2e_index", "f2e.idx out of range"),
+    ("c2f_sign", "c2f.sgn not"),
+    ("e2f_ptr", "e2f.ptr not monotonic"),
+    ("ec_cell", "ec_cell out of range"),
+    ("null_area", "missing"),
+    ("transpose", "transposes"),
+])
+def test_create_rejects_bad_descriptors(defect, message):
+    """Every index a kernel follows is checked on the host before any device call: a corrupt descriptor is an
+    error with a message, never an out-of-bounds access on the GPU (checked here without a GPU)."""
+    import ctypes as C
+
+    from mpcns_uns_amd import workloads
+    from mpcns_uns_amd._lib import MeshDesc, lib
+
+    w = workloads.build(net=(6, 12, 2), hall=False)
+    m, cfg = w["mesh"], w["cfg"]
+    d = MeshDesc.from_buffer_copy(m.desc())
+    keep = []
+
+    def patched(arr, dtype, ctype):
+        a = np.array(arr, dtype=dtype)
+        keep.append(a)
+        return a, a.ctypes.data_as(C.POINTER(ctype))
+
+    if defect == "f2e_index":
+        a, p = patched(m.array("f2e.idx"), np.int32, C.c_int32)
+        a[7] = m.n_edges
+        d.f2e.idx = p
+    elif defect == "c2f_sign":
+        a, p = patched(m.array("c2f.sgn"), np.int8, C.c_int8)
+        a[3] = 0
+        d.c2f.sgn = p
+    elif defect == "e2f_ptr":
+        a, p = patched(m.array("e2f.ptr"), np.int64, C.c_int64)
+        a[5], a[6] = a[6], a[5] - 1
+        d.e2f.ptr = p
+    elif defect == "ec_cell":
+        a, p = patched(m.array("ec_cell"), np.int32, C.c_int32)
+        a[-1] = -1
+        d.ec_cell = p
+    elif defect == "null_area":
+        d.face_area = None
+    elif defect == "transpose":
+        a, p = patched(m.array("e2f.ptr"), np.int64, C.c_int64)
+        a[-1] = a[-2]  # last edge loses its faces: e2f no longer matches f2e
+        d.e2f.ptr = p
+    h = C.c_void_p()
+    assert lib().mhdg_create(C.byref(d), C.byref(cfg), C.byref(h)) != 0
+    assert message in lib().mhdg_last_error(None).decode()

@@ -16,8 +16,8 @@
 // boundary info or the configuration it was built without (setup only).  Arrays stay host-side
 // std::vectors owned by the drivers: each call copies its inputs to the device and its outputs back.
 //
-// Scope: one rank (the Exchanger is the identity, every entity is owned), species 0 (H+) and 1 (Na+) for the
-// fluid calls.  A GhostFn is evaluated on the host once per boundary face and species into the ghost table
+// Scope: one rank (the Exchanger is the identity, every entity is owned); the fluid calls serve any species
+// index below 10 with the caller's free-stream state (the ion slots of MarsSolver's ten-ion fluid).  A GhostFn is evaluated on the host once per boundary face and species into the ghost table
 // of the device call, with the first-order cell state as `inside` (exact for ghost functions that ignore the
 // inside state, which is the only use MarsSolver makes of it: MarsSolver.cpp:124-128, 148-167).
 // The implicit solve runs in FP64 (the reference iteration) unless MHDG_SOLVE_PRECISION=1 selects the

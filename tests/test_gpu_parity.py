@@ -268,3 +268,19 @@ def test_restart_from_field_file(small_explicit, tmp_path):
     assert dev.step() == orc.step()
     for a, b in zip(dev.get_state(), orc.get_state()):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("species", [2, 5, 9])
+def test_fluid_calls_for_ten_ion_species(pair, species):
+    """The per-call fluid entries serve any species index below 10 with the caller's free-stream state (the ion
+    slots of a ten-ion run): bitwise against the oracle."""
+    dev, orc, w, f = pair
+    rng = np.random.default_rng(40 + species)
+    W = f["W_H"].copy().reshape(-1, 5)
+    W[:, 0] *= rng.uniform(0.5, 1.5, W.shape[0])
+    W = W.ravel()
+    fs = np.array([1e-4 * species, -1.0, 0.0, 0.0, 1e-5])
+    for limit in (True, False):
+        g = orc.limited_gradients(W, species, fs, limit)
+        assert np.array_equal(dev.limited_gradients(W, species, fs, limit), g)
+        assert np.array_equal(dev.flux_rhs(species, fs, W, g), orc.flux_rhs(species, fs, W, g))

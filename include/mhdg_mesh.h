@@ -55,6 +55,17 @@ int mhdg_mesh_desc_fill(const mhdg_mesh *m, mhdg_mesh_desc *desc);
  * reconstructed cell vectors. */
 int mhdg_mesh_background(const mhdg_mesh *m, const double B_imf[3], int dipole, const double dip_m[3],
                          const double dip_c[3], double soft, double *PhiAdd, double *BaddCell);
+/* Background with a crustal field of point dipoles (mars::MarsSolver::build_background_,
+ * MarsUns/driver/MarsSolver.cpp:169-200; MARS::DipoleModel, MarsUns/physics/MarsDipoles.cpp:63-107):
+ * PhiAdd = B_imf.A_f + d1(E_A), E_A the edge integrals of A = sum_i m_i x (x-c_i)/(|x-c_i|^2+core_i^2)^{3/2}
+ * (compensated sum over the dipoles in order, two-point Gauss rule on each edge, dipoles farther than `cutoff`
+ * skipped, cutoff 0 = none); BaddCell its reconstructed cell vectors.  pos, mom 3n; core n (nondimensional). */
+int mhdg_mesh_background_dipoles(const mhdg_mesh *m, const double B_imf[3], int32_t n, const double *pos,
+                                 const double *mom, const double *core, double cutoff, double *PhiAdd,
+                                 double *BaddCell);
+/* The edge integrals alone for edges from xa[3e] to xb[3e] (MarsDipoles.cpp:92-107). */
+int mhdg_dipole_edge_integrals(int32_t n, const double *pos, const double *mom, const double *core, double cutoff,
+                               int64_t n_edges, const double *xa, const double *xb, double *out);
 /* uns::reconstruct_cell_vector on the host (Reconstruct.cpp:8-46). */
 int mhdg_mesh_cell_vector(const mhdg_mesh *m, const double *flux, double *b);
 /* Seeded synthetic state: free stream fs[species] in every cell with relative Gaussian noise `noise` on

@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <random>
 #include <stdexcept>
@@ -28,6 +29,58 @@ struct mhdg_fields
 {
     std::vector<mhdg::unsbin::Field> f;
 };
+
+namespace mhdg
+{
+// Crustal field of point dipoles (MarsUns/physics/MarsDipoles.cpp:63-107): vector potential
+// A(x) = sum_i m_i x (x - c_i) / (|x - c_i|^2 + core_i^2)^{3/2}, summed over the dipoles in order with a
+// compensated (Kahan) sum per component, dipoles beyond the cutoff distance (cut2 > 0) skipped; the edge
+// integral of A by the symmetric two-point Gauss rule on the straight edge.
+static void dipole_potential(int32_t n, const double *pos, const double *mom, const double *core, double cut2, const double x[3],
+                             double out[3])
+{
+    double sum[3] = {0, 0, 0}, corr[3] = {0, 0, 0};
+    for (int32_t i = 0; i < n; ++i)
+    {
+        double r[3], d2 = 0;
+        for (int k = 0; k < 3; ++k) { r[k] = x[k] - pos[3 * i + k]; d2 += r[k] * r[k]; }
+        if (!std::isfinite(d2)) throw std::runtime_error("dipoles: non-finite evaluation distance");
+        if (cut2 > 0 && d2 > cut2) continue;
+        const double r2 = d2 + core[i] * core[i];
+        if (!(r2 > 0) || !std::isfinite(r2)) throw std::runtime_error("dipoles: singular evaluation point");
+        const double inv = 1.0 / (r2 * std::sqrt(r2));
+        for (int k = 0; k < 3; ++k)
+        {
+            const int j = (k + 1) % 3, l = (k + 2) % 3;
+            const double term = (mom[3 * i + j] * r[l] - mom[3 * i + l] * r[j]) * inv;
+            const double y = term - corr[k], t = sum[k] + y;
+            corr[k] = (t - sum[k]) - y;
+            sum[k] = t;
+        }
+    }
+    for (int k = 0; k < 3; ++k)
+    {
+        if (!std::isfinite(sum[k])) throw std::runtime_error("dipoles: non-finite vector potential");
+        out[k] = sum[k];
+    }
+}
+double dipole_edge_integral(int32_t n, const double *pos, const double *mom, const double *core, double cut2, const double *a,
+                            const double *b)
+{
+    double mid[3], dr[3];
+    for (int k = 0; k < 3; ++k) { mid[k] = 0.5 * (a[k] + b[k]); dr[k] = b[k] - a[k]; }
+    if (dr[0] == 0 && dr[1] == 0 && dr[2] == 0) return 0.0;
+    const double q = 0.5 / std::sqrt(3.0);
+    double p[3], mq[3], ap[3], am[3];
+    for (int k = 0; k < 3; ++k) { p[k] = mid[k] + q * dr[k]; mq[k] = mid[k] - q * dr[k]; }
+    dipole_potential(n, pos, mom, core, cut2, p, ap);
+    dipole_potential(n, pos, mom, core, cut2, mq, am);
+    double v = 0;
+    for (int k = 0; k < 3; ++k) v += 0.5 * (ap[k] + am[k]) * dr[k];
+    if (!std::isfinite(v)) throw std::runtime_error("dipoles: non-finite edge integral");
+    return v;
+}
+} // namespace mhdg
 
 namespace
 {
@@ -327,6 +380,57 @@ int mhdg_mesh_background(const mhdg_mesh *m, const double B_imf[3], int dipole, 
     }
     reconstruct_cell_vector(cx, g, PhiAdd, BaddCell);
     return 0;
+}
+
+int mhdg_dipole_edge_integrals(int32_t n, const double *pos, const double *mom, const double *core, double cutoff, int64_t n_edges,
+                               const double *xa, const double *xb, double *out)
+{
+    try
+    {
+        const double cut2 = cutoff * cutoff;
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < n_edges; ++e) out[e] = mhdg::dipole_edge_integral(n, pos, mom, core, cut2, xa + 3 * e, xb + 3 * e);
+        return 0;
+    }
+    catch (const std::exception &ex) { g_err = ex.what(); return 1; }
+}
+
+int mhdg_mesh_background_dipoles(const mhdg_mesh *m, const double B_imf[3], int32_t n, const double *pos, const double *mom,
+                                 const double *core, double cutoff, double *PhiAdd, double *BaddCell)
+{
+    try
+    {
+        using namespace mhdg;
+        const Complex &cx = m->cx;
+        const Geometry &g = m->g;
+        const V3 B(B_imf[0], B_imf[1], B_imf[2]);
+        for (int32_t f = 0; f < cx.nf; ++f) PhiAdd[f] = dot(B, ld3(g.face_area.data(), f));
+        if (n > 0)
+        {
+            // A.dl along every edge from its start node to its end node, then the flux d1 A
+            std::vector<double> A(cx.ne, 0.0);
+            const double cut2 = cutoff * cutoff;
+            std::exception_ptr err;
+#pragma omp parallel for schedule(static)
+            for (int32_t e = 0; e < cx.ne; ++e)
+            {
+                int32_t a, b;
+                cx.edge_nodes(e, a, b);
+                try { A[e] = dipole_edge_integral(n, pos, mom, core, cut2, &m->xyz[3 * size_t(a)], &m->xyz[3 * size_t(b)]); }
+                catch (...)
+                {
+#pragma omp critical
+                    err = std::current_exception();
+                }
+            }
+            if (err) std::rethrow_exception(err);
+            for (int32_t f = 0; f < cx.nf; ++f)
+                for (int64_t k = cx.f2e.ptr[f]; k < cx.f2e.ptr[f + 1]; ++k) PhiAdd[f] += cx.f2e.sgn[k] * A[cx.f2e.idx[k]];
+        }
+        reconstruct_cell_vector(cx, g, PhiAdd, BaddCell);
+        return 0;
+    }
+    catch (const std::exception &ex) { g_err = ex.what(); return 1; }
 }
 
 int mhdg_mesh_cell_vector(const mhdg_mesh *m, const double *flux, double *b)

@@ -135,6 +135,16 @@ class Mesh:
             self._desc = d
         return self._desc
 
+    def background_dipoles(self, B_imf, positions, moments, cores=None, cutoff=0.0):
+        """Background of the IMF plus a crustal field of point dipoles (MarsSolver::build_background_ recipe)."""
+        pos, mom = f64(positions).reshape(-1, 3), f64(moments).reshape(-1, 3)
+        core = f64(np.zeros(len(pos)) if cores is None else cores)
+        PhiAdd, Badd = np.zeros(self.n_faces), np.zeros(3 * self.n_cells)
+        if lib().mhdg_mesh_background_dipoles(self._h, ptr(f64(B_imf)), len(pos), ptr(pos), ptr(mom), ptr(core), cutoff, ptr(PhiAdd),
+                                              ptr(Badd)):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+        return PhiAdd, Badd
+
     def background(self, B_imf=(0, 0, 0), dipole_m=None, dipole_c=(0, 0, 0), soft=1e-3):
         PhiAdd = np.zeros(self.n_faces)
         Badd = np.zeros(3 * self.n_cells)
@@ -206,3 +216,13 @@ def read_fields(path):
     finally:
         lib().mhdg_fields_free(h)
     return hs.value, out
+
+
+def dipole_edge_integrals(positions, moments, cores, xa, xb, cutoff=0.0):
+    """Edge integrals of the crustal-dipole vector potential (MarsDipoles.cpp:63-107) for edges xa -> xb."""
+    pos, mom, core = f64(positions).reshape(-1, 3), f64(moments).reshape(-1, 3), f64(cores)
+    xa, xb = f64(xa).reshape(-1, 3), f64(xb).reshape(-1, 3)
+    out = np.zeros(len(xa))
+    if lib().mhdg_dipole_edge_integrals(len(pos), ptr(pos), ptr(mom), ptr(core), cutoff, len(xa), ptr(xa), ptr(xb), ptr(out)):
+        raise RuntimeError(lib().mhdg_mesh_error().decode())
+    return out

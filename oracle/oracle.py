@@ -222,6 +222,7 @@ class Reference:
                 "ref_write_mesh": [_P, C.c_char_p], "ref_read_mesh": [C.c_char_p, _P, _P], "ref_topology_hash": [_P],
                 "ref_write_fields": [C.c_char_p, C.c_uint64, C.c_int, _P, _P, _P, _P, _P],
                 "ref_read_field": [C.c_char_p, C.c_char_p, _P, _P, _P, _P, _P],
+                "ref_dipole_edge_integrals": [C.c_int, _P, _P, _P, _D, C.c_int64, _P, _P, _P],
             }.items():
                 getattr(L, name).argtypes = args
             L.ref_ind_solve.restype = C.c_int

@@ -30,6 +30,11 @@
 #include "uns/geometry/Geometry.h"
 #include "uns/geometry/Reconstruct.h"
 #include "uns/io/MeshIO.h"
+#include "MarsUns/physics/MarsDipoles.h"
+
+#include <cmath>
+#include <iomanip>
+#include <sstream>
 
 #include "../include/mhdg.h"
 
@@ -346,5 +351,28 @@ int ref_read_field(const char *path, const char *name, double *out, int64_t *cou
         return 3;
     }
     catch (const std::exception &e) { std::fprintf(stderr, "ref_read_field: %s\n", e.what()); return 1; }
+}
+
+// MARS::DipoleModel (MarsUns/physics/MarsDipoles.cpp): the reference's crustal-dipole edge integrals, the model read
+// from its own text format with unit reference scales (L = B = 1, mu0 = 4 pi: no rescaling)
+int ref_dipole_edge_integrals(int n, const double *pos, const double *mom, const double *core, double cutoff, int64_t ne, const double *xa,
+                              const double *xb, double *out)
+{
+    try
+    {
+        std::ostringstream s;
+        s << std::setprecision(17) << "MARS_DIPOLES_V1 " << n << "\n";
+        for (int i = 0; i < n; ++i)
+            s << pos[3 * i] << " " << pos[3 * i + 1] << " " << pos[3 * i + 2] << " " << mom[3 * i] << " " << mom[3 * i + 1] << " "
+              << mom[3 * i + 2] << " " << core[i] << "\n";
+        MARS::DipoleModel model;
+        std::istringstream in(s.str());
+        model.Read(in, 1.0, 1.0, 4.0 * std::acos(-1.0), "harness");
+        model.SetCutoff(cutoff);
+        for (int64_t e = 0; e < ne; ++e)
+            out[e] = model.EdgeIntegral({xa[3 * e], xa[3 * e + 1], xa[3 * e + 2]}, {xb[3 * e], xb[3 * e + 1], xb[3 * e + 2]});
+        return 0;
+    }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_dipole_edge_integrals: %s\n", e.what()); return 1; }
 }
 }

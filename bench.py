@@ -22,8 +22,8 @@ sys.path.insert(0, HERE)
 PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 
 
-def kernel_bytes(mesh, name, precision_bytes=8):
-    """Compulsory HBM bytes of one launch of a kernel: every array it reads or writes counted once."""
+def kernel_bytes(mesh, name, precision_bytes=8, ns=2):
+    """Compulsory HBM bytes of one launch of a kernel: every array it reads or writes counted once (ns ion species)."""
     C, F, E = mesh.n_cells, mesh.n_faces, mesh.n_edges
     nc2f, nf2e, nec = 4 * C, 3 * F, int(mesh.array("ec_ptr")[-1])
     b = precision_bytes
@@ -39,12 +39,12 @@ def kernel_bytes(mesh, name, precision_bytes=8):
                       + 2 * E * b + 8 * E + E + E * b,
         # one fused MGS step: w read + write, two basis vectors
         "mgs": 4 * E * b,
-        "prepare": 2 * 5 * C * 8 * 2 + F * 8 + 3 * C * 8 * 5 + C * 8 + 4 * F * 8 + 4 * nc2f + 4 * (C + 1) + C * 8 + C,
-        "grad_current": 2 * (5 * C * 8 + 15 * C * 8) + 3 * C * 8 * 2 + 6 * C * 8 + nc2f * (4 + 8 + 24) + 3 * F * 8 * 2 + 3 * C * 8
+        "prepare": ns * 5 * C * 8 * 2 + F * 8 + 3 * C * 8 * 5 + C * 8 + 4 * F * 8 + 4 * nc2f + 4 * (C + 1) + C * 8 + C,
+        "grad_current": ns * (5 * C * 8 + 15 * C * 8) + 3 * C * 8 * 2 + 6 * C * 8 + nc2f * (4 + 8 + 24) + 3 * F * 8 * 2 + 3 * C * 8
                         + 8 * F + C * 8 + C + 4 * (C + 1),
-        "face": 2 * (5 * C * 8 + 15 * C * 8) + 10 * F * 8 + 3 * F * 8 + 3 * C * 8 * 3 + F * 8 * 2 + 3 * F * 8 * 2 + F * 8 + 8 * F
+        "face": ns * (5 * C * 8 + 15 * C * 8) + 5 * ns * F * 8 + 3 * F * 8 + 3 * C * 8 * 3 + F * 8 * 2 + 3 * F * 8 * 2 + F * 8 + 8 * F
                 + 3 * C * 8 + F,
-        "cell_update": 10 * F * 8 + 2 * 5 * C * 8 * 3 + 3 * C * 8 * 2 + C * 8 + C + 4 * nc2f + 4 * (C + 1),
+        "cell_update": 5 * ns * F * 8 + ns * 5 * C * 8 * 3 + 3 * C * 8 * 2 + C * 8 + C + 4 * nc2f + 4 * (C + 1),
         "edge_emf": 3 * F * 8 + 3 * C * 8 + F * 8 + 4 * (E + 1) + 4 * nf2e + 8 * nf2e + 4 * (E + 1) + 4 * nec + 3 * F * 8 * 2 + F * 8
                     + 3 * E * 8 * 2 + E * 8 + E * 8 + E,
         "faraday": E * 8 + F * 8 * 2 + 4 * (F + 1) + 4 * nf2e + 8 * F,
@@ -118,12 +118,12 @@ def main():
     print(f"# workload {args.workload} cells={mesh.n_cells} faces={mesh.n_faces} edges={mesh.n_edges} setup={setup_s:.1f}s",
           file=sys.stderr, flush=True)
     dev = Device(mesh, cfg)
-    dev.set_background(*w["background"])
+    ns = max(int(cfg.n_species), 2)
     precision = 1 if args.precision == "mixed" else 0
     dt_max = 1e9
 
     def measure(prec, profile):
-        dev.set_state(*w["state"])
+        workloads.load(dev, w)
         run_steps(dev, args.warmup, prec, dt_max)
         dev.sync()
         if profile:
@@ -158,9 +158,9 @@ def main():
         total_ms = sum(v[1] for v in prof.values())
         for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
             kernels[k] = dict(launches=n, ms_total=round(ms, 4), share=round(ms / total_ms, 4))
-        dom = max(prof, key=lambda k: prof[k][1] if kernel_bytes(mesh, k) else -1)
+        dom = max(prof, key=lambda k: prof[k][1] if kernel_bytes(mesh, k, ns=ns) else -1)
         n, ms = prof[dom]
-        byt = kernel_bytes(mesh, dom)
+        byt = kernel_bytes(mesh, dom, ns=ns)
         avg_s = ms / n / 1e3
         achieved = byt / avg_s
         roof = dict(kernel=dom, bound="hbm", achieved=round(achieved / 1e9, 1), peak=PEAK_HBM / 1e9, unit="GB/s",
@@ -182,7 +182,8 @@ def main():
         "dtype": "f64" if precision == 0 else "f64+f32",
         "data": "synthetic (seeded tetrahedral shell, seeded noisy state)",
         "config": {"workload": args.workload, "cells": mesh.n_cells, "faces": mesh.n_faces, "edges": mesh.n_edges,
-                   "net": list(w["spec"]["net"]), "precision": args.precision, "hall": True, "gmres_restart": 30,
+                   "net": list(w["spec"]["net"]), "precision": args.precision, "hall": bool(w["spec"]["hall"]),
+                   "ion_species": ns, "gmres_restart": 30,
                    "rtol": 1e-8, "gmres_iterations_per_step": float(np.mean(its)), "max_rel_residual": float(max(res)),
                    "setup_s": round(setup_s, 1)},
         "fp64_step": fp64,

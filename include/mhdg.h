@@ -73,6 +73,16 @@ typedef struct mhdg_config {
     /* cfg.implicit_ideal (Config.h:52): the ideal EMF's part linear in the induced flux joins the implicit
      * operator once mhdg_ind_set_ideal_state has supplied u (ImplicitInduction.cpp:150-157). */
     int32_t implicit_ideal;
+    /* Ion species of the fluid: 0 or 2 = the two Mercury ions (M_H, M_Na, fs); 3..10 = ion_mass[s] and the
+     * free-stream primitive states ion_fs[s] (a ten-ion fluid in the manner of mars::MarsSolver). */
+    int32_t n_species;
+    double ion_mass[10];
+    double ion_fs[10][5];
+    /* Exponential-atmosphere ion production, added to the step's sources (no counterpart in the Mercury
+     * step): q_s = src_rate[s] exp(-(|x| - src_r0) / src_scale_height) per unit volume and time, ions
+     * created at rest with temperature src_T (energy q_s src_T / (gamma - 1)). */
+    double src_rate[10];
+    double src_scale_height, src_r0, src_T;
 } mhdg_config;
 
 /* Fills `cfg` with the reference defaults (Config.h:19-96 initialisers, Scales::simple(5/3, 1, 1)). */
@@ -175,6 +185,10 @@ int mhdg_explicit_tangent(mhdg_ctx *ctx, int n_steps, double dt_max, const doubl
 /* --------------------------------------------------------------- device-resident step (fused path) */
 /* State: U_H, U_Na (5C each, AoS), induced flux Phi (F); background PhiAdd (F) and BaddCell (3C). */
 int mhdg_set_state(mhdg_ctx *ctx, const double *U_H, const double *U_Na, const double *Phi);
+/* Conservative state (5C) and primitive state of the last prepare (5C) of ion species s < n_species. */
+int mhdg_set_species_state(mhdg_ctx *ctx, int species, const double *U);
+int mhdg_get_species_state(mhdg_ctx *ctx, int species, double *U);
+int mhdg_get_species_prim(mhdg_ctx *ctx, int species, double *W);
 int mhdg_get_state(mhdg_ctx *ctx, double *U_H, double *U_Na, double *Phi);
 int mhdg_set_background(mhdg_ctx *ctx, const double *PhiAdd, const double *BaddCell);
 /* mercury::Solver::step_first_order (MercuryUns/driver/Solver.cpp:228-277) on the resident state:

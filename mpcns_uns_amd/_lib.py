@@ -44,6 +44,11 @@ class Config(C.Structure):
         ("cuct", C.c_double), ("fs", (C.c_double * 5) * 2), ("u_inf", C.c_double * 3), ("B_imf", C.c_double * 3),
         ("wall_dirichlet", C.c_int32),
         ("implicit_ideal", C.c_int32),
+        ("n_species", C.c_int32),
+        ("ion_mass", C.c_double * 10),
+        ("ion_fs", (C.c_double * 5) * 10),
+        ("src_rate", C.c_double * 10),
+        ("src_scale_height", C.c_double), ("src_r0", C.c_double), ("src_T", C.c_double),
     ]
 
 
@@ -96,7 +101,8 @@ def lib():
             "mhdg_ind_set_frozen": [P, P, P], "mhdg_ind_apply_F": [P, P, P],
             "mhdg_ind_solve": [P, P, P, P, D, P, C.c_int, P, P, P],
             "mhdg_ind_flags": [P, P, P, P], "mhdg_ind_set_ideal_state": [P, P, P], "mhdg_ind_ideal_linear": [P, P, P],
-            "mhdg_step_explicit": [P, D, P], "mhdg_ind_apply_G": [P, P, P], "mhdg_ind_jvp": [P, P, P, P],
+            "mhdg_step_explicit": [P, D, P], "mhdg_set_species_state": [P, C.c_int, P], "mhdg_get_species_state": [P, C.c_int, P],
+            "mhdg_get_species_prim": [P, C.c_int, P], "mhdg_ind_apply_G": [P, P, P], "mhdg_ind_jvp": [P, P, P, P],
             "mhdg_ind_newton": [P, P, P, P, D, P, D, C.c_int, D, P, P, P, P],
             "mhdg_shell_energy": [P, D, D, P],
             "mhdg_explicit_tangent": [P, C.c_int, D, P, P, P, P, P, P, P, D, D, P, P, P, P, P], "mhdg_set_recon_rel_limit": [P, D], "mhdg_step_hall": [P, D, C.c_int, P, P, P],

@@ -22,8 +22,8 @@ struct mhdg_ctx
     std::vector<double> taper_h;
     int8_t *dir_mercury = nullptr;
     // resident state and derived fields
-    double *U[2]{}, *Phi = nullptr, *PhiAdd = nullptr, *Badd = nullptr;
-    double *W[2]{}, *grad[2]{}, *Bind = nullptr, *Btot = nullptr, *uplus = nullptr, *J = nullptr, *alpha = nullptr, *B0 = nullptr;
+    double *U[mhdg::kMaxSpecies]{}, *Phi = nullptr, *PhiAdd = nullptr, *Badd = nullptr;
+    double *W[mhdg::kMaxSpecies]{}, *grad[mhdg::kMaxSpecies]{}, *Bind = nullptr, *Btot = nullptr, *uplus = nullptr, *J = nullptr, *alpha = nullptr, *B0 = nullptr;
     double *FA = nullptr, *Ef = nullptr, *E_expl = nullptr, *E_last = nullptr, *dt_part = nullptr, *dt_dev = nullptr;
     // implicit induction work
     double *phi_w = nullptr, *bcell = nullptr, *gcell = nullptr, *corr = nullptr;

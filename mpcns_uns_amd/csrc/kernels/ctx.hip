@@ -131,8 +131,26 @@ static void make_params(const mhdg_config &c, Params &P)
     P.rho_floor_dt = c.rho_floor_dt; P.cuct = c.cuct;
     P.second_order = c.second_order; P.lorentz = c.lorentz_force; P.hall = c.hall && c.hall_coef > 0;
     P.resistive = c.resistive && c.inver_Rem > 0;
-    for (int s = 0; s < 2; ++s)
-        for (int m = 0; m < 5; ++m) P.fs[s][m] = c.fs[s][m];
+    if (c.n_species > kMaxSpecies || c.n_species < 0 || c.n_species == 1) throw std::runtime_error("n_species must be 0 or 2..10");
+    P.ns = c.n_species > 2 ? c.n_species : 2;
+    if (P.ns == 2)
+    {
+        P.M_s[0] = c.M_H;
+        P.M_s[1] = c.M_Na;
+        for (int s = 0; s < 2; ++s)
+            for (int m = 0; m < 5; ++m) P.fs[s][m] = c.fs[s][m];
+    }
+    else
+        for (int s = 0; s < P.ns; ++s)
+        {
+            if (!(c.ion_mass[s] > 0)) throw std::runtime_error("ion_mass must be positive");
+            P.M_s[s] = c.ion_mass[s];
+            for (int m = 0; m < 5; ++m) P.fs[s][m] = c.ion_fs[s][m];
+        }
+    bool src = false;
+    for (int s = 0; s < P.ns; ++s) { P.src_rate[s] = c.src_rate[s]; src = src || c.src_rate[s] != 0.0; }
+    if (src && !(c.src_scale_height > 0)) throw std::runtime_error("src_scale_height must be positive with ion production");
+    P.src_H = c.src_scale_height; P.src_r0 = c.src_r0; P.src_T = c.src_T;
     // E_inf = -1 * (u_inf x B_imf) (MercuryUns/driver/Solver.cpp:162)
     const V3 u(c.u_inf[0], c.u_inf[1], c.u_inf[2]), B(c.B_imf[0], c.B_imf[1], c.B_imf[2]);
     const V3 E = -1.0 * cross(u, B);
@@ -181,19 +199,36 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
     d.recon = upload(t.recon); d.m_e = upload(t.m_e); d.d2 = upload(t.d2); d.taper = upload(t.taper);
     heavy = upload(t.heavy);
     taper_h = t.taper;
+    d.atm = nullptr;
+    bool any_src = false;
+    for (int s = 0; s < P.ns; ++s) any_src = any_src || P.src_rate[s] != 0.0;
+    if (any_src)
+    {
+        // exp(-(|x_c| - r0) / H), evaluated once on the host
+        std::vector<double> atm(C);
+        for (size_t i = 0; i < C; ++i)
+        {
+            const double *x = m.cell_center + 3 * i;
+            atm[i] = std::exp(-(std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) - P.src_r0) / P.src_H);
+        }
+        d.atm = upload(atm);
+    }
     std::vector<int8_t> dir(E);
     for (size_t e = 0; e < E; ++e) dir[e] = c.wall_dirichlet ? m.edge_mode[e] != 0 : m.edge_mode[e] == 2;
     dir_mercury = upload(dir);
 
     auto D = [&](size_t n) { return static_cast<double *>(alloc(n * sizeof(double))); };
-    U[0] = D(5 * C); U[1] = D(5 * C); Phi = D(F); PhiAdd = D(F); Badd = D(3 * C);
+    Phi = D(F); PhiAdd = D(F); Badd = D(3 * C);
     hip_check(hipMemset(PhiAdd, 0, F * 8), "memset"); hip_check(hipMemset(Badd, 0, 3 * C * 8), "memset");
-    hip_check(hipMemset(U[0], 0, 5 * C * 8), "memset"); hip_check(hipMemset(U[1], 0, 5 * C * 8), "memset");
     hip_check(hipMemset(Phi, 0, F * 8), "memset");
-    W[0] = D(5 * C); W[1] = D(5 * C); grad[0] = D(15 * C); grad[1] = D(15 * C);
+    for (int s = 0; s < P.ns; ++s)
+    {
+        U[s] = D(5 * C); W[s] = D(5 * C); grad[s] = D(15 * C);
+        hip_check(hipMemset(U[s], 0, 5 * C * 8), "memset");
+    }
     Bind = D(3 * C); Btot = D(3 * C); uplus = D(3 * C); J = D(3 * C); alpha = D(C); B0 = D(3 * C);
     hip_check(hipMemset(alpha, 0, C * 8), "memset"); hip_check(hipMemset(B0, 0, 3 * C * 8), "memset");
-    FA = D(10 * F); Ef = D(3 * F); E_expl = D(E); E_last = D(E);
+    FA = D(5 * size_t(P.ns) * F); Ef = D(3 * F); E_expl = D(E); E_last = D(E);
     dt_part = D(size_t(nblk(nc)) + 1); dt_dev = D(1);
     // induction work
     phi_w = D(F); bcell = D(3 * C); gcell = D(3 * C); corr = D(E); have_corr = false;
@@ -338,7 +373,11 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
 double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
 {
     const int nb = nblk(nc);
-    LAUNCH("prepare", k_prepare<double>, dim3(nb), dim3(256), 0, stream, d, P, U[0], U[1], Phi, Badd, W[0], W[1], Bind, Btot, uplus, alpha,
+    SpC Uc{}, Wc{};
+    SpW Ww{};
+    for (int s = 0; s < P.ns; ++s) { Uc.p[s] = U[s]; Wc.p[s] = W[s]; Ww.p[s] = W[s]; }
+    LAUNCH("prepare", k_prepare<double>, dim3(nb), dim3(256), 0, stream, d, P, Uc, static_cast<const double *>(Phi),
+           static_cast<const double *>(Badd), Ww, Bind, Btot, uplus, alpha,
                        dt_part);
     LAUNCH("dt_final", k_dt_final<double>, dim3(1), dim3(256), 0, stream, dt_part, nb, dt_max, dt_dev);
     double *dth = static_cast<double *>(host_pin);
@@ -348,26 +387,28 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     hip_check(hipEventRecord(ev, stream), "event");
     SpC Ws{}, none{}, gradsC{};
     SpW grads{}, Us{}, noneW{};
-    Ws.p[0] = W[0]; Ws.p[1] = W[1];
-    grads.p[0] = grad[0]; grads.p[1] = grad[1];
-    gradsC.p[0] = grad[0]; gradsC.p[1] = grad[1];
-    Us.p[0] = U[0]; Us.p[1] = U[1];
+    for (int s = 0; s < P.ns; ++s)
+    {
+        Ws.p[s] = W[s];
+        grads.p[s] = grad[s];
+        gradsC.p[s] = grad[s];
+        Us.p[s] = U[s];
+    }
     if (P.second_order)
-        LAUNCH("grad_current", k_grad_current<double>, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, Ws, none, grads, 1, Bind, J);
+        LAUNCH("grad_current", k_grad_current<double>, dim3(nb), dim3(256), 0, stream, d, P, P.ns, 0, Ws, none, grads, 1, Bind, J);
     else
     {
-        hip_check(hipMemsetAsync(grad[0], 0, 15 * size_t(nc) * 8, stream), "memset");
-        hip_check(hipMemsetAsync(grad[1], 0, 15 * size_t(nc) * 8, stream), "memset");
+        for (int s = 0; s < P.ns; ++s) hip_check(hipMemsetAsync(grad[s], 0, 15 * size_t(nc) * 8, stream), "memset");
         LAUNCH("grad_current", k_grad_current<double>, dim3(nb), dim3(256), 0, stream, d, P, 0, 0, Ws, none, grads, 1, Bind, J);
     }
-    LAUNCH("face", k_face<double>, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 2, 0, Ws, gradsC, none, FA, uplus, Btot, Bind, Phi, PhiAdd, Ef);
+    LAUNCH("face", k_face<double>, dim3(nblk(nf)), dim3(256), 0, stream, d, P, P.ns, 0, Ws, gradsC, none, FA, uplus, Btot, Bind, Phi, PhiAdd, Ef);
     check_launch("explicit stage");
     hip_check(hipEventSynchronize(ev), "event sync");
     (void)hipEventDestroy(ev);
     const double gdt = dth[0];
     if (!std::isfinite(gdt) || gdt <= 0) throw std::runtime_error("invalid time step");
-    LAUNCH("cell_update", k_cell_update<double>, dim3(nb), dim3(256), 0, stream, d, P, 2, 0, static_cast<const double *>(FA), none, W[0], W[1], Btot,
-                       J, dt_dev, 0.0, 1, 1, Us, noneW);
+    LAUNCH("cell_update", k_cell_update<double>, dim3(nb), dim3(256), 0, stream, d, P, P.ns, 0, static_cast<const double *>(FA), none, Wc,
+           static_cast<const double *>(Btot), static_cast<const double *>(J), static_cast<const double *>(dt_dev), 0.0, 1, 1, Us, noneW);
     launch_edge_emf(Ef, uplus, Phi, P.cuct, nullptr, E_expl, 1);
     *iters = 0;
     *res = 0.0;
@@ -526,9 +567,8 @@ int mhdg_flux_rhs(mhdg_ctx *c, int species, const double fs[5], const double *W,
                            static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
                            static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), static_cast<double *>(nullptr));
         hipLaunchKernelGGL(k_cell_update<double>, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, P, 1, species, static_cast<const double *>(fa),
-                           none, static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
-                           static_cast<const double *>(nullptr), static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
-                           0.0, 0, 0, nw, ro);
+                           none, none, static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
+                           static_cast<const double *>(nullptr), 0.0, 0, 0, nw, ro);
         c->check_launch("flux_rhs");
         c->d2h(rhs, ro.p[0], 5 * size_t(c->nc) * 8);
     })
@@ -595,11 +635,14 @@ int mhdg_div_b(mhdg_ctx *c, const double *B, const double *extra, double *max_ab
 int mhdg_local_dt(mhdg_ctx *c, const double *WH, const double *WNa, const double *Bt, double *dt)
 {
     MHDG_TRY(c, {
+        if (c->P.ns != 2) throw std::runtime_error("local_dt: the two-species (H+, Na+) entry needs n_species 2");
         const int nb = nblk(c->nc);
-        const double *a = c->stage_in(0, WH, 5 * size_t(c->nc)), *b = c->stage_in(1, WNa, 5 * size_t(c->nc));
+        SpC Ws{};
+        Ws.p[0] = c->stage_in(0, WH, 5 * size_t(c->nc));
+        Ws.p[1] = c->stage_in(1, WNa, 5 * size_t(c->nc));
         const double *bt = c->stage_in(2, Bt, 3 * size_t(c->nc));
         double *part = c->stage(3, nb), *o = c->stage(4, 1);
-        hipLaunchKernelGGL(k_local_dt, dim3(nb), dim3(256), 0, c->stream, c->d, c->P, a, b, bt, part);
+        hipLaunchKernelGGL(k_local_dt, dim3(nb), dim3(256), 0, c->stream, c->d, c->P, Ws, bt, part);
         hipLaunchKernelGGL(k_dt_final<double>, dim3(1), dim3(256), 0, c->stream, static_cast<const double *>(part), nb, __builtin_inf(), o);
         c->check_launch("local_dt");
         c->d2h(dt, o, 8);
@@ -609,16 +652,19 @@ int mhdg_add_sources(mhdg_ctx *c, int species, const double *WH, const double *W
 {
     MHDG_TRY(c, {
         if (species < 0 || species > 1) throw std::runtime_error("sources: species 0 or 1");
+        if (c->P.ns != 2) throw std::runtime_error("add_sources: the two-species (H+, Na+) entry needs n_species 2");
         const size_t C = c->nc;
-        const double *a = c->stage_in(0, WH, 5 * C), *b = c->stage_in(1, WNa, 5 * C), *bt = c->stage_in(2, Bt, 3 * C),
-                     *j = c->stage_in(3, Jc, 3 * C);
+        SpC Wall{};
+        Wall.p[0] = c->stage_in(0, WH, 5 * C);
+        Wall.p[1] = c->stage_in(1, WNa, 5 * C);
+        const double *bt = c->stage_in(2, Bt, 3 * C), *j = c->stage_in(3, Jc, 3 * C);
         double *r = c->stage_in(4, rhs, 5 * C);
         SpC ri{};
         SpW ro{}, nw{};
         ri.p[0] = r;
         ro.p[0] = r;
         hipLaunchKernelGGL(k_cell_update<double>, dim3(nblk(c->nc)), dim3(256), 0, c->stream, c->d, c->P, 1, species,
-                           static_cast<const double *>(nullptr), ri, a, b, bt, j, static_cast<const double *>(nullptr), dt, 1, 0, nw, ro);
+                           static_cast<const double *>(nullptr), ri, Wall, bt, j, static_cast<const double *>(nullptr), dt, 1, 0, nw, ro);
         c->check_launch("add_sources");
         c->d2h(rhs, r, 5 * C * 8);
     })
@@ -755,6 +801,7 @@ int mhdg_explicit_tangent(mhdg_ctx *c, int n_steps, double dt_max, const double 
 {
     MHDG_TRY(c, {
         if (n_steps < 0) throw std::runtime_error("explicit_tangent: n_steps >= 0");
+        if (c->P.ns != 2) throw std::runtime_error("explicit_tangent: two-species (H+, Na+) contexts only");
         c->tangent_alloc();
         const size_t C = c->nc, F = c->nf;
         auto seed = [&](const double *val, const double *host_d, size_t n, int slot, Dual *out) {
@@ -802,6 +849,30 @@ int mhdg_get_state(mhdg_ctx *c, double *UH, double *UNa, double *Ph)
         hip_check(hipMemcpy(UH, c->U[0], 5 * size_t(c->nc) * 8, hipMemcpyDeviceToHost), "D2H");
         hip_check(hipMemcpy(UNa, c->U[1], 5 * size_t(c->nc) * 8, hipMemcpyDeviceToHost), "D2H");
         hip_check(hipMemcpy(Ph, c->Phi, size_t(c->nf) * 8, hipMemcpyDeviceToHost), "D2H");
+    })
+}
+int mhdg_set_species_state(mhdg_ctx *c, int species, const double *U)
+{
+    MHDG_TRY(c, {
+        if (species < 0 || species >= c->P.ns) throw std::runtime_error("species out of range for this context's n_species");
+        hip_check(hipMemcpy(c->U[species], U, 5 * size_t(c->nc) * 8, hipMemcpyHostToDevice), "H2D");
+        c->have_corr = false;
+    })
+}
+int mhdg_get_species_state(mhdg_ctx *c, int species, double *U)
+{
+    MHDG_TRY(c, {
+        if (species < 0 || species >= c->P.ns) throw std::runtime_error("species out of range for this context's n_species");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        hip_check(hipMemcpy(U, c->U[species], 5 * size_t(c->nc) * 8, hipMemcpyDeviceToHost), "D2H");
+    })
+}
+int mhdg_get_species_prim(mhdg_ctx *c, int species, double *W)
+{
+    MHDG_TRY(c, {
+        if (species < 0 || species >= c->P.ns) throw std::runtime_error("species out of range for this context's n_species");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        hip_check(hipMemcpy(W, c->W[species], 5 * size_t(c->nc) * 8, hipMemcpyDeviceToHost), "D2H");
     })
 }
 int mhdg_set_background(mhdg_ctx *c, const double *PA, const double *BA)

@@ -45,6 +45,7 @@ struct DevMesh
     const double *m_e;              // eta_e / star1_e
     const double *d2;               // sum of star2 over the faces of e
     const double *taper;            // per cell
+    const double *atm;              // exp(-(|x_c| - r0) / H) per cell for the ion production, null without it
 };
 
 constexpr int kMaxSpecies = 10;
@@ -59,6 +60,12 @@ struct Params
     // tangent seeds of the free-stream states and of the imposed inflow EMF (used by the R = Dual kernels)
     double dfs[kMaxSpecies][5];
     double dE_inf[3];
+    // ion species: count, molar masses (M_s[0] = M_H, M_s[1] = M_Na for the two-ion Mercury fluid)
+    int32_t ns;
+    double M_s[kMaxSpecies];
+    // exponential-atmosphere ion production q_s = rate_s exp(-(|x| - r0) / H), new ions at rest at temperature T
+    double src_rate[kMaxSpecies];
+    double src_H, src_r0, src_T;
 };
 
 // Per-species array pointers passed by value (kernel arguments)
@@ -85,20 +92,15 @@ template <class R> __device__ __forceinline__ void to_cons(const PrimT<R> &W, do
     U[4] = W.p / (gamma - 1.0) + 0.5 * W.rho * (W.u * W.u + W.v * W.v + W.w * W.w);
 }
 
-// Common/physics/Scales.cpp:50-62
-template <class R> struct NumInfoT { R nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa; };
-template <class R> __device__ __forceinline__ NumInfoT<R> numinfo(const Params &P, R rhoH, R rhoNa)
+// n_s = max(rho_s, 0) / M_s * M_ref (Common/physics/Scales.cpp:52-53 for any ion)
+template <class R> __device__ __forceinline__ R ion_density(const Params &P, int s, const R &rho) { return mx(rho, 0.0) / P.M_s[s] * P.M_ref; }
+// n_e = sum of the ion densities in species order (Scales.cpp:50-62: n_e = n_H + n_Na for the two Mercury ions,
+// the same operations for any count of ions); n_e,eff = sqrt(n_e^2 + floor^2) is formed by the callers
+template <class R> __device__ __forceinline__ R electron_density(const Params &P, const R *const *W, long c)
 {
-    NumInfoT<R> n;
-    n.nH = mx(rhoH, 0.0) / P.M_H * P.M_ref;
-    n.nNa = mx(rhoNa, 0.0) / P.M_Na * P.M_ref;
-    n.ne = n.nH + n.nNa;
-    n.ne_eff = sqrt(n.ne * n.ne + P.ne_floor * P.ne_floor);
-    if (n.ne > 1e-300) { n.chiH = n.nH / n.ne; n.chiNa = n.nNa / n.ne; }
-    else { n.chiH = 1.0; n.chiNa = 0.0; }
-    n.wH = n.nH / n.ne_eff;
-    n.wNa = n.nNa / n.ne_eff;
-    return n;
+    R ne = ion_density(P, 0, W[0][5 * c]);
+    for (int s = 1; s < P.ns; ++s) ne = ne + ion_density(P, s, W[s][5 * c]);
+    return ne;
 }
 
 // Global thread index of a one-entity-per-thread launch.  (An XCD-contiguous block order, which keeps the

@@ -45,14 +45,17 @@ template <class R> __device__ V3T<R> cell_vector(const DevMesh &d, int32_t c, co
 }
 
 // MercuryUns/fluid/MercurySources.cpp:104-145 for one fluid cell (+inf if no face restricts it)
-template <class R> __device__ R cell_dt(const DevMesh &d, const Params &P, int32_t c, const R *wH, const R *wNa, const V3T<R> &b)
+// (every ion species; the total density of the fast speed summed in species order)
+template <class R> __device__ R cell_dt(const DevMesh &d, const Params &P, int32_t c, const SpCT<R> &Ws, const V3T<R> &b)
 {
     R dt = __builtin_inf();
     const R B2 = dot(b, b);
-    const R rho_tot = mx(wH[0] + wNa[0], P.rho_floor_dt);
-    for (int s = 0; s < 2; ++s)
+    R rho_sum = Ws.p[0][5 * size_t(c)];
+    for (int s = 1; s < P.ns; ++s) rho_sum = rho_sum + Ws.p[s][5 * size_t(c)];
+    const R rho_tot = mx(rho_sum, P.rho_floor_dt);
+    for (int s = 0; s < P.ns; ++s)
     {
-        const R *w = s == 0 ? wH : wNa;
+        const R *w = Ws.p[s] + 5 * size_t(c);
         const R cs2 = P.gamma * w[4] / mx(w[0], 1e-30);
         R denom = 0;
         for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
@@ -86,20 +89,24 @@ template <class R> __device__ __forceinline__ R block_min(R v, R *sh)
 } // namespace
 
 // Solver::prepare (MercuryUns/driver/Solver.cpp:94-125) + cell_time_steps + n_e,eff / alpha of the
-// frozen Hall state (Solver.cpp:257-260, ImplicitInduction.cpp:105-111).  One thread per cell.
-template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, Params P, const R *U0, const R *U1, const R *Phi,
-                                                 const R *Badd, R *W0, R *W1, R *Bind, R *Btot,
-                                                 R *uplus, R *alpha, R *dt_part)
+// frozen Hall state (Solver.cpp:257-260, ImplicitInduction.cpp:105-111), for P.ns ion species (u+ and n_e summed
+// in species order: for the two Mercury ions the reference's expressions).  One thread per cell.
+template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, Params P, SpCT<R> Us, const R *Phi, const R *Badd, SpWT<R> Ws,
+                                                                    R *Bind, R *Btot, R *uplus, R *alpha, R *dt_part)
 {
     __shared__ R sh[256];
     const int32_t c = gid();
     R dtc = __builtin_inf();
     if (c < d.nc)
     {
-        const PrimT<R> a = to_prim(U0 + 5 * size_t(c), P.gamma, P.rho_floor, P.p_floor);
-        const PrimT<R> b = to_prim(U1 + 5 * size_t(c), P.gamma, P.rho_floor, P.p_floor);
-        const R wa[5] = {a.rho, a.u, a.v, a.w, a.p}, wb[5] = {b.rho, b.u, b.v, b.w, b.p};
-        for (int m = 0; m < 5; ++m) { W0[5 * size_t(c) + m] = wa[m]; W1[5 * size_t(c) + m] = wb[m]; }
+        for (int s = 0; s < P.ns; ++s)
+        {
+            const PrimT<R> a = to_prim(Us.p[s] + 5 * size_t(c), P.gamma, P.rho_floor, P.p_floor);
+            R *w = Ws.p[s] + 5 * size_t(c);
+            w[0] = a.rho; w[1] = a.u; w[2] = a.v; w[3] = a.w; w[4] = a.p;
+        }
+        SpCT<R> Wc;
+        for (int s = 0; s < kMaxSpecies; ++s) Wc.p[s] = Ws.p[s];
         const V3T<R> bi = cell_vector(d, c, Phi);
         const V3T<R> ba = L3(Badd, c);
         const V3T<R> bt(bi.x + ba.x, bi.y + ba.y, bi.z + ba.z);
@@ -109,10 +116,17 @@ template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, P
         R al = 0.0;
         if (d.mat[c] == 0)
         {
-            const NumInfoT<R> ni = numinfo(P, wa[0], wb[0]);
-            up = V3T<R>(ni.chiH * wa[1] + ni.chiNa * wb[1], ni.chiH * wa[2] + ni.chiNa * wb[2], ni.chiH * wa[3] + ni.chiNa * wb[3]);
-            al = (P.hall && ni.ne_eff > 0) ? d.taper[c] * P.hall_coef / ni.ne_eff : 0.0;
-            dtc = cell_dt(d, P, c, wa, wb, bt);
+            const R ne = electron_density(P, Wc.p, c);
+            const R ne_eff = sqrt(ne * ne + P.ne_floor * P.ne_floor);
+            for (int s = 0; s < P.ns; ++s)
+            {
+                const R *w = Wc.p[s] + 5 * size_t(c);
+                const R chi = ne > 1e-300 ? ion_density(P, s, w[0]) / ne : (s == 0 ? R(1.0) : R(0.0));
+                if (s == 0) up = V3T<R>(chi * w[1], chi * w[2], chi * w[3]);
+                else up = V3T<R>(up.x + chi * w[1], up.y + chi * w[2], up.z + chi * w[3]);
+            }
+            al = (P.hall && ne_eff > 0) ? d.taper[c] * P.hall_coef / ne_eff : 0.0;
+            dtc = cell_dt(d, P, c, Wc, bt);
         }
         st3(uplus, c, up);
         alpha[c] = al;
@@ -402,7 +416,7 @@ template <class R> __global__ void __launch_bounds__(256) k_face(DevMesh d, Para
 // update with floors (Solver.cpp:246-252, 171-191).  rhs_out (nullable) receives the rhs.
 // FA nullable (no flux term), rhs_in nullable (start from zero); species of slot s is s0 + s.
 template <class R> __global__ void __launch_bounds__(256) k_cell_update(DevMesh d, Params P, int32_t ns, int32_t s0, const R *FA, SpCT<R> rhs_in,
-                                                     const R *W0, const R *W1, const R *Btot, const R *J,
+                                                     SpCT<R> Wall, const R *Btot, const R *J,
                                                      const R *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<R> Us,
                                                      SpWT<R> rhs_out)
 {
@@ -429,15 +443,23 @@ template <class R> __global__ void __launch_bounds__(256) k_cell_update(DevMesh 
             }
             if (sources)
             {
-                const R *W = sp == 0 ? W0 : W1;
-                const NumInfoT<R> ni = numinfo(P, W0[5 * size_t(c)], W1[5 * size_t(c)]);
-                const R ws = sp == 0 ? ni.wH : ni.wNa;
+                // mercury::add_sources for ion sp: w_s = n_s / n_e,eff over all P.ns ions (MercurySources.cpp:20-63)
+                const R *W = Wall.p[sp];
+                const R ne = electron_density(P, Wall.p, c);
+                const R ws = ion_density(P, sp, W[5 * size_t(c)]) / sqrt(ne * ne + P.ne_floor * P.ne_floor);
                 const R rho = mx(W[5 * size_t(c)], 0.0);
                 const V3T<R> u(W[5 * size_t(c) + 1], W[5 * size_t(c) + 2], W[5 * size_t(c) + 3]);
                 V3T<R> S;
                 if (P.lorentz) S += (P.inver_MA2 * ws) * cross(L3(J, c), L3(Btot, c));
                 rhs[1] += S.x; rhs[2] += S.y; rhs[3] += S.z;
                 rhs[4] += dot(u, S) + (rho > 1e-30 ? 0.5 * dt * dot(S, S) / rho : 0.0);
+                if (P.src_rate[sp] != 0.0 && d.atm)
+                {
+                    // exponential atmosphere (profile tabulated at set-up): ions produced at rest with temperature src_T
+                    const double q = P.src_rate[sp] * d.atm[c];
+                    rhs[0] += q;
+                    rhs[4] += q * P.src_T / (P.gamma - 1.0);
+                }
             }
         }
         if (rhs_out.p[s])
@@ -768,13 +790,12 @@ __global__ void __launch_bounds__(256) k_cell_vector(DevMesh d, const double *fl
 }
 
 // cell_time_steps + block minima from given primitives (local_time_step, MercurySources.cpp:104-116)
-__global__ void __launch_bounds__(256) k_local_dt(DevMesh d, Params P, const double *W0, const double *W1, const double *Btot,
-                                                  double *dt_part)
+__global__ void __launch_bounds__(256) k_local_dt(DevMesh d, Params P, SpC Ws, const double *Btot, double *dt_part)
 {
     __shared__ double sh[256];
     const int32_t c = gid();
     double dtc = __builtin_inf();
-    if (c < d.nc && d.mat[c] == 0) dtc = cell_dt(d, P, c, W0 + 5 * size_t(c), W1 + 5 * size_t(c), L3(Btot, c));
+    if (c < d.nc && d.mat[c] == 0) dtc = cell_dt(d, P, c, Ws, L3(Btot, c));
     const double m = block_min(dtc, sh);
     if (threadIdx.x == 0) dt_part[blockIdx.x] = m;
 }
@@ -810,19 +831,19 @@ __global__ void __launch_bounds__(256) k_shell_energy(DevMesh d, const R *phi, c
 // the step (double) and its tangent-linear (Dual)
 template __global__ void k_shell_energy<double>(DevMesh, const double *, const double *, double, double, double *);
 template __global__ void k_shell_energy<Dual>(DevMesh, const Dual *, const Dual *, double, double, Dual *);
-template __global__ void k_prepare<double>(DevMesh d, Params P, const double *U0, const double *U1, const double *Phi, const double *Badd, double *W0, double *W1, double *Bind, double *Btot, double *uplus, double *alpha, double *dt_part);
+template __global__ void k_prepare<double>(DevMesh d, Params P, SpCT<double> Us, const double *Phi, const double *Badd, SpWT<double> Ws, double *Bind, double *Btot, double *uplus, double *alpha, double *dt_part);
 template __global__ void k_dt_final<double>(const double *part, int32_t n, double dt_max, double *out);
 template __global__ void k_grad_current<double>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> ghosts, SpWT<double> grads, int32_t limit, const double *Bind, double *J);
 template __global__ void k_face<double>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> grads, SpCT<double> ghosts, double *FA, const double *uplus, const double *Btot, const double *Bind, const double *Phi, const double *PhiAdd, double *Ef);
-template __global__ void k_cell_update<double>(DevMesh d, Params P, int32_t ns, int32_t s0, const double *FA, SpCT<double> rhs_in, const double *W0, const double *W1, const double *Btot, const double *J, const double *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<double> Us, SpWT<double> rhs_out);
+template __global__ void k_cell_update<double>(DevMesh d, Params P, int32_t ns, int32_t s0, const double *FA, SpCT<double> rhs_in, SpCT<double> Wall, const double *Btot, const double *J, const double *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<double> Us, SpWT<double> rhs_out);
 template __global__ void k_edge_emf<double>(DevMesh d, Params P, const double *Ef, const double *u_cell, const double *phi_ind, double cuct, int32_t vmax, const double *E_in, double *E_out, int32_t apply_bc);
 template __global__ void k_edge_emf_heavy<double>(DevMesh d, Params P, const double *Ef, const double *u_cell, const double *phi_ind, double cuct, const int32_t *heavy, const double *E_in, double *E_out, int32_t apply_bc);
 template __global__ void k_faraday<double>(DevMesh d, const double *E, const double *dte, const double *dt_dev, double dt_val, double *B);
-template __global__ void k_prepare<Dual>(DevMesh d, Params P, const Dual *U0, const Dual *U1, const Dual *Phi, const Dual *Badd, Dual *W0, Dual *W1, Dual *Bind, Dual *Btot, Dual *uplus, Dual *alpha, Dual *dt_part);
+template __global__ void k_prepare<Dual>(DevMesh d, Params P, SpCT<Dual> Us, const Dual *Phi, const Dual *Badd, SpWT<Dual> Ws, Dual *Bind, Dual *Btot, Dual *uplus, Dual *alpha, Dual *dt_part);
 template __global__ void k_dt_final<Dual>(const Dual *part, int32_t n, double dt_max, Dual *out);
 template __global__ void k_grad_current<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<Dual> Ws, SpCT<Dual> ghosts, SpWT<Dual> grads, int32_t limit, const Dual *Bind, Dual *J);
 template __global__ void k_face<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<Dual> Ws, SpCT<Dual> grads, SpCT<Dual> ghosts, Dual *FA, const Dual *uplus, const Dual *Btot, const Dual *Bind, const Dual *Phi, const Dual *PhiAdd, Dual *Ef);
-template __global__ void k_cell_update<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, const Dual *FA, SpCT<Dual> rhs_in, const Dual *W0, const Dual *W1, const Dual *Btot, const Dual *J, const Dual *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<Dual> Us, SpWT<Dual> rhs_out);
+template __global__ void k_cell_update<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, const Dual *FA, SpCT<Dual> rhs_in, SpCT<Dual> Wall, const Dual *Btot, const Dual *J, const Dual *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<Dual> Us, SpWT<Dual> rhs_out);
 template __global__ void k_edge_emf<Dual>(DevMesh d, Params P, const Dual *Ef, const Dual *u_cell, const Dual *phi_ind, double cuct, int32_t vmax, const Dual *E_in, Dual *E_out, int32_t apply_bc);
 template __global__ void k_edge_emf_heavy<Dual>(DevMesh d, Params P, const Dual *Ef, const Dual *u_cell, const Dual *phi_ind, double cuct, const int32_t *heavy, const Dual *E_in, Dual *E_out, int32_t apply_bc);
 template __global__ void k_faraday<Dual>(DevMesh d, const Dual *E, const double *dte, const Dual *dt_dev, double dt_val, Dual *B);

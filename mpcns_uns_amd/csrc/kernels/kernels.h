@@ -10,18 +10,18 @@ constexpr int32_t kLightMax = 32;   // edges up to this valence use one thread e
 template <class R> constexpr int32_t kHeavyShared() { return sizeof(R) == sizeof(double) ? kHeavyMax : kHeavyMax / 2; }
 
 // explicit.hip: templates on the real type R (double: the step; Dual: its tangent-linear)
-template <class R> __global__ void k_prepare(DevMesh d, Params P, const R *U0, const R *U1, const R *Phi, const R *Badd, R *W0,
-                          R *W1, R *Bind, R *Btot, R *uplus, R *alpha, R *dt_part);
+template <class R> __global__ void k_prepare(DevMesh d, Params P, SpCT<R> Us, const R *Phi, const R *Badd, SpWT<R> Ws, R *Bind, R *Btot,
+                                             R *uplus, R *alpha, R *dt_part);
 template <class R> __global__ void k_dt_final(const R *part, int32_t n, double dt_max, R *out);
 template <class R> __global__ void k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads, int32_t limit,
                                const R *Bind, R *J);
 template <class R> __global__ void k_face(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> grads, SpCT<R> ghosts, R *FA, const R *uplus,
                        const R *Btot, const R *Bind, const R *Phi, const R *PhiAdd, R *Ef);
-template <class R> __global__ void k_cell_update(DevMesh d, Params P, int32_t ns, int32_t s0, const R *FA, SpCT<R> rhs_in, const R *W0,
-                              const R *W1, const R *Btot, const R *J, const R *dt_dev, double dt_val,
-                              int32_t sources, int32_t update, SpWT<R> Us, SpWT<R> rhs_out);
+template <class R> __global__ void k_cell_update(DevMesh d, Params P, int32_t ns, int32_t s0, const R *FA, SpCT<R> rhs_in, SpCT<R> Wall,
+                                                 const R *Btot, const R *J, const R *dt_dev, double dt_val, int32_t sources, int32_t update,
+                                                 SpWT<R> Us, SpWT<R> rhs_out);
 __global__ void k_cell_vector(DevMesh d, const double *flux, double *b);
-__global__ void k_local_dt(DevMesh d, Params P, const double *W0, const double *W1, const double *Btot, double *dt_part);
+__global__ void k_local_dt(DevMesh d, Params P, SpC Ws, const double *Btot, double *dt_part);
 template <class R> __global__ void k_edge_emf(DevMesh d, Params P, const R *Ef, const R *u_cell, const R *phi_ind, double cuct, int32_t vmax,
                            const R *E_in, R *E_out, int32_t apply_bc);
 template <class R> __global__ void k_edge_emf_heavy(DevMesh d, Params P, const R *Ef, const R *u_cell, const R *phi_ind, double cuct,

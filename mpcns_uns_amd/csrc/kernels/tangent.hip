@@ -230,15 +230,18 @@ void mhdg_ctx::explicit_tangent(int n_steps, double dt_max, const Params &Pt)
 {
     const int nb = nbk(nc);
     SpCT<Dual> Ws{}, none{}, gradsC{};
-    SpWT<Dual> grads{}, Us{}, noneW{};
+    SpWT<Dual> grads{}, Us{}, noneW{}, Ww{};
+    SpCT<Dual> Uc{};
     Ws.p[0] = tw.W[0]; Ws.p[1] = tw.W[1];
+    Ww.p[0] = tw.W[0]; Ww.p[1] = tw.W[1];
+    Uc.p[0] = tw.U[0]; Uc.p[1] = tw.U[1];
     grads.p[0] = tw.grad[0]; grads.p[1] = tw.grad[1];
     gradsC.p[0] = tw.grad[0]; gradsC.p[1] = tw.grad[1];
     Us.p[0] = tw.U[0]; Us.p[1] = tw.U[1];
     for (int s = 0; s < n_steps; ++s)
     {
-        LAUNCH("prepare_t", k_prepare<Dual>, dim3(nb), dim3(256), 0, stream, d, Pt, tw.U[0], tw.U[1], tw.Phi, tw.Badd, tw.W[0], tw.W[1],
-               tw.Bind, tw.Btot, tw.uplus, tw.alpha, tw.dt_part);
+        LAUNCH("prepare_t", k_prepare<Dual>, dim3(nb), dim3(256), 0, stream, d, Pt, Uc, static_cast<const Dual *>(tw.Phi),
+               static_cast<const Dual *>(tw.Badd), Ww, tw.Bind, tw.Btot, tw.uplus, tw.alpha, tw.dt_part);
         LAUNCH("dt_final_t", k_dt_final<Dual>, dim3(1), dim3(256), 0, stream, tw.dt_part, nb, dt_max, tw.dt_dev);
         if (!Pt.second_order)
         {
@@ -251,7 +254,7 @@ void mhdg_ctx::explicit_tangent(int n_steps, double dt_max, const Params &Pt)
                static_cast<const Dual *>(tw.uplus), static_cast<const Dual *>(tw.Btot), static_cast<const Dual *>(tw.Bind),
                static_cast<const Dual *>(tw.Phi), static_cast<const Dual *>(tw.PhiAdd), tw.Ef);
         LAUNCH("cell_update_t", k_cell_update<Dual>, dim3(nb), dim3(256), 0, stream, d, Pt, 2, 0, static_cast<const Dual *>(tw.FA), none,
-               static_cast<const Dual *>(tw.W[0]), static_cast<const Dual *>(tw.W[1]), static_cast<const Dual *>(tw.Btot),
+               Ws, static_cast<const Dual *>(tw.Btot),
                static_cast<const Dual *>(tw.J), static_cast<const Dual *>(tw.dt_dev), 0.0, 1, 1, Us, noneW);
         LAUNCH("edge_emf_t", k_edge_emf<Dual>, dim3(nbk(ne)), dim3(256), 0, stream, d, Pt, static_cast<const Dual *>(tw.Ef),
                static_cast<const Dual *>(tw.uplus), static_cast<const Dual *>(tw.Phi), Pt.cuct, kLightMax, static_cast<const Dual *>(nullptr),

@@ -182,6 +182,21 @@ class Device:
         self._ok(lib().mhdg_get_state(self._h, ptr(UH), ptr(UNa), ptr(Phi)))
         return UH, UNa, Phi
 
+    def set_species_state(self, species, U):
+        """Conservative state (5 per cell) of ion species `species` < n_species (ten-ion fluids)."""
+        self._ok(lib().mhdg_set_species_state(self._h, int(species), ptr(f64(U))))
+
+    def get_species_state(self, species):
+        U = np.zeros(5 * self.nc)
+        self._ok(lib().mhdg_get_species_state(self._h, int(species), ptr(U)))
+        return U
+
+    def get_species_prim(self, species):
+        """Primitive state of `species` written by the last step's prepare."""
+        W = np.zeros(5 * self.nc)
+        self._ok(lib().mhdg_get_species_prim(self._h, int(species), ptr(W)))
+        return W
+
     def set_background(self, PhiAdd, BaddCell):
         self._ok(lib().mhdg_set_background(self._h, ptr(f64(PhiAdd)), ptr(f64(BaddCell))))
 

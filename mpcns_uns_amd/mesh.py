@@ -163,20 +163,42 @@ class Mesh:
         lib().mhdg_mesh_seeded_state(self._h, C.byref(cfg), noise, seed, ptr(f64(Badd)), ptr(UH), ptr(UNa), ptr(Phi))
         return UH, UNa, Phi
 
+    def seeded_species_state(self, cfg, fs_list, noise, seed, Badd):
+        """Conservative states of len(fs_list) species (free-stream primitives with seeded relative noise, as
+        seeded_state, two species per draw with seeds seed, seed+1, ...) and the seeded induced flux of the first draw."""
+        states, Phi = [], None
+        for k in range(0, len(fs_list), 2):
+            pair = list(fs_list[k:k + 2])
+            if len(pair) == 1:
+                pair.append(pair[0])
+            c = Config.from_buffer_copy(cfg)
+            for s in range(2):
+                for m in range(5):
+                    c.fs[s][m] = float(pair[s][m])
+            a, b, phi = self.seeded_state(c, noise, seed + k // 2, Badd)
+            states += [a, b]
+            if Phi is None:
+                Phi = phi
+        return states[:len(fs_list)], Phi
+
 
 def default_config(**kw):
-    """mhdg_config with the reference defaults, then keyword overrides (fs as 2x5, u_inf, B_imf as 3)."""
+    """mhdg_config with the reference defaults, then keyword overrides.
+
+    Array fields take sequences: fs (2x5), ion_fs (up to 10x5), ion_mass and src_rate (up to 10), u_inf and
+    B_imf (3); missing trailing entries stay zero."""
     c = Config()
     lib().mhdg_config_defaults(C.byref(c))
     for k, v in kw.items():
-        if k == "fs":
-            for s in range(2):
-                for m in range(5):
-                    c.fs[s][m] = float(v[s][m])
-        elif k in ("u_inf", "B_imf"):
+        if k in ("fs", "ion_fs"):
             arr = getattr(c, k)
-            for i in range(3):
-                arr[i] = float(v[i])
+            for s, row in enumerate(v):
+                for m in range(5):
+                    arr[s][m] = float(row[m])
+        elif k in ("u_inf", "B_imf", "ion_mass", "src_rate"):
+            arr = getattr(c, k)
+            for i, x in enumerate(v):
+                arr[i] = float(x)
         else:
             setattr(c, k, v)
     return c

@@ -3,6 +3,8 @@
 shell_60k      FP64 ideal two-ion MHD, Hall off, one explicit stage; 24x48 net, 9 layers (59,616 tets)
 shell_2m_hall  Hall + uniform resistivity, GMRES(30) to 1e-8, all FP64; 128x256 net, 10 layers (1.95M)
 shell_8m_mixed Hall + resistivity, mixed-precision GMRES + multigrid; 256x512 net, 10 layers (7.83M)
+shell_4m_mars  ten-ion Mars-like fluid, crustal dipoles, exponential-atmosphere ion production, Hall on;
+               shell r in [1, 3], 192x384 net, 9 layers (3.98M)
 
 Hall runs hold the EMF of the conducting wall at zero in the implicit solve as well (Dirichlet wall edges,
 the MarsSolver convention): with free wall edges (the Mercury convention, whose wall is a solid-planet
@@ -13,14 +15,25 @@ r = 1, far field at r = 4 (inflow on the day side); gamma = 5/3, inver_MA2 = 1, 
 second order with relative limiter 0.25, cuct 0.5.  Hall runs: hall_coef 0.5 (taper off), inver_Rem 1e-3
 everywhere.  Initial state: 1% seeded Gaussian noise, induced flux d1 of a seeded edge potential.
 Seeds: 20261202 + workload index.
+
+shell_4m_mars: ion masses 1, 4, 16, 32, 44, 2, 14, 28, 30, 12 (M_ref 1); upstream v = (-1, 0, 0), density 1 for
+the first ion and 1e-4 for the others, p = 0.1 rho; no planetary dipole: the IMF plus 100 crustal point dipoles
+at r = 0.98 (seeded uniform positions and orientations on the sphere, moments lognormal(mu=-3, sigma=1) x
+0.02^3, i.e. surface fields of order exp(-3) above each source); ion production q_s exp(-(r - 1) / 0.02) with
+seeded rates q_s = 10^U(-3, -1) for the nine heavy ions (none for the first), created at temperature 1e-3;
+Hall as above, 5 steps.
 """
+import numpy as np
+
 from .mesh import Mesh, default_config
 
 WORKLOADS = {
     "shell_60k": dict(index=1, net=(24, 48, 9), hall=False),
     "shell_2m_hall": dict(index=2, net=(128, 256, 10), hall=True),
     "shell_8m_mixed": dict(index=3, net=(256, 512, 10), hall=True),
+    "shell_4m_mars": dict(index=5, net=(192, 384, 9), hall=True, mars=True),
 }
+MARS_ION_MASS = (1.0, 4.0, 16.0, 32.0, 44.0, 2.0, 14.0, 28.0, 30.0, 12.0)
 
 DIPOLE_M = (0.0, 0.0, -0.2)
 B_IMF = (0.0, 0.0, -0.05)
@@ -49,9 +62,54 @@ def build(name=None, net=None, hall=None, seed=None, noise=0.01, **cfg_kw):
     if hall is not None:
         spec["hall"] = hall
     seed = SEED0 + spec["index"] if seed is None else seed
+    if spec.get("mars"):
+        return _build_mars(spec, seed, noise, **cfg_kw)
     n_lat, n_lon, n_r = spec["net"]
     mesh = Mesh.shell(n_lat, n_lon, n_r, seed=seed).boundary({1: "wall", 2: "farfield"}, u_inf=U_INF)
     cfg = physics_config(spec["hall"], **cfg_kw)
     PhiAdd, Badd = mesh.background(B_IMF, DIPOLE_M, (0.0, 0.0, 0.0), 1e-3)
     UH, UNa, Phi = mesh.seeded_state(cfg, noise, seed, Badd)
     return dict(mesh=mesh, cfg=cfg, state=(UH, UNa, Phi), background=(PhiAdd, Badd), seed=seed, spec=spec)
+
+
+def mars_config(hall=True, n_dipoles=100, seed=SEED0 + 5, **kw):
+    """Ten-ion configuration of shell_4m_mars (seeded ion production rates)."""
+    rng = np.random.default_rng(seed)
+    rates = [0.0] + list(10.0 ** rng.uniform(-3.0, -1.0, size=9))
+    fs = [(1.0 if s == 0 else 1e-4, U_INF[0], U_INF[1], U_INF[2], 0.1 * (1.0 if s == 0 else 1e-4)) for s in range(10)]
+    base = dict(n_species=10, ion_mass=MARS_ION_MASS, ion_fs=fs, fs=fs[:2], M_H=MARS_ION_MASS[0], M_Na=MARS_ION_MASS[1],
+                src_rate=rates, src_scale_height=0.02, src_r0=1.0, src_T=1e-3)
+    base.update(kw)
+    return physics_config(hall, **base)
+
+
+def crustal_dipoles(n=100, seed=SEED0 + 5, r=0.98, depth=0.02):
+    """Seeded point dipoles at radius r: positions and orientations uniform on the sphere, moments
+    lognormal(-3, 1) * depth^3."""
+    rng = np.random.default_rng(seed + 1000)
+    def unit(k):
+        v = rng.normal(size=(k, 3))
+        return v / np.linalg.norm(v, axis=1)[:, None]
+    pos = r * unit(n)
+    mom = unit(n) * (rng.lognormal(-3.0, 1.0, size=n) * depth ** 3)[:, None]
+    return pos, mom
+
+
+def _build_mars(spec, seed, noise, **cfg_kw):
+    n_lat, n_lon, n_r = spec["net"]
+    mesh = Mesh.shell(n_lat, n_lon, n_r, r_in=1.0, r_out=3.0, seed=seed).boundary({1: "wall", 2: "farfield"}, u_inf=U_INF)
+    cfg = mars_config(spec["hall"], seed=seed, **cfg_kw)
+    pos, mom = crustal_dipoles(seed=seed)
+    PhiAdd, Badd = mesh.background_dipoles(B_IMF, pos, mom)
+    fs = [tuple(cfg.ion_fs[s][m] for m in range(5)) for s in range(cfg.n_species)]
+    states, Phi = mesh.seeded_species_state(cfg, fs, noise, seed, Badd)
+    return dict(mesh=mesh, cfg=cfg, state=(states[0], states[1], Phi), species_state=states, background=(PhiAdd, Badd),
+                dipoles=(pos, mom), seed=seed, spec=spec)
+
+
+def load(dev, w):
+    """Background and initial state of workload `w` onto Device `dev` (every species of a ten-ion fluid)."""
+    dev.set_background(*w["background"])
+    dev.set_state(*w["state"])
+    for s, U in enumerate(w.get("species_state", [])[2:], start=2):
+        dev.set_species_state(s, U)

@@ -77,22 +77,6 @@ void rusanov(const Prim &L, const Prim &R, const double n[3], double gamma, doub
     for (int m = 0; m < 5; ++m) F[m] = 0.5 * (FL[m] + FR[m]) - 0.5 * s * (UR[m] - UL[m]);
 }
 
-// Common/physics/Scales.cpp:50-62
-struct NumInfo { double nH = 0, nNa = 0, ne = 0, ne_eff = 0, chiH = 1, chiNa = 0, wH = 0, wNa = 0; };
-NumInfo numinfo(const mhdg_config &c, double rhoH, double rhoNa)
-{
-    NumInfo n;
-    n.nH = std::max(rhoH, 0.0) / c.M_H * c.M_ref;
-    n.nNa = std::max(rhoNa, 0.0) / c.M_Na * c.M_ref;
-    n.ne = n.nH + n.nNa;
-    n.ne_eff = std::sqrt(n.ne * n.ne + c.ne_floor * c.ne_floor);
-    if (n.ne > 1e-300) { n.chiH = n.nH / n.ne; n.chiNa = n.nNa / n.ne; }
-    else { n.chiH = 1.0; n.chiNa = 0.0; }
-    n.wH = n.nH / n.ne_eff;
-    n.wNa = n.nNa / n.ne_eff;
-    return n;
-}
-
 enum Kind : int8_t { Interior = 0, Wall = 1, Inflow = 2, Outflow = 3, Skip = 4 };
 
 // Common/fluid/Reconstruction.cpp:9-24
@@ -156,8 +140,12 @@ struct orc_ctx
     std::vector<int64_t> ce_ptr;
     std::vector<int32_t> ce_edge;
     std::vector<V3> ce_wt;
-    // step state (mercury::Solver members, Solver.h:128-131)
-    std::vector<double> U[2], Phi, PhiAdd, BaddCell, W[2], grad[2], rhs[2], Bind, Btot, uplus, J, PhiTot, E;
+    // ion species of the step: ns = 2 (H+, Na+: M_H, M_Na, fs) or n_species (ion_mass, ion_fs)
+    int ns = 2;
+    double M_s[10] = {}, fs_s[10][5] = {};
+    std::vector<double> atm; // exp(-(|x_c| - r0) / H) per cell, empty without ion production
+    // step state (mercury::Solver members, Solver.h:128-131), one row per species
+    std::vector<double> U[10], Phi, PhiAdd, BaddCell, W[10], grad[10], rhs[10], Bind, Btot, uplus, J, PhiTot, E;
     std::string err;
 };
 
@@ -489,17 +477,30 @@ void flux_rhs(const orc_ctx &o, int species, const Prim &fs, const double *W, co
     }
 }
 
-// MercuryUns/fluid/MercurySources.cpp:20-63 (Lorentz part; chemistry and ambipolar are off)
-void add_sources(const orc_ctx &o, int s, const double *WH, const double *WNa, const double *B, const double *J, double dt, double *rhs)
+// n_s = max(rho_s, 0) / M_s * M_ref and n_e = sum of n_s in species order (Common/physics/Scales.cpp:50-62 for
+// the two Mercury ions, the same expressions for any number of ions)
+double ion_n(const orc_ctx &o, int s, double rho) { return std::max(rho, 0.0) / o.M_s[s] * o.cfg.M_ref; }
+double electron_n(const orc_ctx &o, const double *const *Wall, size_t i)
 {
-    const double *W = s == 0 ? WH : WNa;
+    double ne = ion_n(o, 0, Wall[0][5 * i]);
+    for (int s = 1; s < o.ns; ++s) ne = ne + ion_n(o, s, Wall[s][5 * i]);
+    return ne;
+}
+
+// MercuryUns/fluid/MercurySources.cpp:20-63 (Lorentz part; chemistry and ambipolar are off) for ion s of the
+// o.ns ions Wall, w_s = n_s / n_e,eff; with `production`, the exponential-atmosphere ion source of the step
+void add_sources(const orc_ctx &o, int s, const double *const *Wall, const double *B, const double *J, double dt, double *rhs,
+                 bool production)
+{
+    const double *W = Wall[s];
+    const double src = production ? o.cfg.src_rate[s] : 0.0;
 #pragma omp parallel for schedule(static)
     for (int32_t c = 0; c < o.nc; ++c)
     {
         if (o.mat[c] == 1) continue;
         const size_t i = c;
-        const NumInfo ni = numinfo(o.cfg, WH[5 * i], WNa[5 * i]);
-        const double ws = s == 0 ? ni.wH : ni.wNa;
+        const double ne = electron_n(o, Wall, i);
+        const double ws = ion_n(o, s, W[5 * i]) / std::sqrt(ne * ne + o.cfg.ne_floor * o.cfg.ne_floor);
         const double rho = std::max(W[5 * i], 0.0);
         const V3 u(W[5 * i + 1], W[5 * i + 2], W[5 * i + 3]);
         double *r = rhs + 5 * i;
@@ -511,11 +512,18 @@ void add_sources(const orc_ctx &o, int s, const double *WH, const double *WNa, c
         }
         r[1] += S.x; r[2] += S.y; r[3] += S.z;
         r[4] += dot(u, S) + (rho > 1e-30 ? 0.5 * dt * dot(S, S) / rho : 0.0);
+        if (src != 0.0 && !o.atm.empty())
+        {
+            // ions produced at rest with temperature src_T
+            const double q = src * o.atm[i];
+            r[0] += q;
+            r[4] += q * o.cfg.src_T / (o.cfg.gamma - 1.0);
+        }
     }
 }
 
 // MercuryUns/fluid/MercurySources.cpp:65-116
-double local_dt(const orc_ctx &o, const double *WH, const double *WNa, const double *B)
+double local_dt(const orc_ctx &o, const double *const *Wall, const double *B)
 {
     const double inf = std::numeric_limits<double>::infinity();
     std::vector<double> d(o.nc, 0.0);
@@ -527,10 +535,12 @@ double local_dt(const orc_ctx &o, const double *WH, const double *WNa, const dou
         double dt = inf;
         const V3 b(B[3 * size_t(c)], B[3 * size_t(c) + 1], B[3 * size_t(c) + 2]);
         const double B2 = dot(b, b);
-        const double rho_tot = std::max(WH[5 * size_t(c)] + WNa[5 * size_t(c)], cfg.rho_floor_dt);
-        for (int s = 0; s < 2; ++s)
+        double rho_sum = Wall[0][5 * size_t(c)];
+        for (int s = 1; s < o.ns; ++s) rho_sum = rho_sum + Wall[s][5 * size_t(c)];
+        const double rho_tot = std::max(rho_sum, cfg.rho_floor_dt);
+        for (int s = 0; s < o.ns; ++s)
         {
-            const double *w = (s == 0 ? WH : WNa) + 5 * size_t(c);
+            const double *w = Wall[s] + 5 * size_t(c);
             const double cs2 = cfg.gamma * w[4] / std::max(w[0], 1e-30);
             double denom = 0;
             for (int64_t k = o.c2f.ptr[c]; k < o.c2f.ptr[c + 1]; ++k)
@@ -853,7 +863,7 @@ void prepare(orc_ctx &o)
 {
     const int32_t nc = o.nc;
     const mhdg_config &c = o.cfg;
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < o.ns; ++k)
 #pragma omp parallel for schedule(static)
         for (int32_t i = 0; i < nc; ++i)
         {
@@ -870,8 +880,15 @@ void prepare(orc_ctx &o)
         double up[3] = {0, 0, 0};
         if (o.mat[i] == 0)
         {
-            const NumInfo ni = numinfo(c, o.W[0][5 * size_t(i)], o.W[1][5 * size_t(i)]);
-            for (int a = 0; a < 3; ++a) up[a] = ni.chiH * o.W[0][5 * size_t(i) + 1 + a] + ni.chiNa * o.W[1][5 * size_t(i) + 1 + a];
+            // u+ = sum of chi_s u_s, chi_s = n_s / n_e (chi = 1, 0, ... for n_e <= 1e-300)
+            const double *Wall[10];
+            for (int s = 0; s < o.ns; ++s) Wall[s] = o.W[s].data();
+            const double ne = electron_n(o, Wall, i);
+            for (int s = 0; s < o.ns; ++s)
+            {
+                const double chi = ne > 1e-300 ? ion_n(o, s, o.W[s][5 * size_t(i)]) / ne : (s == 0 ? 1.0 : 0.0);
+                for (int a = 0; a < 3; ++a) up[a] = s == 0 ? chi * o.W[0][5 * size_t(i) + 1 + a] : up[a] + chi * o.W[s][5 * size_t(i) + 1 + a];
+            }
         }
         for (int a = 0; a < 3; ++a) o.uplus[3 * size_t(i) + a] = up[a];
     }
@@ -883,7 +900,7 @@ void prepare(orc_ctx &o)
 void enforce_floors(orc_ctx &o)
 {
     const mhdg_config &c = o.cfg;
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < o.ns; ++k)
 #pragma omp parallel for schedule(static)
         for (int32_t i = 0; i < o.nc; ++i)
         {
@@ -907,16 +924,18 @@ double step(orc_ctx &o, double dt_max, int *iters, double *res)
     const mhdg_config &c = o.cfg;
     const int32_t nc = o.nc, ne = o.ne;
     prepare(o);
-    double gdt = local_dt(o, o.W[0].data(), o.W[1].data(), o.Btot.data());
+    const double *Wall[10];
+    for (int s = 0; s < o.ns; ++s) Wall[s] = o.W[s].data();
+    double gdt = local_dt(o, Wall, o.Btot.data());
     gdt = std::min(gdt, dt_max);
     if (!std::isfinite(gdt) || gdt <= 0) throw std::runtime_error("invalid time step");
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < o.ns; ++k)
     {
-        if (c.second_order) limited_gradients(o, o.W[k].data(), k, fs_of(c.fs[k]), true, nullptr, o.grad[k].data());
+        if (c.second_order) limited_gradients(o, o.W[k].data(), k, fs_of(o.fs_s[k]), true, nullptr, o.grad[k].data());
         else std::fill(o.grad[k].begin(), o.grad[k].end(), 0.0);
-        flux_rhs(o, k, fs_of(c.fs[k]), o.W[k].data(), o.grad[k].data(), nullptr, o.rhs[k].data());
+        flux_rhs(o, k, fs_of(o.fs_s[k]), o.W[k].data(), o.grad[k].data(), nullptr, o.rhs[k].data());
     }
-    for (int k = 0; k < 2; ++k) add_sources(o, k, o.W[0].data(), o.W[1].data(), o.Btot.data(), o.J.data(), gdt, o.rhs[k].data());
+    for (int k = 0; k < o.ns; ++k) add_sources(o, k, Wall, o.Btot.data(), o.J.data(), gdt, o.rhs[k].data(), true);
     std::fill(o.E.begin(), o.E.end(), 0.0);
     ideal_emf(o, o.uplus.data(), o.Btot.data(), o.Bind.data(), o.PhiTot.data(), o.Phi.data(), c.cuct, o.E.data());
     const V3 u_inf(c.u_inf[0], c.u_inf[1], c.u_inf[2]), B_imf(c.B_imf[0], c.B_imf[1], c.B_imf[2]);
@@ -926,7 +945,7 @@ double step(orc_ctx &o, double dt_max, int *iters, double *res)
         if (o.emode[e] == 1) o.E[e] = 0.0;
         else if (o.emode[e] == 2) o.E[e] = dot(E_inf, L3(o.evec, e));
     }
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < o.ns; ++k)
 #pragma omp parallel for schedule(static)
         for (int32_t i = 0; i < nc; ++i)
         {
@@ -941,7 +960,11 @@ double step(orc_ctx &o, double dt_max, int *iters, double *res)
     {
         std::vector<double> ne_eff(nc, 0.0);
         for (int32_t i = 0; i < nc; ++i)
-            if (o.mat[i] == 0) ne_eff[i] = numinfo(c, o.W[0][5 * size_t(i)], o.W[1][5 * size_t(i)]).ne_eff;
+            if (o.mat[i] == 0)
+            {
+                const double n = electron_n(o, Wall, i);
+                ne_eff[i] = std::sqrt(n * n + c.ne_floor * c.ne_floor);
+            }
         for (size_t i = 0; i < o.B0_.size(); ++i) o.B0_[i] = o.Btot[i];
         for (int32_t i = 0; i < nc; ++i) o.alpha_[i] = (o.hall && ne_eff[i] > 0) ? o.taper_[i] * c.hall_coef / ne_eff[i] : 0.0;
         std::vector<int8_t> dir(ne);
@@ -971,9 +994,27 @@ int orc_create(const mhdg_mesh_desc *d, const mhdg_config *cfg, orc_ctx **out)
         o->ecp = cp(d->ec_ptr, E + 1); o->ecc = cp(d->ec_cell, size_t(o->ecp[E])); o->ecw = cp(d->ec_weight, size_t(o->ecp[E]));
         o->efw = cp(d->ef_weight, o->e2f.idx.size()); o->star1 = cp(d->star1, E); o->star2 = cp(d->star2, F);
         o->cfg = *cfg;
+        if (cfg->n_species > 10 || cfg->n_species < 0 || cfg->n_species == 1) throw std::runtime_error("n_species must be 0 or 2..10");
+        o->ns = cfg->n_species > 2 ? cfg->n_species : 2;
+        for (int s = 0; s < o->ns; ++s)
+        {
+            o->M_s[s] = o->ns == 2 ? (s == 0 ? cfg->M_H : cfg->M_Na) : cfg->ion_mass[s];
+            for (int m = 0; m < 5; ++m) o->fs_s[s][m] = o->ns == 2 ? cfg->fs[s][m] : cfg->ion_fs[s][m];
+        }
+        bool production = false;
+        for (int s = 0; s < o->ns; ++s) production = production || cfg->src_rate[s] != 0.0;
+        if (production)
+        {
+            o->atm.resize(C);
+            for (size_t i = 0; i < C; ++i)
+            {
+                const double *x = &o->ccen[3 * i];
+                o->atm[i] = std::exp(-(std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) - cfg->src_r0) / cfg->src_scale_height);
+            }
+        }
         build_stencil(*o);
         build_induction(*o);
-        for (int k = 0; k < 2; ++k) { o->U[k].assign(5 * C, 0.0); o->W[k].assign(5 * C, 0.0); o->grad[k].assign(15 * C, 0.0); o->rhs[k].assign(5 * C, 0.0); }
+        for (int k = 0; k < o->ns; ++k) { o->U[k].assign(5 * C, 0.0); o->W[k].assign(5 * C, 0.0); o->grad[k].assign(15 * C, 0.0); o->rhs[k].assign(5 * C, 0.0); }
         o->Phi.assign(F, 0.0); o->PhiAdd.assign(F, 0.0); o->BaddCell.assign(3 * C, 0.0); o->PhiTot.assign(F, 0.0);
         o->Bind.assign(3 * C, 0.0); o->Btot.assign(3 * C, 0.0); o->uplus.assign(3 * C, 0.0); o->J.assign(3 * C, 0.0); o->E.assign(E, 0.0);
         *out = o;
@@ -1023,10 +1064,18 @@ int orc_div_b(orc_ctx *o, const double *B, const double *extra, double *max_abs,
     *max_abs = ma; *max_rel = mr; *worst = wc;
     return 0;
 }
-int orc_local_dt(orc_ctx *o, const double *WH, const double *WNa, const double *B, double *dt) { *dt = local_dt(*o, WH, WNa, B); return 0; }
+int orc_local_dt(orc_ctx *o, const double *WH, const double *WNa, const double *B, double *dt)
+{
+    if (o->ns != 2) return 1;
+    const double *Wall[2] = {WH, WNa};
+    *dt = local_dt(*o, Wall, B);
+    return 0;
+}
 int orc_add_sources(orc_ctx *o, int s, const double *WH, const double *WNa, const double *B, const double *J, double dt, double *rhs)
 {
-    add_sources(*o, s, WH, WNa, B, J, dt, rhs);
+    if (o->ns != 2 || s < 0 || s > 1) return 1;
+    const double *Wall[2] = {WH, WNa};
+    add_sources(*o, s, Wall, B, J, dt, rhs, false);
     return 0;
 }
 int orc_ind_set_frozen(orc_ctx *o, const double *B0, const double *ne)
@@ -1077,6 +1126,24 @@ int orc_get_state(orc_ctx *o, double *UH, double *UNa, double *Phi)
     std::copy(o->U[0].begin(), o->U[0].end(), UH);
     std::copy(o->U[1].begin(), o->U[1].end(), UNa);
     std::copy(o->Phi.begin(), o->Phi.end(), Phi);
+    return 0;
+}
+int orc_set_species_state(orc_ctx *o, int s, const double *U)
+{
+    if (s < 0 || s >= o->ns) return 1;
+    std::copy(U, U + o->U[s].size(), o->U[s].begin());
+    return 0;
+}
+int orc_get_species_state(orc_ctx *o, int s, double *U)
+{
+    if (s < 0 || s >= o->ns) return 1;
+    std::copy(o->U[s].begin(), o->U[s].end(), U);
+    return 0;
+}
+int orc_get_species_prim(orc_ctx *o, int s, double *W)
+{
+    if (s < 0 || s >= o->ns) return 1;
+    std::copy(o->W[s].begin(), o->W[s].end(), W);
     return 0;
 }
 int orc_set_background(orc_ctx *o, const double *PhiAdd, const double *Badd)

@@ -68,6 +68,8 @@ class Oracle:
                 "orc_set_state": [_P, _P, _P, _P], "orc_get_state": [_P, _P, _P, _P],
                 "orc_set_background": [_P, _P, _P], "orc_step": [_P, _D, _P, _P, _P],
                 "orc_get_derived": [_P, C.c_int, _P],
+                "orc_set_species_state": [_P, C.c_int, _P], "orc_get_species_state": [_P, C.c_int, _P],
+                "orc_get_species_prim": [_P, C.c_int, _P],
             }.items():
                 getattr(L, name).argtypes = args
             cls._lib = L
@@ -170,6 +172,22 @@ class Oracle:
         UH, UNa, Phi = np.zeros(5 * self.nc), np.zeros(5 * self.nc), np.zeros(self.nf)
         self.lib().orc_get_state(self._h, _ptr(UH), _ptr(UNa), _ptr(Phi))
         return UH, UNa, Phi
+
+    def set_species_state(self, species, U):
+        if self.lib().orc_set_species_state(self._h, int(species), _ptr(_f(U))):
+            raise RuntimeError("species out of range")
+
+    def get_species_state(self, species):
+        U = np.zeros(5 * self.nc)
+        if self.lib().orc_get_species_state(self._h, int(species), _ptr(U)):
+            raise RuntimeError("species out of range")
+        return U
+
+    def get_species_prim(self, species):
+        W = np.zeros(5 * self.nc)
+        if self.lib().orc_get_species_prim(self._h, int(species), _ptr(W)):
+            raise RuntimeError("species out of range")
+        return W
 
     def set_background(self, PhiAdd, Badd):
         self.lib().orc_set_background(self._h, _ptr(_f(PhiAdd)), _ptr(_f(Badd)))

@@ -284,3 +284,61 @@ def test_fluid_calls_for_ten_ion_species(pair, species):
         g = orc.limited_gradients(W, species, fs, limit)
         assert np.array_equal(dev.limited_gradients(W, species, fs, limit), g)
         assert np.array_equal(dev.flux_rhs(species, fs, W, g), orc.flux_rhs(species, fs, W, g))
+
+
+def _ten_ion_pair(hall, net=(12, 24, 3)):
+    from mpcns_uns_amd import Device, workloads
+    from oracle.oracle import Oracle
+
+    w = workloads.build("shell_4m_mars", net=net, hall=hall)
+    dev, orc = Device(w["mesh"], w["cfg"]), Oracle(w["mesh"], w["cfg"])
+    workloads.load(dev, w)
+    orc.set_background(*w["background"])
+    orc.set_state(*w["state"])
+    for s, U in enumerate(w["species_state"][2:], start=2):
+        orc.set_species_state(s, U)
+    return dev, orc, w
+
+
+def test_ten_ion_step_explicit_bitwise():
+    """Fused step of the ten-ion fluid (crustal dipoles, ion production, no Hall): bitwise for every species."""
+    dev, orc, w = _ten_ion_pair(False)
+    for _ in range(3):
+        assert dev.step() == orc.step()
+    for s in range(10):
+        assert np.array_equal(dev.get_species_state(s), orc.get_species_state(s)), s
+        assert np.array_equal(dev.get_species_prim(s), orc.get_species_prim(s)), s
+    assert np.array_equal(dev.get_state()[2], orc.get_state()[2])
+    for n in ("B_ind", "B_tot", "J", "u_plus", "E"):
+        assert np.array_equal(dev.derived(n), orc.derived(n)), n
+
+
+def test_ten_ion_step_hall_fp64_and_mixed():
+    from mpcns_uns_amd import Device, workloads
+
+    dev, orc, w = _ten_ion_pair(True)
+    mix = Device(w["mesh"], w["cfg"])
+    workloads.load(mix, w)
+    for _ in range(3):
+        dt_d, it_d, r_d = dev.step(precision=0)
+        dt_o, it_o, r_o = orc.step()
+        dt_m, it_m, r_m = mix.step(precision=1)
+        assert r_d < 1e-8 and r_m < 1e-8 and abs(it_d - it_o) <= 1
+    for s in range(10):
+        assert maxrel(dev.get_species_state(s), orc.get_species_state(s)) < 1e-12, s
+        assert maxrel(mix.get_species_state(s), dev.get_species_state(s)) < 1e-8, s
+    Phi = dev.get_state()[2]
+    assert maxrel(Phi, orc.get_state()[2]) < 1e-12
+    assert maxrel(mix.get_state()[2], Phi) < 1e-8
+    _, rel, _ = dev.div_b(Phi, w["background"][0])
+    assert rel < 1e-12
+
+
+def test_species_entries_reject_out_of_range():
+    dev, _, _ = _ten_ion_pair(False, net=(6, 12, 2))
+    with pytest.raises(RuntimeError):
+        dev.get_species_state(10)
+    with pytest.raises(RuntimeError):
+        dev.set_species_state(-1, np.zeros(5 * dev.nc))
+    with pytest.raises(RuntimeError):  # the two-ion per-call entries refuse a ten-ion context
+        dev.local_dt(np.zeros(5 * dev.nc), np.zeros(5 * dev.nc), np.zeros(3 * dev.nc))

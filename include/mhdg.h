@@ -182,6 +182,16 @@ int mhdg_explicit_tangent(mhdg_ctx *ctx, int n_steps, double dt_max, const doubl
                           const double *dE_inf, double r0, double r1, double *J, double *dJ, double *dU_H_out,
                           double *dU_Na_out, double *dPhi_out);
 
+/* Reverse mode (adjoint) of mhdg_explicit_tangent: J, the shell magnetic energy after n_steps explicit steps from
+ * the resident state (left unchanged), and its gradient with respect to the initial U_H, U_Na (5C each), Phi (F),
+ * the background PhiAdd (F) and BaddCell (3C), the free-stream states (2x5, species-major) and the imposed inflow
+ * EMF E_inf (3).  Every output may be NULL.  The forward sweep keeps each step's state (n_steps x (10C + F)
+ * doubles of device memory) and the reverse sweep recomputes the step's limited gradients, fluxes and EMFs from
+ * it.  Two-ion contexts.  No single reference function: the adjoint of step_first_order's explicit stage
+ * (MercuryUns/driver/Solver.cpp:228-277), the counterpart of mhdg_ind_jvp on the explicit side. */
+int mhdg_step_vjp(mhdg_ctx *ctx, int n_steps, double dt_max, double r0, double r1, double *J, double *gU_H, double *gU_Na,
+                  double *gPhi, double *gPhiAdd, double *gBadd, double *gfs, double *gE_inf);
+
 /* --------------------------------------------------------------- device-resident step (fused path) */
 /* State: U_H, U_Na (5C each, AoS), induced flux Phi (F); background PhiAdd (F) and BaddCell (3C). */
 int mhdg_set_state(mhdg_ctx *ctx, const double *U_H, const double *U_Na, const double *Phi);

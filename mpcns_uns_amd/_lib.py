@@ -105,6 +105,7 @@ def lib():
             "mhdg_get_species_prim": [P, C.c_int, P], "mhdg_ind_apply_G": [P, P, P], "mhdg_ind_jvp": [P, P, P, P],
             "mhdg_ind_newton": [P, P, P, P, D, P, D, C.c_int, D, P, P, P, P],
             "mhdg_shell_energy": [P, D, D, P],
+            "mhdg_step_vjp": [P, C.c_int, D, D, D, P, P, P, P, P, P, P, P],
             "mhdg_explicit_tangent": [P, C.c_int, D, P, P, P, P, P, P, P, D, D, P, P, P, P, P], "mhdg_set_recon_rel_limit": [P, D], "mhdg_step_hall": [P, D, C.c_int, P, P, P],
             "mhdg_set_state": [P, P, P, P], "mhdg_get_state": [P, P, P, P],
             "mhdg_set_background": [P, P, P], "mhdg_step": [P, D, C.c_int, P, P, P],

@@ -43,6 +43,17 @@ struct mhdg_ctx
                          *Btot = nullptr, *uplus = nullptr, *J = nullptr, *alpha = nullptr, *FA = nullptr, *Ef = nullptr, *E = nullptr,
                          *dt_part = nullptr, *dt_dev = nullptr, *part = nullptr; } tw;
     double *energy_part = nullptr;
+    // reverse mode of the explicit step (adjoint.hip): gather maps, recomputed intermediates, adjoints, slots,
+    // and the state of every step of the forward sweep
+    struct AdjointWork { int32_t *fsl = nullptr, *f2e_pos = nullptr, *cec_ptr = nullptr, *cec = nullptr, *argc = nullptr;
+                         double *U[2]{}, *W[2]{}, *G[2]{}, *Ub[2]{}, *Wb[2]{}, *Gb[2]{}, *sF[2]{}, *sG[2]{}, *sFA[2]{};
+                         double *Phi = nullptr, *Bind = nullptr, *Btot = nullptr, *up = nullptr, *J = nullptr, *alpha = nullptr, *FA = nullptr,
+                                *Ef = nullptr, *E = nullptr, *dt_part = nullptr, *dt_dev = nullptr;
+                         double *Phib = nullptr, *PhiAddb = nullptr, *Baddb = nullptr, *Eb = nullptr, *Efb = nullptr, *FAb = nullptr, *upb = nullptr,
+                                *Btb = nullptr, *Bib = nullptr, *Jb = nullptr, *sE = nullptr, *sPhi = nullptr, *eF = nullptr, *eC = nullptr,
+                                *part = nullptr;
+                         mhdg::Dual *wd = nullptr;
+                         std::vector<double *> ck; } aw;
     struct NewtonWork { double *phik = nullptr, *G = nullptr, *R = nullptr, *rhs = nullptr, *delta = nullptr, *E = nullptr,
                         *lin_b = nullptr, *B0 = nullptr, *zero_phi = nullptr, *corr_save = nullptr; } nt;
     std::vector<double *> V;
@@ -84,6 +95,9 @@ struct mhdg_ctx
     void tangent_alloc();
     void explicit_tangent(int n_steps, double dt_max, const mhdg::Params &Pt);
     template <class R> void shell_energy(const R *phi, const R *badd, double r0, double r1, R *out);
+    void vjp_alloc(int n_steps);
+    double vjp_stage(double dt_max, bool update);
+    double step_vjp(int n_steps, double dt_max, double r0, double r1, double *fs_b, double *einf_b);
     void apply_G(const double *phi, double *out);
     void jvp(const double *phi_lin, const double *dphi, double *out);
     int newton(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, const double *E0, double rtol, int max_newton,

@@ -115,6 +115,7 @@ void mhdg_ctx::prof_flush()
 mhdg_ctx::~mhdg_ctx()
 {
     delete mixed;
+    for (double *p : aw.ck) (void)hipFree(p);
     for (auto &r : prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (void *p : owned) (void)hipFree(p);
     for (auto &kv : staging) if (kv.second.first) (void)hipFree(kv.second.first);
@@ -831,6 +832,22 @@ int mhdg_explicit_tangent(mhdg_ctx *c, int n_steps, double dt_max, const double 
         out(c->tw.U[0], 5 * C, dU_H_out);
         out(c->tw.U[1], 5 * C, dU_Na_out);
         out(c->tw.Phi, F, dPhi_out);
+    })
+}
+int mhdg_step_vjp(mhdg_ctx *c, int n_steps, double dt_max, double r0, double r1, double *J, double *gU_H, double *gU_Na, double *gPhi,
+                  double *gPhiAdd, double *gBadd, double *gfs, double *gE_inf)
+{
+    MHDG_TRY(c, {
+        if (n_steps < 0) throw std::runtime_error("step_vjp: n_steps >= 0");
+        if (c->P.ns != 2) throw std::runtime_error("step_vjp: two-species (H+, Na+) contexts only");
+        const size_t C = c->nc, F = c->nf;
+        const double v = c->step_vjp(n_steps, dt_max, r0, r1, gfs, gE_inf);
+        if (J) *J = v;
+        if (gU_H) c->d2h(gU_H, c->aw.Ub[0], 5 * C * 8);
+        if (gU_Na) c->d2h(gU_Na, c->aw.Ub[1], 5 * C * 8);
+        if (gPhi) c->d2h(gPhi, c->aw.Phib, F * 8);
+        if (gPhiAdd) c->d2h(gPhiAdd, c->aw.PhiAddb, F * 8);
+        if (gBadd) c->d2h(gBadd, c->aw.Baddb, 3 * C * 8);
     })
 }
 int mhdg_set_state(mhdg_ctx *c, const double *UH, const double *UNa, const double *Ph)

@@ -221,7 +221,7 @@ void mhdg_ctx::tangent_alloc()
     tw.Bind = D(3 * C); tw.Btot = D(3 * C); tw.uplus = D(3 * C); tw.J = D(3 * C); tw.alpha = D(C);
     tw.FA = D(10 * F); tw.Ef = D(3 * F); tw.E = D(E);
     tw.dt_part = D(size_t(nbk(nc)) + 1); tw.dt_dev = D(1); tw.part = D(size_t(nbk(nc)));
-    energy_part = static_cast<double *>(alloc(size_t(nbk(nc)) * 8));
+    if (!energy_part) energy_part = static_cast<double *>(alloc(size_t(nbk(nc)) * 8));
 }
 
 // n steps of step_first_order's explicit stage (prepare, CFL, gradients, fluxes, sources, update, UCT EMF with
@@ -273,8 +273,17 @@ template <class R> void mhdg_ctx::shell_energy(const R *phi, const R *badd, doub
 {
     const int nb = nbk(nc);
     R *part = nullptr;
-    if constexpr (std::is_same<R, Dual>::value) part = tw.part;
-    else part = energy_part;
+    if constexpr (std::is_same<R, Dual>::value)
+    {
+        if (!tw.part) throw std::runtime_error("shell energy: tangent work arrays not allocated");
+        part = tw.part;
+    }
+    else
+    {
+        // per-block partial sums of the double evaluation (any caller: the energy entry, the tangent, the adjoint)
+        if (!energy_part) energy_part = static_cast<double *>(alloc(size_t(nb) * 8));
+        part = energy_part;
+    }
     hipLaunchKernelGGL(k_shell_energy<R>, dim3(nb), dim3(256), 0, stream, d, phi, badd, r0, r1, part);
     check_launch("shell energy");
     std::vector<R> h(nb);

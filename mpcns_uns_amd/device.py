@@ -162,6 +162,17 @@ class Device:
                                              *[ptr(a) for a in outs]))
         return (J.value, dJ.value, *outs) if states else (J.value, dJ.value)
 
+    def step_vjp(self, n_steps, r0, r1, dt_max=1e9):
+        """Reverse mode of explicit_tangent: J and its gradient with respect to the initial state and the
+        background / free-stream parameters, as a dict (U_H, U_Na, Phi, PhiAdd, Badd, fs (2x5), E_inf)."""
+        J = C.c_double()
+        g = dict(U_H=np.zeros(5 * self.nc), U_Na=np.zeros(5 * self.nc), Phi=np.zeros(self.nf), PhiAdd=np.zeros(self.nf),
+                 Badd=np.zeros(3 * self.nc), fs=np.zeros(10), E_inf=np.zeros(3))
+        self._ok(lib().mhdg_step_vjp(self._h, n_steps, dt_max, r0, r1, C.byref(J), *[ptr(g[k]) for k in
+                                     ("U_H", "U_Na", "Phi", "PhiAdd", "Badd", "fs", "E_inf")]))
+        g["fs"] = g["fs"].reshape(2, 5)
+        return J.value, g
+
     # ------------------------------------------------------------------ resident state
     def set_state(self, U_H, U_Na, Phi):
         self._ok(lib().mhdg_set_state(self._h, ptr(f64(U_H)), ptr(f64(U_Na)), ptr(f64(Phi))))

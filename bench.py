@@ -77,6 +77,36 @@ def cpu_baseline(args):
         return {"error": str(e)}
 
 
+def differentiation(args):
+    """Gradient of J = sum |B|^2 V over 1.2 <= r <= 1.5 after `steps` explicit steps with respect to the 3 IMF
+    components and the upstream density: 4 tangent sweeps against one adjoint sweep (all inputs at once)."""
+    from mpcns_uns_amd import Device, workloads
+
+    w = workloads.build(args.workload, net=tuple(args.net) if args.net else None, hall=False)
+    mesh = w["mesh"]
+    dev = Device(mesh, w["cfg"])
+    workloads.load(dev, w)
+    r0, r1, n = 1.2, 1.5, args.steps
+    dPhiAdd = np.ascontiguousarray(mesh.array("face_area").reshape(-1, 3)[:, 0])
+    dBadd = mesh.cell_vector(dPhiAdd)
+    dev.step_vjp(n, r0, r1)  # warm-up: allocation and gather maps
+    dev.explicit_tangent(n, r0, r1, dPhiAdd=dPhiAdd, dBadd=dBadd)
+    dev.sync()
+    t = time.perf_counter()
+    J, g = dev.step_vjp(n, r0, r1)
+    dev.sync()
+    t_adj = time.perf_counter() - t
+    t = time.perf_counter()
+    _, dJ = dev.explicit_tangent(n, r0, r1, dPhiAdd=dPhiAdd, dBadd=dBadd)
+    dev.sync()
+    t_tan = time.perf_counter() - t
+    agree = abs(g["PhiAdd"] @ dPhiAdd + g["Badd"] @ dBadd - dJ) / abs(dJ)
+    print(json.dumps({"metric": "shell-energy gradient after n explicit steps, wall s", "workload": args.workload,
+                      "cells": mesh.n_cells, "steps": n, "adjoint_s": round(t_adj, 4), "tangent_sweep_s": round(t_tan, 4),
+                      "tangent_4_params_s": round(4 * t_tan, 4), "J": J,
+                      "adjoint_vs_tangent_rel": agree}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,7 +122,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a device-to-device copy of GIB GiB five times (scale of the HBM byte counters)")
+    ap.add_argument("--differentiation", action="store_true",
+                    help="only time the gradient of the shell energy after --steps explicit steps (forward mode: one "
+                         "tangent sweep per parameter; reverse mode: mhdg_step_vjp) on the workload's mesh, Hall off")
     args = ap.parse_args()
+    if args.differentiation:
+        differentiation(args)
+        return
     if args.copy_calibration > 0:
         import torch
 

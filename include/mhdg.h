@@ -201,6 +201,9 @@ int mhdg_get_species_state(mhdg_ctx *ctx, int species, double *U);
 int mhdg_get_species_prim(mhdg_ctx *ctx, int species, double *W);
 int mhdg_get_state(mhdg_ctx *ctx, double *U_H, double *U_Na, double *Phi);
 int mhdg_set_background(mhdg_ctx *ctx, const double *PhiAdd, const double *BaddCell);
+/* Solver::prepare (MercuryUns/driver/Solver.cpp:94-125) on the resident state: primitives, B_ind, B_tot, u+ and
+ * J of the current state for mhdg_get_derived (what Solver::write_vtk outputs, Solver.cpp:677-696). */
+int mhdg_prepare(mhdg_ctx *ctx);
 /* mercury::Solver::step_first_order (MercuryUns/driver/Solver.cpp:228-277) on the resident state:
  * prepare, CFL step (min with dt_max), limited gradients, Rusanov RHS, Lorentz sources, forward Euler
  * + floors, ideal UCT EMF + boundary EMF, [implicit Hall/resistive solve], CT.  `precision` as in

@@ -102,6 +102,21 @@ int mhdg_fields_read(const char *path, mhdg_fields **out, uint64_t *hash, int32_
 int mhdg_fields_get(const mhdg_fields *f, int32_t i, mhdg_field *field);
 void mhdg_fields_free(mhdg_fields *f);
 
+/* VTK XML output as uns::io::write_vtu / write_pvtu / write_pvd (src/uns/io/VtkWriter.h:20-26,
+ * VtkWriter.cpp:42-116): every cell a polyhedron, raw appended binary payloads; byte-identical with the
+ * reference's writer for the same mesh and arrays.  values: ncomp doubles per cell (cell arrays) or per
+ * node (point arrays). */
+typedef struct mhdg_vtk_array {
+    const char *name;
+    int32_t ncomp;
+    const double *values;
+} mhdg_vtk_array;
+int mhdg_mesh_write_vtu(const mhdg_mesh *m, const char *path, int32_t n_cell_arrays, const mhdg_vtk_array *cell_arrays,
+                        int32_t n_point_arrays, const mhdg_vtk_array *point_arrays);
+int mhdg_write_pvtu(const char *path, int32_t n_pieces, const char *const *pieces, int32_t n_cell_arrays,
+                    const mhdg_vtk_array *cell_arrays, int32_t n_point_arrays, const mhdg_vtk_array *point_arrays);
+int mhdg_write_pvd(const char *path, int32_t n, const double *times, const char *const *files);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,9 @@ def lib():
         L.mhdg_mesh_tag_names.argtypes = [P, C.c_char_p, C.c_int]
         L.mhdg_mesh_set_tag_names.argtypes = [P, C.c_char_p]
         L.mhdg_fields_write.argtypes = [C.c_char_p, C.c_uint64, P, I32]
+        L.mhdg_mesh_write_vtu.argtypes = [P, C.c_char_p, I32, P, I32, P]
+        L.mhdg_write_pvtu.argtypes = [C.c_char_p, I32, P, I32, P, I32, P]
+        L.mhdg_write_pvd.argtypes = [C.c_char_p, I32, P, P]
         L.mhdg_fields_read.argtypes = [C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64), C.POINTER(I32)]
         L.mhdg_fields_get.argtypes = [P, I32, P]
         L.mhdg_fields_free.argtypes = [P]
@@ -109,7 +112,7 @@ def lib():
             "mhdg_explicit_tangent": [P, C.c_int, D, P, P, P, P, P, P, P, D, D, P, P, P, P, P], "mhdg_set_recon_rel_limit": [P, D], "mhdg_step_hall": [P, D, C.c_int, P, P, P],
             "mhdg_set_state": [P, P, P, P], "mhdg_get_state": [P, P, P, P],
             "mhdg_set_background": [P, P, P], "mhdg_step": [P, D, C.c_int, P, P, P],
-            "mhdg_get_derived": [P, C.c_int, P], "mhdg_sync": [P], "mhdg_device_info": [C.c_char_p, C.c_int],
+            "mhdg_get_derived": [P, C.c_int, P], "mhdg_prepare": [P], "mhdg_sync": [P], "mhdg_device_info": [C.c_char_p, C.c_int],
             "mhdg_profile": [P, C.c_int], "mhdg_profile_report": [P, C.c_char_p, C.c_int],
         }.items():
             getattr(L, name).argtypes = args

@@ -13,6 +13,7 @@
 #include "geometry.h"
 #include "shell.h"
 #include "unsbin.h"
+#include "vtu.h"
 #include "vec3.h"
 
 struct mhdg_mesh
@@ -90,6 +91,20 @@ mhdg_mesh *finish_mesh(mhdg_mesh *m)
 {
     m->g = mhdg::compute_geometry(m->cx, m->xyz.data());
     return m;
+}
+} // namespace
+
+namespace
+{
+std::vector<mhdg::vtk::Array> vtk_arrays(int32_t n, const mhdg_vtk_array *a)
+{
+    std::vector<mhdg::vtk::Array> out;
+    for (int32_t i = 0; i < n; ++i)
+    {
+        if (!a[i].name || a[i].ncomp < 1 || !a[i].values) throw std::runtime_error("vtk array: name, ncomp >= 1 and values required");
+        out.push_back({a[i].name, a[i].ncomp, a[i].values});
+    }
+    return out;
 }
 } // namespace
 
@@ -476,5 +491,44 @@ int mhdg_mesh_seeded_state(const mhdg_mesh *m, const mhdg_config *cfg, double no
     const double scale = rms_phi > 0 ? noise * rms_b / rms_phi : 0.0;
     for (int32_t f = 0; f < cx.nf; ++f) Phi[f] *= scale;
     return 0;
+}
+
+int mhdg_mesh_write_vtu(const mhdg_mesh *m, const char *path, int32_t n_cell_arrays, const mhdg_vtk_array *cell_arrays,
+                        int32_t n_point_arrays, const mhdg_vtk_array *point_arrays)
+{
+    try
+    {
+        mhdg::vtk::write_vtu(path, m->cx, m->xyz, m->cx.nc, vtk_arrays(n_cell_arrays, cell_arrays), vtk_arrays(n_point_arrays, point_arrays));
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+int mhdg_write_pvtu(const char *path, int32_t n_pieces, const char *const *pieces, int32_t n_cell_arrays, const mhdg_vtk_array *cell_arrays,
+                    int32_t n_point_arrays, const mhdg_vtk_array *point_arrays)
+{
+    try
+    {
+        // master files carry names only: values may be NULL here
+        auto names = [](int32_t n, const mhdg_vtk_array *a) {
+            std::vector<mhdg::vtk::Array> out;
+            for (int32_t i = 0; i < n; ++i) out.push_back({a[i].name ? a[i].name : "", a[i].ncomp, nullptr});
+            return out;
+        };
+        std::vector<std::string> p(pieces, pieces + n_pieces);
+        mhdg::vtk::write_pvtu(path, p, names(n_cell_arrays, cell_arrays), names(n_point_arrays, point_arrays));
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+int mhdg_write_pvd(const char *path, int32_t n, const double *times, const char *const *files)
+{
+    try
+    {
+        std::vector<std::pair<double, std::string>> e;
+        for (int32_t i = 0; i < n; ++i) e.emplace_back(times[i], files[i]);
+        mhdg::vtk::write_pvd(path, e);
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
 }
 }

@@ -935,6 +935,21 @@ int mhdg_step_hall(mhdg_ctx *c, double dt_max, int precision, double *dt_used, i
         if (rel_res) *rel_res = rr;
     })
 }
+int mhdg_prepare(mhdg_ctx *c)
+{
+    MHDG_TRY(c, {
+        const int nb = nblk(c->nc);
+        SpC Uc{}, none{};
+        SpW Ww{}, noneW{};
+        for (int s = 0; s < c->P.ns; ++s) { Uc.p[s] = c->U[s]; Ww.p[s] = c->W[s]; }
+        hipLaunchKernelGGL(k_prepare<double>, dim3(nb), dim3(256), 0, c->stream, c->d, c->P, Uc, static_cast<const double *>(c->Phi),
+                           static_cast<const double *>(c->Badd), Ww, c->Bind, c->Btot, c->uplus, c->alpha, c->dt_part);
+        hipLaunchKernelGGL(k_grad_current<double>, dim3(nb), dim3(256), 0, c->stream, c->d, c->P, 0, 0, none, none, noneW, 1,
+                           static_cast<const double *>(c->Bind), c->J);
+        c->check_launch("prepare");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    })
+}
 int mhdg_get_derived(mhdg_ctx *c, int which, double *out)
 {
     MHDG_TRY(c, {

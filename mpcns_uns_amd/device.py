@@ -226,6 +226,10 @@ class Device:
         self._ok(lib().mhdg_step_hall(self._h, dt_max, precision, C.byref(dt), C.byref(it), C.byref(rr)))
         return dt.value, it.value, rr.value
 
+    def prepare(self):
+        """Solver::prepare on the resident state: derived() then gives the current state's fields."""
+        self._ok(lib().mhdg_prepare(self._h))
+
     def derived(self, name):
         n = {0: 5, 1: 5, 6: None}.get(DERIVED[name], 3)
         out = np.zeros(self.ne if n is None else n * self.nc)

@@ -77,6 +77,14 @@ class Mesh:
         if lib().mhdg_mesh_write_unsbin(self._h, str(path).encode()):
             raise RuntimeError(lib().mhdg_mesh_error().decode())
 
+    def write_vtu(self, path, cell_data=(), point_data=()):
+        """VTK unstructured-grid file of every cell as a polyhedron (uns::io::write_vtu layout).
+        cell_data / point_data: [(name, ncomp, values)] with ncomp values per cell / node."""
+        keep = []
+        cells, points = _vtk_arrays(cell_data, self.n_cells, keep), _vtk_arrays(point_data, self.n_nodes, keep)
+        if lib().mhdg_mesh_write_vtu(self._h, str(path).encode(), len(cell_data), cells, len(point_data), points):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+
     @property
     def topology_hash(self):
         h = C.c_uint64()
@@ -202,6 +210,40 @@ def default_config(**kw):
         else:
             setattr(c, k, v)
     return c
+
+
+# --------------------------------------------------------------------------- VTK output
+class _VtkArray(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("ncomp", C.c_int32), ("values", C.c_void_p)]
+
+
+def _vtk_arrays(arrays, count, keep):
+    out = (_VtkArray * max(len(arrays), 1))()
+    for i, (name, ncomp, values) in enumerate(arrays):
+        v = None if values is None else f64(values)
+        if v is not None and count is not None and v.size != count * ncomp:
+            raise ValueError(f"{name}: {v.size} values, expected {count} x {ncomp}")
+        keep.append(v)
+        out[i] = _VtkArray(name.encode(), int(ncomp), None if v is None else v.ctypes.data)
+    return out
+
+
+def write_pvtu(path, pieces, cell_data=(), point_data=()):
+    """Parallel master file over piece files (uns::io::write_pvtu); arrays as [(name, ncomp)]."""
+    keep = []
+    cells = _vtk_arrays([(n, c, None) for n, c in cell_data], None, keep)
+    points = _vtk_arrays([(n, c, None) for n, c in point_data], None, keep)
+    names = (C.c_char_p * max(len(pieces), 1))(*[str(p).encode() for p in pieces])
+    if lib().mhdg_write_pvtu(str(path).encode(), len(pieces), names, len(cell_data), cells, len(point_data), points):
+        raise RuntimeError(lib().mhdg_mesh_error().decode())
+
+
+def write_pvd(path, entries):
+    """Time-series collection (uns::io::write_pvd): entries = [(time, file)]."""
+    times = np.array([float(t) for t, _ in entries] or [0.0])
+    files = (C.c_char_p * max(len(entries), 1))(*[str(f).encode() for _, f in entries])
+    if lib().mhdg_write_pvd(str(path).encode(), len(entries), times.ctypes.data, files):
+        raise RuntimeError(lib().mhdg_mesh_error().decode())
 
 
 # --------------------------------------------------------------------------- field files

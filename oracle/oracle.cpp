@@ -1163,6 +1163,12 @@ int orc_step(orc_ctx *o, double dt_max, double *dt_used, int32_t *iters, double 
     }
     catch (const std::exception &e) { o->err = e.what(); return 1; }
 }
+// Solver::prepare (Solver.cpp:94-125) on the current state
+int orc_prepare(orc_ctx *o)
+{
+    prepare(*o);
+    return 0;
+}
 int orc_get_derived(orc_ctx *o, int which, double *out)
 {
     const std::vector<double> *v = nullptr;

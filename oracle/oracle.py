@@ -67,7 +67,7 @@ class Oracle:
                 "orc_ind_set_ideal_state": [_P, _P, _P], "orc_ind_ideal_linear": [_P, _P, _P],
                 "orc_set_state": [_P, _P, _P, _P], "orc_get_state": [_P, _P, _P, _P],
                 "orc_set_background": [_P, _P, _P], "orc_step": [_P, _D, _P, _P, _P],
-                "orc_get_derived": [_P, C.c_int, _P],
+                "orc_get_derived": [_P, C.c_int, _P], "orc_prepare": [_P],
                 "orc_set_species_state": [_P, C.c_int, _P], "orc_get_species_state": [_P, C.c_int, _P],
                 "orc_get_species_prim": [_P, C.c_int, _P],
             }.items():
@@ -198,6 +198,9 @@ class Oracle:
             raise RuntimeError(self.lib().orc_last_error(self._h).decode())
         return dt.value, it.value, rr.value
 
+    def prepare(self):
+        self.lib().orc_prepare(self._h)
+
     def derived(self, name):
         idx = {"W_H": 0, "W_Na": 1, "B_ind": 2, "B_tot": 3, "J": 4, "u_plus": 5, "E": 6}[name]
         n = {0: 5 * self.nc, 1: 5 * self.nc, 6: self.ne}.get(idx, 3 * self.nc)
@@ -237,7 +240,7 @@ class Reference:
                 "ref_ind_set_ideal_state": [_P, _P, _P], "ref_ind_ideal_linear": [_P, _P, _P],
                 "ref_set_state": [_P, _P, _P, _P], "ref_get_state": [_P, _P, _P, _P], "ref_step": [_P, _D, _P],
                 "ref_get_derived": [_P, C.c_int, _P], "ref_last_induction_residual": [_P],
-                "ref_write_mesh": [_P, C.c_char_p], "ref_read_mesh": [C.c_char_p, _P, _P], "ref_topology_hash": [_P],
+                "ref_write_mesh": [_P, C.c_char_p], "ref_set_output_dir": [_P, C.c_char_p], "ref_write_vtk": [_P], "ref_read_mesh": [C.c_char_p, _P, _P], "ref_topology_hash": [_P],
                 "ref_write_fields": [C.c_char_p, C.c_uint64, C.c_int, _P, _P, _P, _P, _P],
                 "ref_read_field": [C.c_char_p, C.c_char_p, _P, _P, _P, _P, _P],
                 "ref_dipole_edge_integrals": [C.c_int, _P, _P, _P, _D, C.c_int64, _P, _P, _P],
@@ -388,6 +391,14 @@ class Reference:
         out = np.zeros(self.ne)
         self.lib().ref_ind_ideal_linear(self._h, _ptr(_f(phi)), _ptr(out))
         return out
+
+    def write_vtk(self, out_dir):
+        """mercury::Solver::write_vtk (MercuryUns/driver/Solver.cpp:677-718) into out_dir (set before solver_init)."""
+        if self.lib().ref_write_vtk(self._h):
+            raise RuntimeError("ref_write_vtk failed")
+
+    def set_output_dir(self, out_dir):
+        self.lib().ref_set_output_dir(self._h, str(out_dir).encode())
 
     def solver_init(self):
         if self.lib().ref_solver_init(self._h):

@@ -30,6 +30,7 @@
 #include "uns/geometry/Geometry.h"
 #include "uns/geometry/Reconstruct.h"
 #include "uns/io/MeshIO.h"
+#include "uns/io/VtkWriter.h"
 #include "MarsUns/physics/MarsDipoles.h"
 
 #include <cmath>
@@ -254,6 +255,13 @@ void ref_ind_set_ideal_state(ref_ctx *r, const double *u, const int8_t *mask)
 }
 void ref_ind_ideal_linear(ref_ctx *r, const double *phi, double *out) { r->ind->ideal_linear(phi, out); }
 
+// output directory of the solver's VTK files (before ref_solver_init) and Solver::write_vtk
+void ref_set_output_dir(ref_ctx *r, const char *dir) { r->cfg.output_dir = dir; }
+int ref_write_vtk(ref_ctx *r)
+{
+    try { r->solver->write_vtk(); return 0; }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_write_vtk: %s\n", e.what()); return 1; }
+}
 // ---- full step through mercury::Solver (MercuryUns/driver/Solver.cpp:228-277)
 int ref_solver_init(ref_ctx *r)
 {
@@ -315,6 +323,51 @@ int ref_read_mesh(const char *path, int64_t counts[4], uint64_t *hash)
     catch (const std::exception &e) { std::fprintf(stderr, "ref_read_mesh: %s\n", e.what()); return 1; }
 }
 uint64_t ref_topology_hash(ref_ctx *r) { return topology_hash(r->lm.mesh.cx); }
+// uns::io VTK writers (src/uns/io/VtkWriter.cpp): arrays as (name, ncomp, values of `counts` doubles)
+static std::vector<uns::io::VtkField> vtk_fields(int n, const char *const *names, const int32_t *ncomp, const double *const *values,
+                                                 const int64_t *counts)
+{
+    std::vector<uns::io::VtkField> f(size_t(n > 0 ? n : 0));
+    for (int i = 0; i < n; ++i)
+    {
+        f[size_t(i)].name = names[i];
+        f[size_t(i)].ncomp = ncomp[i];
+        if (values && values[i]) f[size_t(i)].values.assign(values[i], values[i] + counts[i]);
+    }
+    return f;
+}
+int ref_write_vtu(ref_ctx *r, const char *path, int nc, const char *const *cn, const int32_t *cc, const double *const *cv, const int64_t *ccount,
+                  int np, const char *const *pn, const int32_t *pc, const double *const *pv, const int64_t *pcount)
+{
+    try
+    {
+        uns::io::write_vtu(path, r->lm.mesh, r->lm.mesh.cx.count(Dim::Cell), vtk_fields(nc, cn, cc, cv, ccount), vtk_fields(np, pn, pc, pv, pcount));
+        return 0;
+    }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_write_vtu: %s\n", e.what()); return 1; }
+}
+int ref_write_pvtu(const char *path, int n, const char *const *pieces, int nc, const char *const *cn, const int32_t *cc, int np,
+                   const char *const *pn, const int32_t *pc)
+{
+    try
+    {
+        uns::io::write_pvtu(path, std::vector<std::string>(pieces, pieces + n), vtk_fields(nc, cn, cc, nullptr, nullptr),
+                            vtk_fields(np, pn, pc, nullptr, nullptr));
+        return 0;
+    }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_write_pvtu: %s\n", e.what()); return 1; }
+}
+int ref_write_pvd(const char *path, int n, const double *times, const char *const *files)
+{
+    try
+    {
+        std::vector<std::pair<double, std::string>> e;
+        for (int i = 0; i < n; ++i) e.emplace_back(times[i], files[i]);
+        uns::io::write_pvd(path, e);
+        return 0;
+    }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_write_pvd: %s\n", e.what()); return 1; }
+}
 int ref_write_fields(const char *path, uint64_t hash, int n, const char *const *names, const int32_t *dims, const int32_t *ncomps,
                      const double *const *values, const int64_t *counts)
 {

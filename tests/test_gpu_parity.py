@@ -342,3 +342,26 @@ def test_species_entries_reject_out_of_range():
         dev.set_species_state(-1, np.zeros(5 * dev.nc))
     with pytest.raises(RuntimeError):  # the two-ion per-call entries refuse a ten-ion context
         dev.local_dt(np.zeros(5 * dev.nc), np.zeros(5 * dev.nc), np.zeros(3 * dev.nc))
+
+
+def test_prepare_and_solution_files(small_explicit, tmp_path):
+    """Solver::prepare on the resident state and the Solver::write_vtk file set written from the device: the same bytes
+    as from the oracle's fields (which match the reference solver's own files, tests/test_vtu.py)."""
+    from mpcns_uns_amd.output import SolutionWriter, solution_fields
+
+    dev, orc = _fresh(small_explicit)
+    m = small_explicit["mesh"]
+    a, b = SolutionWriter(tmp_path / "dev"), SolutionWriter(tmp_path / "orc")
+    t = 0.0
+    for step in (1, 2):
+        dt = dev.step()[0]
+        assert dt == orc.step()[0]
+        t += dt
+        dev.prepare()
+        orc.prepare()
+        for n in ("W_H", "W_Na", "B_ind", "B_tot", "J", "u_plus"):
+            assert np.array_equal(dev.derived(n), orc.derived(n)), n
+        a.write_device(dev, step, t)
+        b.write(m, solution_fields(orc, m), step, t)
+    for name in ("sol_0000002_r00000.vtu", "sol_0000002.pvtu", "solution.pvd"):
+        assert (tmp_path / "dev" / name).read_bytes() == (tmp_path / "orc" / name).read_bytes(), name

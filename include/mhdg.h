@@ -204,6 +204,11 @@ int mhdg_set_background(mhdg_ctx *ctx, const double *PhiAdd, const double *BaddC
 /* Solver::prepare (MercuryUns/driver/Solver.cpp:94-125) on the resident state: primitives, B_ind, B_tot, u+ and
  * J of the current state for mhdg_get_derived (what Solver::write_vtk outputs, Solver.cpp:677-696). */
 int mhdg_prepare(mhdg_ctx *ctx);
+/* Device-side copy of the resident state (U of every species, Phi) and its restoration, which also drops the
+ * implicit solve's warm start (as at the first step).  For repeated runs of one step from a fixed state; no
+ * counterpart in the reference. */
+int mhdg_state_save(mhdg_ctx *ctx);
+int mhdg_state_restore(mhdg_ctx *ctx);
 /* mercury::Solver::step_first_order (MercuryUns/driver/Solver.cpp:228-277) on the resident state:
  * prepare, CFL step (min with dt_max), limited gradients, Rusanov RHS, Lorentz sources, forward Euler
  * + floors, ideal UCT EMF + boundary EMF, [implicit Hall/resistive solve], CT.  `precision` as in

@@ -43,6 +43,7 @@ struct mhdg_ctx
                          *Btot = nullptr, *uplus = nullptr, *J = nullptr, *alpha = nullptr, *FA = nullptr, *Ef = nullptr, *E = nullptr,
                          *dt_part = nullptr, *dt_dev = nullptr, *part = nullptr; } tw;
     double *energy_part = nullptr;
+    double *snapshot = nullptr; // mhdg_state_save: U of every species and Phi
     // reverse mode of the explicit step (adjoint.hip): gather maps, recomputed intermediates, adjoints, slots,
     // and the state of every step of the forward sweep
     struct AdjointWork { int32_t *fsl = nullptr, *f2e_pos = nullptr, *cec_ptr = nullptr, *cec = nullptr, *argc = nullptr;

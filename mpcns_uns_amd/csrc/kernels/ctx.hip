@@ -262,7 +262,8 @@ void mhdg_ctx::launch_edge_emf(const double *Efp, const double *u, const double 
 
 double mhdg_ctx::dot(const double *a, const double *b, int op, int slot)
 {
-    LAUNCH("dot", k_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, a, b, ne, rs, scal + slot, op);
+    LAUNCH("dot", k_dot<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, a, b, ne, RedScratch{rs.partial, nullptr}, scal + slot, op);
+    LAUNCH("finish", k_finish, dim3(1), dim3(kRedThreads), 0, stream, static_cast<const double *>(rs.partial), kRedBlocks, scal + slot, op);
     check_launch("dot");
     double v;
     d2h(&v, scal + slot, 8);
@@ -330,10 +331,15 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
         for (; j < m && total < max_it; ++j, ++total)
         {
             apply_op(nullptr, V[j], invdiag, dir, dt, 1, 1, nullptr, wvec);
+            const RedScratch parts{rs.partial, nullptr};
             for (int i = 0; i <= j; ++i)
+            {
                 LAUNCH("mgs", k_mgs<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, wvec, i > 0 ? V[i - 1] : nullptr, V[i], ne,
-                                   hcol, i, 0, rs);
-            LAUNCH("mgs", k_mgs<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, wvec, V[j], nullptr, ne, hcol, j + 1, 1, rs);
+                       hcol, i, 0, parts);
+                LAUNCH("finish", k_finish, dim3(1), dim3(kRedThreads), 0, stream, static_cast<const double *>(rs.partial), kRedBlocks, hcol + i, 0);
+            }
+            LAUNCH("mgs", k_mgs<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, wvec, V[j], nullptr, ne, hcol, j + 1, 1, parts);
+            LAUNCH("finish", k_finish, dim3(1), dim3(kRedThreads), 0, stream, static_cast<const double *>(rs.partial), kRedBlocks, hcol + j + 1, 1);
             LAUNCH("scale", k_scale_div<double>, dim3(nblk(ne)), dim3(256), 0, stream, wvec, hcol + j + 1, ne, V[j + 1]);
             check_launch("arnoldi");
             d2h(hc.data(), hcol, sizeof(double) * (j + 2));
@@ -933,6 +939,27 @@ int mhdg_step_hall(mhdg_ctx *c, double dt_max, int precision, double *dt_used, i
         if (dt_used) *dt_used = dt;
         if (iterations) *iterations = it;
         if (rel_res) *rel_res = rr;
+    })
+}
+int mhdg_state_save(mhdg_ctx *c)
+{
+    MHDG_TRY(c, {
+        const size_t C = c->nc, F = c->nf, ns = size_t(c->P.ns);
+        if (!c->snapshot) c->snapshot = static_cast<double *>(c->alloc((5 * C * ns + F) * 8));
+        for (size_t s = 0; s < ns; ++s)
+            hip_check(hipMemcpyAsync(c->snapshot + 5 * C * s, c->U[s], 5 * C * 8, hipMemcpyDeviceToDevice, c->stream), "copy");
+        hip_check(hipMemcpyAsync(c->snapshot + 5 * C * ns, c->Phi, F * 8, hipMemcpyDeviceToDevice, c->stream), "copy");
+    })
+}
+int mhdg_state_restore(mhdg_ctx *c)
+{
+    MHDG_TRY(c, {
+        if (!c->snapshot) throw std::runtime_error("state_restore: no saved state");
+        const size_t C = c->nc, F = c->nf, ns = size_t(c->P.ns);
+        for (size_t s = 0; s < ns; ++s)
+            hip_check(hipMemcpyAsync(c->U[s], c->snapshot + 5 * C * s, 5 * C * 8, hipMemcpyDeviceToDevice, c->stream), "copy");
+        hip_check(hipMemcpyAsync(c->Phi, c->snapshot + 5 * C * ns, F * 8, hipMemcpyDeviceToDevice, c->stream), "copy");
+        c->have_corr = false;
     })
 }
 int mhdg_prepare(mhdg_ctx *c)

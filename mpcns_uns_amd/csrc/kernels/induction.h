@@ -12,6 +12,7 @@ struct RedScratch
     unsigned *ticket;  // one counter, zero between launches
 };
 
+__global__ void k_finish(const double *partial, int32_t n, double *out, int op);
 template <class T> __global__ void k_dot(const T *a, const T *b, int32_t n, RedScratch rs, double *out, int op);
 template <class T> __global__ void k_mgs(T *w, const T *Vprev, const T *Vi, int32_t n, double *h, int32_t i, int32_t norm, RedScratch rs);
 template <class T> __global__ void k_scale_div(const T *w, const double *h, int32_t n, T *out);

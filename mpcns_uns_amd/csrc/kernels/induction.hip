@@ -1,7 +1,9 @@
 // Implicit Hall/resistive induction operator (ImplicitInduction::apply_F / apply_A,
 // Common/induction/ImplicitInduction.cpp:127-198) as face, cell and edge gathers, plus the GMRES vector
-// kernels.  Reductions are deterministic: a fixed grid, a fixed per-block tree, and the last block to
-// finish (ticket counter) combines the block partials in a fixed tree, so reruns are bit-identical.
+// kernels.  Reductions are deterministic: a fixed grid, a fixed per-block tree, and one block combines the
+// block partials in a fixed tree (k_finish, a second launch), so reruns are bit-identical.  (A last-block
+// ticket in the same launch costs ~90 us per reduction here: the 1024 blocks finish together and their
+// fenced atomics on one counter serialise.)
 #include "induction.h"
 
 namespace mhdg
@@ -28,6 +30,12 @@ __device__ void grid_finish(double part, RedScratch rs, double *out, int op)
 {
     __shared__ double sh[kRedThreads];
     __shared__ int last;
+    if (!rs.ticket)
+    {
+        // partials only; k_finish combines them
+        if (threadIdx.x == 0) rs.partial[blockIdx.x] = part;
+        return;
+    }
     if (threadIdx.x == 0)
     {
         rs.partial[blockIdx.x] = part;
@@ -47,6 +55,16 @@ __device__ void grid_finish(double part, RedScratch rs, double *out, int op)
         *out = op == 1 ? sqrt(v) : v;
         *rs.ticket = 0u;
     }
+}
+
+// out = op(sum of the n block partials) in the fixed tree of grid_finish's last block (one block)
+__global__ void __launch_bounds__(kRedThreads) k_finish(const double *partial, int32_t n, double *out, int op)
+{
+    __shared__ double sh[kRedThreads];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += kRedThreads) v += partial[i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) *out = op == 1 ? sqrt(v) : v;
 }
 
 template <class T>

@@ -19,6 +19,7 @@ struct DevMesh32
 };
 
 constexpr int kDotChunk = 16; // basis vectors per multi-dot pass
+constexpr int kOrthoBlocks = 1024; // grid of the FP32 orthogonalisation kernels (measured best of 512..4096)
 
 __global__ void k_d1_f32(DevMesh d, DevMesh32 s, const float *pv, const float *dte, float *phi);
 __global__ void k_bcell_f32(DevMesh d, DevMesh32 s, const float *phi, float *b);
@@ -44,7 +45,7 @@ struct MixedSolver
     float *phi = nullptr, *b = nullptr, *g = nullptr, *w = nullptr, *dte32 = nullptr, *pv = nullptr;
     std::vector<float *> V;
     float **Vptrs = nullptr;
-    double *h = nullptr, *part = nullptr;
+    double *h = nullptr, *part = nullptr, *part_axpy = nullptr;
     int m = 30;
     bool built = false;
 };

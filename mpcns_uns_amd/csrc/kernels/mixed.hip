@@ -5,7 +5,10 @@
 //                                     classical Gram-Schmidt with FP64 accumulation (one multi-dot pass and
 //                                     one multi-axpy pass per step, re-orthogonalised when the DGKS test
 //                                     asks for it), Hessenberg and Givens in FP64 on the host.
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 
 #include "ctx.h"
@@ -223,6 +226,7 @@ __global__ void __launch_bounds__(256) k_multidot_f32(const float *const *V, int
         const int k = threadIdx.x;
         part[size_t(blockIdx.x) * (kDotChunk + 1) + k] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
     }
+    if (!rs.ticket) return; // partials only: k_multidot_final sums them
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(rs.ticket, 1u) == gridDim.x - 1;
@@ -311,6 +315,134 @@ __global__ void __launch_bounds__(256) k_multiaxpy_f32(const float *const *V, in
     if (threadIdx.x == 0)
     {
         nrm[0] = sqrt(((sh[0] + sh[1]) + sh[2]) + sh[3]);
+        *rs.ticket = 0u;
+    }
+}
+
+// out[i0 + k] (k < nv) and out[63] (with_ww) from the per-block partials of k_multidot_f32 (one block)
+__global__ void __launch_bounds__(256) k_multidot_final(const double *part, int32_t nblocks, int32_t i0, int32_t nv, int32_t with_ww,
+                                                        double *out)
+{
+    __shared__ double sh[4][kDotChunk + 1];
+    double t[kDotChunk + 1];
+#pragma unroll
+    for (int k = 0; k <= kDotChunk; ++k) t[k] = 0.0;
+    for (int32_t bl = threadIdx.x; bl < nblocks; bl += 256)
+#pragma unroll
+        for (int k = 0; k <= kDotChunk; ++k) t[k] += part[size_t(bl) * (kDotChunk + 1) + k];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k <= kDotChunk; ++k)
+    {
+        const double s = wave_sum(t[k]);
+        if (lane == 0) sh[wv][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x <= kDotChunk)
+    {
+        const int k = threadIdx.x;
+        const double s = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
+        if (k < nv) out[i0 + k] = s;
+        if (k == kDotChunk && with_ww) out[63] = s;
+    }
+}
+
+// Classical Gram-Schmidt update fused with the normalisation: with h[0..nv) = <w, V_i> and h[63] = <w, w> from the
+// multi-dot pass, hn = sqrt(<w,w> - sum h_i^2) (Pythagoras, FP64) is the norm of w' = w - sum h_i V_i.  When
+// hn >= dgks |w| the kernel writes V_nv = w' / hn and pv = dir ? 0 : V_nv invdiag directly and h[nv] = hn; else
+// (severe cancellation: a second projection follows) it writes w' into w and h[nv] = |w'| computed from w'.
+// h[62] = 1 in the second case, 0 in the first.  Every block takes the same branch (same h).
+__global__ void __launch_bounds__(256) k_multiaxpy_scale_f32(const float *const *V, int32_t nv, double *h, float *w, int32_t n, double dgks,
+                                                             const float *invdiag, const int8_t *dir, float *vnext, float *pv, RedScratch rs)
+{
+    __shared__ double sh[4];
+    __shared__ int last;
+    __shared__ double hs[64];
+    __shared__ const float *vp[64];
+    if (threadIdx.x < nv) { hs[threadIdx.x] = h[threadIdx.x]; vp[threadIdx.x] = V[threadIdx.x]; }
+    __syncthreads();
+    double ss = 0.0;
+    for (int32_t i = 0; i < nv; ++i) ss += hs[i] * hs[i];
+    const double ww = h[63];
+    const double hn2 = ww - ss;
+    const double hp = hn2 > 0 ? sqrt(hn2) : 0.0;
+    const bool direct = hp >= dgks * sqrt(ww > 0 ? ww : 0.0) && hp > 1e-300;
+    const double inv = direct ? 1.0 / hp : 0.0;
+    double acc = 0.0;
+    const int32_t n4 = n >> 2;
+    float4 *w4 = reinterpret_cast<float4 *>(w);
+    for (int32_t q = blockIdx.x * 256 + threadIdx.x; q < n4; q += gridDim.x * 256)
+    {
+        const float4 a = w4[q];
+        double x = a.x, y = a.y, z = a.z, u = a.w;
+        for (int32_t i = 0; i < nv; ++i)
+        {
+            const float4 b = reinterpret_cast<const float4 *>(vp[i])[q];
+            const double hi = hs[i];
+            x -= hi * b.x; y -= hi * b.y; z -= hi * b.z; u -= hi * b.w;
+        }
+        if (direct)
+        {
+            const float4 v = make_float4(float(x * inv), float(y * inv), float(z * inv), float(u * inv));
+            reinterpret_cast<float4 *>(vnext)[q] = v;
+            const float4 iv = reinterpret_cast<const float4 *>(invdiag)[q];
+            const int32_t e = 4 * q;
+            reinterpret_cast<float4 *>(pv)[q] = make_float4(dir[e] ? 0.0f : v.x * iv.x, dir[e + 1] ? 0.0f : v.y * iv.y,
+                                                            dir[e + 2] ? 0.0f : v.z * iv.z, dir[e + 3] ? 0.0f : v.w * iv.w);
+        }
+        else
+        {
+            const float4 r = make_float4(float(x), float(y), float(z), float(u));
+            w4[q] = r;
+            acc += (double(r.x) * r.x + double(r.y) * r.y) + (double(r.z) * r.z + double(r.w) * r.w);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3))
+    {
+        const int32_t e = 4 * n4 + threadIdx.x;
+        double we = w[e];
+        for (int32_t i = 0; i < nv; ++i) we -= hs[i] * static_cast<double>(vp[i][e]);
+        if (direct)
+        {
+            const float v = static_cast<float>(we * inv);
+            vnext[e] = v;
+            pv[e] = dir[e] ? 0.0f : v * invdiag[e];
+        }
+        else
+        {
+            const float wf = static_cast<float>(we);
+            w[e] = wf;
+            acc += double(wf) * wf;
+        }
+    }
+    if (direct)
+    {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { h[nv] = hp; h[62] = 0.0; }
+        return;
+    }
+    const double s = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        rs.partial[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        __threadfence();
+        last = atomicAdd(rs.ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double t = 0.0;
+    for (uint32_t bl = threadIdx.x; bl < gridDim.x; bl += 256)
+        t += __hip_atomic_load(&rs.partial[bl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double s2 = wave_sum(t);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        h[nv] = sqrt(((sh[0] + sh[1]) + sh[2]) + sh[3]);
+        h[62] = 1.0;
         *rs.ticket = 0u;
     }
 }
@@ -426,7 +558,8 @@ void mhdg_ctx::mixed_build()
     M.Vptrs = static_cast<float **>(alloc(sizeof(float *) * M.V.size()));
     chk(hipMemcpy(M.Vptrs, M.V.data(), sizeof(float *) * M.V.size(), hipMemcpyHostToDevice), "upload");
     M.h = static_cast<double *>(alloc(64 * sizeof(double)));
-    M.part = static_cast<double *>(alloc(size_t(kRedBlocks) * (kDotChunk + 1) * sizeof(double)));
+    M.part = static_cast<double *>(alloc(size_t(kOrthoBlocks) * (kDotChunk + 1) * sizeof(double)));
+    M.part_axpy = static_cast<double *>(alloc(size_t(kOrthoBlocks) * sizeof(double)));
     chk(hipStreamSynchronize(stream), "sync");
     M.built = true;
 }
@@ -463,12 +596,20 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
     std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
     std::vector<double> cs(m), sn(m), g(m + 1), yv(m), hc(64), c2(64);
     const float *const *Vp = M.Vptrs;
+    // MHDG_SOLVE_TRACE=1: one stderr line per refinement cycle (FP64 residual, inner iterations, inner estimate)
+    static const bool trace = std::getenv("MHDG_SOLVE_TRACE") && std::getenv("MHDG_SOLVE_TRACE")[0] == '1';
+    int cycle_start = 0;
+    double est = 0;
     for (int outer = 0; outer < 200; ++outer)
     {
         // FP64 residual of x0 + delta
         LAUNCH("axpy", k_add, dim3(nb(ne)), dim3(256), 0, stream, x0, delta, ne, tmp);
         apply_op(nullptr, tmp, nullptr, dir, dt, 1, 0, bvec, rvec);
         beta = dot(rvec, rvec, 1, 1);
+        if (trace)
+            std::fprintf(stderr, "[mixed] cycle %d: residual %.3e |b|, previous cycle %d iterations, inner estimate %.3e |b|\n", outer,
+                         beta / bnorm, total - cycle_start, est / bnorm);
+        cycle_start = total;
         if (!(beta > rtol * bnorm) || total >= max_it) break;
         LAUNCH("scale", k_to_f32_scaled, dim3(nb(ne)), dim3(256), 0, stream, rvec, scal + 1, M.s.invdiag, dir, ne, M.V[0], M.pv);
         std::fill(g.begin(), g.end(), 0.0);
@@ -489,14 +630,21 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
             if (n_heavy)
                 LAUNCH("edge_apply_f32_heavy", k_edge_apply_f32_heavy, dim3(n_heavy), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall),
                        static_cast<const int32_t *>(heavy), M.phi, M.g, M.V[j], M.pv, dir, dtf, M.w);
-            // classical Gram-Schmidt against V_0..V_j
+            // classical Gram-Schmidt against V_0..V_j; the update also normalises V_{j+1} unless the DGKS test asks
+            // for a second projection
             for (int i0 = 0; i0 <= j; i0 += kDotChunk)
-                LAUNCH("multidot", k_multidot_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, i0, std::min(kDotChunk, j + 1 - i0), M.w, ne,
-                       M.part, i0 == 0 ? 1 : 0, rs, M.h);
-            LAUNCH("multiaxpy", k_multiaxpy_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, j + 1, M.h, M.w, ne, rs, M.h + j + 1);
+            {
+                const int nv = std::min(kDotChunk, j + 1 - i0);
+                LAUNCH("multidot", k_multidot_f32, dim3(kOrthoBlocks), dim3(256), 0, stream, Vp, i0, nv, M.w, ne, M.part, i0 == 0 ? 1 : 0,
+                       RedScratch{nullptr, nullptr}, M.h);
+                LAUNCH("multidot_final", k_multidot_final, dim3(1), dim3(256), 0, stream, static_cast<const double *>(M.part), kOrthoBlocks,
+                       i0, nv, i0 == 0 ? 1 : 0, M.h);
+            }
+            LAUNCH("multiaxpy", k_multiaxpy_scale_f32, dim3(kOrthoBlocks), dim3(256), 0, stream, Vp, j + 1, M.h, M.w, ne, double(kDgks),
+                   M.s.invdiag, dir, M.V[j + 1], M.pv, RedScratch{M.part_axpy, rs.ticket});
             d2h(hc.data(), M.h, sizeof(double) * 64);
             double hn = hc[j + 1];
-            if (hn < kDgks * std::sqrt(std::max(hc[63], 0.0)))
+            if (hc[62] != 0.0)
             {
                 // DGKS: one more projection
                 for (int i0 = 0; i0 <= j; i0 += kDotChunk)
@@ -507,8 +655,8 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
                 for (int i = 0; i <= j; ++i) hc[i] += c2[i];
                 hn = c2[j + 1];
                 chk(hipMemcpyAsync(M.h + j + 1, &hn, 8, hipMemcpyHostToDevice, stream), "H2D");
+                LAUNCH("scale", k_scale_f32, dim3(nb(ne)), dim3(256), 0, stream, M.w, M.h + j + 1, M.s.invdiag, dir, ne, M.V[j + 1], M.pv);
             }
-            LAUNCH("scale", k_scale_f32, dim3(nb(ne)), dim3(256), 0, stream, M.w, M.h + j + 1, M.s.invdiag, dir, ne, M.V[j + 1], M.pv);
             for (int i = 0; i <= j; ++i) H[i][j] = hc[i];
             H[j][j] += 1.0; // Arnoldi ran on A P - I
             H[j + 1][j] = hn;
@@ -522,6 +670,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
             cs[j] = den > 0 ? H[j][j] / den : 1.0; sn[j] = den > 0 ? H[j + 1][j] / den : 0.0;
             H[j][j] = den; H[j + 1][j] = 0.0;
             g[j + 1] = -sn[j] * g[j]; g[j] = cs[j] * g[j];
+            est = std::abs(g[j + 1]);
             if (std::abs(g[j + 1]) <= stop || !(hn > 1e-300)) { ++j; ++total; break; }
         }
         for (int i = j - 1; i >= 0; --i)

@@ -226,6 +226,14 @@ class Device:
         self._ok(lib().mhdg_step_hall(self._h, dt_max, precision, C.byref(dt), C.byref(it), C.byref(rr)))
         return dt.value, it.value, rr.value
 
+    def state_save(self):
+        """Device-side copy of the resident state (every species, Phi)."""
+        self._ok(lib().mhdg_state_save(self._h))
+
+    def state_restore(self):
+        """Back to the saved state, without the implicit solve's warm start."""
+        self._ok(lib().mhdg_state_restore(self._h))
+
     def prepare(self):
         """Solver::prepare on the resident state: derived() then gives the current state's fields."""
         self._ok(lib().mhdg_prepare(self._h))

@@ -70,6 +70,8 @@ struct mhdg_ctx
     bool prof_on = false;
     struct ProfRec { const char *name; hipEvent_t a, b; };
     std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> prof_pool; // events reused across launches (creating two per launch cost ~5% of a step)
+    size_t prof_used = 0;
     std::map<std::string, std::pair<long, double>> prof_acc; // name -> (launches, milliseconds)
     void prof_begin(const char *name);
     void prof_end();

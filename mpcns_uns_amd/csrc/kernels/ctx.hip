@@ -84,9 +84,14 @@ void mhdg_ctx::check_launch(const char *what)
 void mhdg_ctx::prof_begin(const char *name)
 {
     if (!prof_on) return;
-    ProfRec r{name, nullptr, nullptr};
-    hip_check(hipEventCreate(&r.a), "event");
-    hip_check(hipEventCreate(&r.b), "event");
+    while (prof_pool.size() < prof_used + 2)
+    {
+        hipEvent_t ev;
+        hip_check(hipEventCreate(&ev), "event");
+        prof_pool.push_back(ev);
+    }
+    ProfRec r{name, prof_pool[prof_used], prof_pool[prof_used + 1]};
+    prof_used += 2;
     hip_check(hipEventRecord(r.a, stream), "event");
     prof_pending.push_back(r);
 }
@@ -106,17 +111,16 @@ void mhdg_ctx::prof_flush()
         auto &acc = prof_acc[r.name];
         acc.first += 1;
         acc.second += ms;
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
     }
     prof_pending.clear();
+    prof_used = 0;
 }
 
 mhdg_ctx::~mhdg_ctx()
 {
     delete mixed;
     for (double *p : aw.ck) (void)hipFree(p);
-    for (auto &r : prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (hipEvent_t ev : prof_pool) (void)hipEventDestroy(ev);
     for (void *p : owned) (void)hipFree(p);
     for (auto &kv : staging) if (kv.second.first) (void)hipFree(kv.second.first);
     if (host_pin) (void)hipHostFree(host_pin);

@@ -22,8 +22,9 @@ constexpr int kDotChunk = 16; // basis vectors per multi-dot pass
 constexpr int kOrthoBlocks = 1024; // grid of the FP32 orthogonalisation kernels (measured best of 512..4096)
 
 __global__ void k_d1_f32(DevMesh d, DevMesh32 s, const float *pv, const float *dte, float *phi);
-__global__ void k_bcell_f32(DevMesh d, DevMesh32 s, const float *phi, float *b);
-__global__ void k_hall_g_f32(DevMesh d, DevMesh32 s, const float *b, float *g);
+constexpr int kCellLanes = 4; // lanes per cell of the FP32 cell kernels
+template <int G> __global__ void k_bcell_f32_g(DevMesh d, DevMesh32 s, const float *phi, float *b);
+template <int G> __global__ void k_hall_g_f32_g(DevMesh d, DevMesh32 s, const float *b, float *g);
 __global__ void k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi, const float *g, const float *v,
                                  const float *pv, const int8_t *dir, float dt, float *out);
 __global__ void k_edge_apply_f32_heavy(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const int32_t *heavy, const float *phi,

@@ -54,54 +54,6 @@ __global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, DevMesh32 s, const fl
     phi[f] = sum;
 }
 
-// b_c = sum_k recon_k Phi'_{f_k}
-__global__ void __launch_bounds__(256) k_bcell_f32(DevMesh d, DevMesh32 s, const float *phi, float *b)
-{
-    const int32_t c = gid();
-    if (c >= d.nc) return;
-    int32_t k0, k1;
-    rows(s.cdeg, d.c2f_ptr, c, k0, k1);
-    float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for (int32_t k = k0; k < k1; ++k)
-    {
-        const float v = phi[pid(d.c2f[k])];
-        bx += s.recon[3 * size_t(k)] * v; by += s.recon[3 * size_t(k) + 1] * v; bz += s.recon[3 * size_t(k) + 2] * v;
-    }
-    b[3 * size_t(c)] = bx; b[3 * size_t(c) + 1] = by; b[3 * size_t(c) + 2] = bz;
-}
-
-// g_c = alpha_c J_c x B0_c with J_c = sum_k (s A_f / V_c)_k x b_f, b_f the face average with the neighbour
-// (per-(cell, face) table: coalesced, no face-indexed gathers but the neighbour's b)
-__global__ void __launch_bounds__(256) k_hall_g_f32(DevMesh d, DevMesh32 s, const float *b, float *g)
-{
-    const int32_t c = gid();
-    if (c >= d.nc) return;
-    const float a = s.alpha[c];
-    F3 out{0.0f, 0.0f, 0.0f};
-    if (a != 0.0f)
-    {
-        int32_t k0, k1;
-        rows(s.cdeg, d.c2f_ptr, c, k0, k1);
-        const F3 bc = f3(b, c);
-        F3 J{0.0f, 0.0f, 0.0f};
-        for (int32_t k = k0; k < k1; ++k)
-        {
-            const int32_t nb = s.nbr[k];
-            F3 bf = bc;
-            if (nb >= 0)
-            {
-                const F3 bo = f3(b, nb);
-                bf = {0.5f * (bc.x + bo.x), 0.5f * (bc.y + bo.y), 0.5f * (bc.z + bo.z)};
-            }
-            const F3 c3 = fcross(f3(s.aV, k), bf);
-            J.x += c3.x; J.y += c3.y; J.z += c3.z;
-        }
-        const F3 q = fcross(J, f3(s.B0, c));
-        out = {a * q.x, a * q.y, a * q.z};
-    }
-    g[3 * size_t(c)] = out.x; g[3 * size_t(c) + 1] = out.y; g[3 * size_t(c) + 2] = out.z;
-}
-
 // out = dir ? 0 : (A P - I) v = (pv - v) + dt F(Phi') with F = m_e sum_f s star2_f Phi'_f + sum_c w_ec g_c . t_e
 // (the shifted operator: same Krylov space as A P, and the O(1) identity part never enters the FP32 vector)
 __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi,
@@ -127,6 +79,75 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, 
     }
     out[e] = (pv[e] - v[e]) + dt * (s.m_e[e] * p0 + p1);
 }
+
+// Cell kernels with G lanes per cell: the lanes read the cell's (cell, face) table entries side by side (coalesced
+// across the wavefront) and combine their partial sums with shuffles (G = 4 measured 1.2-1.3x faster than one
+// thread per cell on the 8M shell; one thread per edge stays faster for the edge kernel).
+// b_c = sum_k recon_k Phi'_{f_k}
+template <int G> __device__ __forceinline__ float group_sum(float v)
+{
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+template <int G> __global__ void __launch_bounds__(256) k_bcell_f32_g(DevMesh d, DevMesh32 s, const float *phi, float *b)
+{
+    const int32_t t = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    const int32_t c = t / G, lane = t % G;
+    const bool ok = c < d.nc;
+    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    if (ok)
+    {
+        int32_t k0, k1;
+        rows(s.cdeg, d.c2f_ptr, c, k0, k1);
+        for (int32_t k = k0 + lane; k < k1; k += G)
+        {
+            const float v = phi[pid(d.c2f[k])];
+            bx += s.recon[3 * size_t(k)] * v; by += s.recon[3 * size_t(k) + 1] * v; bz += s.recon[3 * size_t(k) + 2] * v;
+        }
+    }
+    bx = group_sum<G>(bx); by = group_sum<G>(by); bz = group_sum<G>(bz);
+    if (ok && lane == 0) { b[3 * size_t(c)] = bx; b[3 * size_t(c) + 1] = by; b[3 * size_t(c) + 2] = bz; }
+}
+// g_c = alpha_c J_c x B0_c with J_c = sum_k (s A_f / V_c)_k x b_f, b_f the face average with the neighbour
+template <int G> __global__ void __launch_bounds__(256) k_hall_g_f32_g(DevMesh d, DevMesh32 s, const float *b, float *g)
+{
+    const int32_t t = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    const int32_t c = t / G, lane = t % G;
+    const bool ok = c < d.nc;
+    const float a = ok ? s.alpha[c] : 0.0f;
+    F3 J{0.0f, 0.0f, 0.0f};
+    if (ok && a != 0.0f)
+    {
+        int32_t k0, k1;
+        rows(s.cdeg, d.c2f_ptr, c, k0, k1);
+        const F3 bc = f3(b, c);
+        for (int32_t k = k0 + lane; k < k1; k += G)
+        {
+            const int32_t nb = s.nbr[k];
+            F3 bf = bc;
+            if (nb >= 0)
+            {
+                const F3 bo = f3(b, nb);
+                bf = {0.5f * (bc.x + bo.x), 0.5f * (bc.y + bo.y), 0.5f * (bc.z + bo.z)};
+            }
+            const F3 c3 = fcross(f3(s.aV, k), bf);
+            J.x += c3.x; J.y += c3.y; J.z += c3.z;
+        }
+    }
+    J.x = group_sum<G>(J.x); J.y = group_sum<G>(J.y); J.z = group_sum<G>(J.z);
+    if (ok && lane == 0)
+    {
+        F3 out{0.0f, 0.0f, 0.0f};
+        if (a != 0.0f)
+        {
+            const F3 q = fcross(J, f3(s.B0, c));
+            out = {a * q.x, a * q.y, a * q.z};
+        }
+        g[3 * size_t(c)] = out.x; g[3 * size_t(c) + 1] = out.y; g[3 * size_t(c) + 2] = out.z;
+    }
+}
+template __global__ void k_bcell_f32_g<4>(DevMesh, DevMesh32, const float *, float *);
+template __global__ void k_hall_g_f32_g<4>(DevMesh, DevMesh32, const float *, float *);
 
 // The same for the high-valence edges, one block per edge (tree sums: order-free in FP32)
 __global__ void __launch_bounds__(256) k_edge_apply_f32_heavy(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const int32_t *heavy,
@@ -622,11 +643,11 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
             LAUNCH("d1_f32", k_d1_f32, dim3(nb(nf)), dim3(256), 0, stream, d, M.s, M.pv, dte32, M.phi);
             if (op_hall)
             {
-                LAUNCH("bcell_f32", k_bcell_f32, dim3(nb(nc)), dim3(256), 0, stream, d, M.s, M.phi, M.b);
-                LAUNCH("hall_g_f32", k_hall_g_f32, dim3(nb(nc)), dim3(256), 0, stream, d, M.s, M.b, M.g);
+                LAUNCH("bcell_f32", k_bcell_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.phi, M.b);
+                LAUNCH("hall_g_f32", k_hall_g_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.b, M.g);
             }
-            LAUNCH("edge_apply_f32", k_edge_apply_f32, dim3(nb(ne)), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall), M.phi, M.g, M.V[j],
-                   M.pv, dir, dtf, M.w);
+            LAUNCH("edge_apply_f32", k_edge_apply_f32, dim3(nb(ne)), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall), M.phi,
+                       M.g, M.V[j], M.pv, dir, dtf, M.w);
             if (n_heavy)
                 LAUNCH("edge_apply_f32_heavy", k_edge_apply_f32_heavy, dim3(n_heavy), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall),
                        static_cast<const int32_t *>(heavy), M.phi, M.g, M.V[j], M.pv, dir, dtf, M.w);

@@ -165,24 +165,36 @@ def main():
     dt_max = 1e9
 
     def measure(prec, profile, restore=True):
+        """Warmup steps with every kernel timed (the per-kernel breakdown), then the timed steps with HIP events
+        around the dominant kernel only (its launch durations for the roofline)."""
         workloads.load(dev, w)
         dev.state_save()
+        if profile:
+            dev.profile_only(None)
+            dev.profile(None)
+            dev.profile(True)
         run_steps(dev, args.warmup, prec, dt_max, restore)
         dev.sync()
+        breakdown, dom = None, None
         if profile:
+            dev.profile(False)
+            breakdown = dev.profile_report()
+            dom = max(breakdown, key=lambda k: breakdown[k][1] if kernel_bytes(mesh, k, ns=ns) else -1)
             dev.profile(None)
+            dev.profile_only(dom)
             dev.profile(True)
         t = time.perf_counter()
         its, res, dts = run_steps(dev, args.steps, prec, dt_max, restore)
         dev.sync()
         el = time.perf_counter() - t
-        prof = None
+        timed = None
         if profile:
             dev.profile(False)
-            prof = dev.profile_report()
-        return el, its, res, prof, dts
+            timed = dev.profile_report()
+            dev.profile_only(None)
+        return el, its, res, (breakdown, dom, timed), dts
 
-    # main measurement: profiled timed region (events are recorded on the context stream)
+    # main measurement; the timed region carries events around the dominant kernel's launches only
     el, its, res, prof, _ = measure(precision, True)
     value = mesh.n_cells * args.steps / el
     print(f"# {args.precision}: {el / args.steps * 1e3:.2f} ms/step, GMRES it/step {np.mean(its):.1f}, max res {max(res):.2e}",
@@ -194,15 +206,15 @@ def main():
                     gmres_iterations_per_step=float(np.mean(its64)), max_rel_residual=float(max(res64)))
         print(f"# fp64: {fp64['ms_per_step']:.2f} ms/step, GMRES it/step {np.mean(its64):.1f}", file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel (largest total time in the timed region)
+    # roofline of the dominant kernel (largest total time in the warmup breakdown), timed in the timed region
     roof = None
     kernels = {}
-    if prof:
-        total_ms = sum(v[1] for v in prof.values())
-        for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+    breakdown, dom, timed = prof
+    if breakdown and dom in (timed or {}):
+        total_ms = sum(v[1] for v in breakdown.values())
+        for k, (n, ms) in sorted(breakdown.items(), key=lambda kv: -kv[1][1]):
             kernels[k] = dict(launches=n, ms_total=round(ms, 4), share=round(ms / total_ms, 4))
-        dom = max(prof, key=lambda k: prof[k][1] if kernel_bytes(mesh, k, ns=ns) else -1)
-        n, ms = prof[dom]
+        n, ms = timed[dom]
         byt = kernel_bytes(mesh, dom, ns=ns)
         avg_s = ms / n / 1e3
         achieved = byt / avg_s
@@ -240,7 +252,8 @@ def main():
         "fp64_step": fp64,
         "vs_fp64": round(value / fp64["value"], 3) if fp64 else None,
         "roofline": roof,
-        "kernels": kernels,
+        "kernels": kernels,  # per-kernel totals over the warmup steps
+        "kernels_over_steps": args.warmup,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

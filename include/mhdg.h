@@ -231,6 +231,9 @@ int mhdg_sync(mhdg_ctx *ctx);
  * name: "<name> <launches> <milliseconds>". */
 int mhdg_profile(mhdg_ctx *ctx, int enable);
 int mhdg_profile_report(mhdg_ctx *ctx, char *buf, int buflen);
+/* Restricts the timing to the kernels of one name (NULL or "": every kernel), so that a timed region carries
+ * events around the kernel of interest only. */
+int mhdg_profile_only(mhdg_ctx *ctx, const char *name);
 
 #ifdef __cplusplus
 }

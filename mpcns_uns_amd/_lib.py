@@ -113,7 +113,7 @@ def lib():
             "mhdg_set_state": [P, P, P, P], "mhdg_get_state": [P, P, P, P],
             "mhdg_set_background": [P, P, P], "mhdg_step": [P, D, C.c_int, P, P, P],
             "mhdg_get_derived": [P, C.c_int, P], "mhdg_prepare": [P], "mhdg_state_save": [P], "mhdg_state_restore": [P], "mhdg_sync": [P], "mhdg_device_info": [C.c_char_p, C.c_int],
-            "mhdg_profile": [P, C.c_int], "mhdg_profile_report": [P, C.c_char_p, C.c_int],
+            "mhdg_profile": [P, C.c_int], "mhdg_profile_only": [P, C.c_char_p], "mhdg_profile_report": [P, C.c_char_p, C.c_int],
         }.items():
             getattr(L, name).argtypes = args
         _lib = L

@@ -72,6 +72,8 @@ struct mhdg_ctx
     std::vector<ProfRec> prof_pending;
     std::vector<hipEvent_t> prof_pool; // events reused across launches (creating two per launch cost ~5% of a step)
     size_t prof_used = 0;
+    std::string prof_only; // when set, only kernels of this name are timed
+    bool prof_armed = false; // prof_begin recorded the start of the current launch
     std::map<std::string, std::pair<long, double>> prof_acc; // name -> (launches, milliseconds)
     void prof_begin(const char *name);
     void prof_end();

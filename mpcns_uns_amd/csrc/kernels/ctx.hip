@@ -83,7 +83,8 @@ void mhdg_ctx::check_launch(const char *what)
 
 void mhdg_ctx::prof_begin(const char *name)
 {
-    if (!prof_on) return;
+    prof_armed = false;
+    if (!prof_on || (!prof_only.empty() && prof_only != name)) return;
     while (prof_pool.size() < prof_used + 2)
     {
         hipEvent_t ev;
@@ -94,10 +95,12 @@ void mhdg_ctx::prof_begin(const char *name)
     prof_used += 2;
     hip_check(hipEventRecord(r.a, stream), "event");
     prof_pending.push_back(r);
+    prof_armed = true;
 }
 void mhdg_ctx::prof_end()
 {
-    if (!prof_on) return;
+    if (!prof_armed) return;
+    prof_armed = false;
     hip_check(hipEventRecord(prof_pending.back().b, stream), "event");
 }
 void mhdg_ctx::prof_flush()
@@ -505,6 +508,13 @@ int mhdg_profile(mhdg_ctx *c, int enable)
         c->prof_flush();
         if (enable < 0) c->prof_acc.clear();
         else c->prof_on = enable != 0;
+    })
+}
+int mhdg_profile_only(mhdg_ctx *c, const char *name)
+{
+    MHDG_TRY(c, {
+        c->prof_flush();
+        c->prof_only = name ? name : "";
     })
 }
 int mhdg_profile_report(mhdg_ctx *c, char *buf, int len)

@@ -251,6 +251,10 @@ class Device:
         """Per-kernel HIP-event timing on the context stream (True on, False off, None clear)."""
         self._ok(lib().mhdg_profile(self._h, -1 if enable is None else int(bool(enable))))
 
+    def profile_only(self, name=None):
+        """Time only the kernels called `name` (None: every kernel)."""
+        self._ok(lib().mhdg_profile_only(self._h, None if name is None else name.encode()))
+
     def profile_report(self):
         """{kernel name: (launches, total milliseconds)} accumulated while profiling was on."""
         buf = C.create_string_buffer(1 << 16)

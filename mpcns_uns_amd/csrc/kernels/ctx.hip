@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -415,7 +416,16 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
         for (int s = 0; s < P.ns; ++s) hip_check(hipMemsetAsync(grad[s], 0, 15 * size_t(nc) * 8, stream), "memset");
         LAUNCH("grad_current", k_grad_current<double>, dim3(nb), dim3(256), 0, stream, d, P, 0, 0, Ws, none, grads, 1, Bind, J);
     }
-    LAUNCH("face", k_face<double>, dim3(nblk(nf)), dim3(256), 0, stream, d, P, P.ns, 0, Ws, gradsC, none, FA, uplus, Btot, Bind, Phi, PhiAdd, Ef);
+    if (P.ns > 2)
+    {
+        // many species: one thread per (face, species) for the fluxes, then the face EMF candidates alone
+        const int per = 256 / P.ns;
+        LAUNCH("face", k_face_species<double>, dim3(nblk(nf, per)), dim3(256), 0, stream, d, P, P.ns, Ws, gradsC, FA);
+        LAUNCH("face_emf", k_face<double>, dim3(nblk(nf)), dim3(256), 0, stream, d, P, 0, 0, Ws, gradsC, none, static_cast<double *>(nullptr),
+               uplus, Btot, Bind, Phi, PhiAdd, Ef);
+    }
+    else
+        LAUNCH("face", k_face<double>, dim3(nblk(nf)), dim3(256), 0, stream, d, P, P.ns, 0, Ws, gradsC, none, FA, uplus, Btot, Bind, Phi, PhiAdd, Ef);
     check_launch("explicit stage");
     hip_check(hipEventSynchronize(ev), "event sync");
     (void)hipEventDestroy(ev);

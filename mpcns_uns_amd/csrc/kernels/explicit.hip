@@ -66,7 +66,8 @@ template <class R> __global__ void __launch_bounds__(256) k_dt_final(const R *pa
 }
 
 // Stencil::limited_gradients (Common/fluid/Reconstruction.cpp:80-142) for `ns` species and cell_current
-// (Common/induction/Current.cpp:9-31).  ghost: optional per-face table of 5 primitives per species.
+// (Common/induction/Current.cpp:9-31, by the blocks of grid row 0).  ghost: optional per-face table of 5
+// primitives per species.
 // Each block stages its cells' 15-value gradient rows in LDS and writes them as one contiguous range (the
 // per-thread 120-byte rows written directly left partially written cache lines behind: 2.2x write traffic).
 template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads,
@@ -78,7 +79,7 @@ template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh
     const int32_t n_here = min(int32_t(blockDim.x), d.nc - c_first);
     const bool act = c < d.nc;
     const int32_t k0 = act ? d.c2f_ptr[c] : 0, k1 = act ? d.c2f_ptr[c + 1] : 0;
-    if (act && J)
+    if (act && J && blockIdx.y == 0)
     {
         V3T<R> sum;
         const V3T<R> bc = L3(Bind, c);
@@ -94,7 +95,9 @@ template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh
         st3(J, c, V3T<R>(sum.x * invV, sum.y * invV, sum.z * invV));
     }
     const V3T<R> xc = act ? L3(d.ccen, c) : V3T<R>();
-    for (int32_t s = 0; s < ns; ++s)
+    // species s of the block: blockIdx.y, blockIdx.y + gridDim.y, ... (one grid row, every species, measured
+    // faster than one row per species)
+    for (int32_t s = int32_t(blockIdx.y); s < ns; s += int32_t(gridDim.y))
     {
         const int32_t sp = s0 + s;
         R *out = stage + 15 * threadIdx.x;
@@ -183,6 +186,81 @@ template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh
 }
 
 
+namespace
+{
+// Rusanov flux F*|A| of species slot s (species sp) through non-skipped face f with at least one fluid cell
+// (Common/fluid/FluxRHS.cpp:43-70)
+template <class R>
+__device__ __forceinline__ void face_species_flux(const DevMesh &d, const Params &P, int32_t f, int32_t l, int32_t r, int8_t kind, const R &A,
+                                                  const V3T<R> &n, int32_t s, int32_t sp, const SpCT<R> &Ws, const SpCT<R> &grads,
+                                                  const SpCT<R> &ghosts, R *out)
+{
+    const R nv[3] = {n.x, n.y, n.z};
+    const V3T<R> xf = L3(d.fcen, f);
+    const R *W = Ws.p[s], *G = grads.p[s];
+    PrimT<R> L, Rt;
+    if (l >= 0 && r >= 0 && kind == kInterior)
+    {
+        L = face_state(W, G, l, xf - L3(d.ccen, l));
+        Rt = face_state(W, G, r, xf - L3(d.ccen, r));
+    }
+    else
+    {
+        // the single fluid cell; f2c order puts the lower cell id first (FluxRHS.cpp:57-62)
+        int32_t c;
+        R sg;
+        if (l >= 0 && r >= 0)
+        {
+            const int32_t first = l < r ? l : r, second = l < r ? r : l;
+            c = d.mat[first] == 1 ? second : first;
+        }
+        else c = l >= 0 ? l : r;
+        sg = c == l ? 1.0 : -1.0;
+        const PrimT<R> in = face_state(W, G, c, xf - L3(d.ccen, c));
+        PrimT<R> gh;
+        if (ghosts.p[s])
+        {
+            const R *q = ghosts.p[s] + 5 * size_t(f);
+            gh = PrimT<R>{q[0], q[1], q[2], q[3], q[4]};
+        }
+        else
+        {
+            R fsv[5];
+            for (int m = 0; m < 5; ++m) fsv[m] = seeded<R>(P.fs[sp][m], P.dfs[sp][m]);
+            gh = ghost_state(kind, sp, in, sg * n, fsv);
+        }
+        if (sg > 0) { L = in; Rt = gh; } else { Rt = in; L = gh; }
+    }
+    R F[5];
+    rusanov(L, Rt, nv, P.gamma, F);
+    for (int m = 0; m < 5; ++m) out[m] = F[m] * A;
+}
+} // namespace
+
+// Face fluxes of many species, one thread per (face, species) with the species fastest: a block's threads
+// write its faces' 5*ns flux values as one contiguous range (FA layout of k_face).  The face EMF candidate
+// is left to k_face.
+template <class R> __global__ void __launch_bounds__(256) k_face_species(DevMesh d, Params P, int32_t ns, SpCT<R> Ws, SpCT<R> grads, R *FA)
+{
+    const int32_t per = int32_t(blockDim.x) / ns;
+    const int32_t t = int32_t(threadIdx.x);
+    const int32_t f = int32_t(blockIdx.x) * per + t / ns, s = t % ns;
+    if (t >= per * ns || f >= d.nf) return;
+    const int32_t l = d.fL[f], r = d.fR[f];
+    const int8_t kind = d.kind[f];
+    const bool any = (l >= 0 && d.mat[l] == 0) || (r >= 0 && d.mat[r] == 0);
+    R *out = FA + size_t(5 * ns) * f + 5 * s;
+    if (kind == kSkip || !any)
+    {
+        for (int m = 0; m < 5; ++m) out[m] = 0.0;
+        return;
+    }
+    const R A = d.famag[f];
+    const V3T<R> n = (1.0 / A) * L3(d.farea, f);
+    SpCT<R> none{};
+    face_species_flux(d, P, f, l, r, kind, A, n, s, s, Ws, grads, none, out);
+}
+
 // Face kernel: fluid_flux_rhs's face body (Common/fluid/FluxRHS.cpp:36-73) for `ns` species, storing
 // F*|A| per face and species (FA, 5*ns per face, zero for skipped faces), and optionally the ideal EMF
 // face candidate E_f (Common/induction/IdealEMF.cpp:32-55, 3 per face).
@@ -206,51 +284,7 @@ template <class R> __global__ void __launch_bounds__(256) k_face(DevMesh d, Para
         if (kind == kSkip || !any)
             for (int q = 0; q < 5 * ns; ++q) out[q] = 0.0;
         else
-        {
-            const R nv[3] = {n.x, n.y, n.z};
-            const V3T<R> xf = L3(d.fcen, f);
-            for (int32_t s = 0; s < ns; ++s)
-            {
-                const int32_t sp = s0 + s;
-                const R *W = Ws.p[s], *G = grads.p[s];
-                PrimT<R> L, Rt;
-                if (l >= 0 && r >= 0 && kind == kInterior)
-                {
-                    L = face_state(W, G, l, xf - L3(d.ccen, l));
-                    Rt = face_state(W, G, r, xf - L3(d.ccen, r));
-                }
-                else
-                {
-                    // the single fluid cell; f2c order puts the lower cell id first (FluxRHS.cpp:57-62)
-                    int32_t c;
-                    R sg;
-                    if (l >= 0 && r >= 0)
-                    {
-                        const int32_t first = l < r ? l : r, second = l < r ? r : l;
-                        c = d.mat[first] == 1 ? second : first;
-                    }
-                    else c = l >= 0 ? l : r;
-                    sg = c == l ? 1.0 : -1.0;
-                    const PrimT<R> in = face_state(W, G, c, xf - L3(d.ccen, c));
-                    PrimT<R> gh;
-                    if (ghosts.p[s])
-                    {
-                        const R *q = ghosts.p[s] + 5 * size_t(f);
-                        gh = PrimT<R>{q[0], q[1], q[2], q[3], q[4]};
-                    }
-                    else
-                    {
-                        R fsv[5];
-                        for (int m = 0; m < 5; ++m) fsv[m] = seeded<R>(P.fs[sp][m], P.dfs[sp][m]);
-                        gh = ghost_state(kind, sp, in, sg * n, fsv);
-                    }
-                    if (sg > 0) { L = in; Rt = gh; } else { Rt = in; L = gh; }
-                }
-                R F[5];
-                rusanov(L, Rt, nv, P.gamma, F);
-                for (int m = 0; m < 5; ++m) out[5 * s + m] = F[m] * A;
-            }
-        }
+            for (int32_t s = 0; s < ns; ++s) face_species_flux(d, P, f, l, r, kind, A, n, s, s0 + s, Ws, grads, ghosts, out + 5 * s);
     }
     if (staged)
     {
@@ -295,6 +329,8 @@ template <class R> __global__ void __launch_bounds__(256) k_cell_update(DevMesh 
     const bool fluid = d.mat[c] == 0;
     const R dt = dt_dev ? dt_dev[0] : dt_val;
     const R V = d.cvol[c];
+    // n_e of the cell's ions, once for every species' source
+    const R ne_c = (sources && fluid) ? electron_density(P, Wall.p, c) : R(0.0);
     for (int32_t s = 0; s < ns; ++s)
     {
         const int32_t sp = s0 + s;
@@ -315,7 +351,7 @@ template <class R> __global__ void __launch_bounds__(256) k_cell_update(DevMesh 
             {
                 // mercury::add_sources for ion sp: w_s = n_s / n_e,eff over all P.ns ions (MercurySources.cpp:20-63)
                 const R *W = Wall.p[sp];
-                const R ne = electron_density(P, Wall.p, c);
+                const R ne = ne_c;
                 const R ws = ion_density(P, sp, W[5 * size_t(c)]) / sqrt(ne * ne + P.ne_floor * P.ne_floor);
                 const R rho = mx(W[5 * size_t(c)], 0.0);
                 const V3T<R> u(W[5 * size_t(c) + 1], W[5 * size_t(c) + 2], W[5 * size_t(c) + 3]);
@@ -654,6 +690,7 @@ template __global__ void k_shell_energy<Dual>(DevMesh, const Dual *, const Dual 
 template __global__ void k_prepare<double>(DevMesh d, Params P, SpCT<double> Us, const double *Phi, const double *Badd, SpWT<double> Ws, double *Bind, double *Btot, double *uplus, double *alpha, double *dt_part);
 template __global__ void k_dt_final<double>(const double *part, int32_t n, double dt_max, double *out);
 template __global__ void k_grad_current<double>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> ghosts, SpWT<double> grads, int32_t limit, const double *Bind, double *J);
+template __global__ void k_face_species<double>(DevMesh d, Params P, int32_t ns, SpCT<double> Ws, SpCT<double> grads, double *FA);
 template __global__ void k_face<double>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> grads, SpCT<double> ghosts, double *FA, const double *uplus, const double *Btot, const double *Bind, const double *Phi, const double *PhiAdd, double *Ef);
 template __global__ void k_cell_update<double>(DevMesh d, Params P, int32_t ns, int32_t s0, const double *FA, SpCT<double> rhs_in, SpCT<double> Wall, const double *Btot, const double *J, const double *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<double> Us, SpWT<double> rhs_out);
 template __global__ void k_edge_emf<double>(DevMesh d, Params P, const double *Ef, const double *u_cell, const double *phi_ind, double cuct, int32_t vmax, const double *E_in, double *E_out, int32_t apply_bc);

@@ -54,6 +54,7 @@ template <class R> __device__ R cell_dt(const DevMesh &d, const Params &P, int32
     R rho_sum = Ws.p[0][5 * row];
     for (int s = 1; s < P.ns; ++s) rho_sum = rho_sum + Ws.p[s][5 * row];
     const R rho_tot = mx(rho_sum, P.rho_floor_dt);
+    const R vA2 = P.inver_MA2 * B2 / rho_tot; // the same for every face and species
     for (int s = 0; s < P.ns; ++s)
     {
         const R *w = Ws.p[s] + 5 * row;
@@ -64,7 +65,6 @@ template <class R> __device__ R cell_dt(const DevMesh &d, const Params &P, int32
             const int32_t f = pid(d.c2f[k]);
             const R A = d.famag[f];
             const V3T<R> n = (1.0 / A) * L3(d.farea, f);
-            const R vA2 = P.inver_MA2 * B2 / rho_tot;
             const R vAn2 = P.inver_MA2 * dot(b, n) * dot(b, n) / rho_tot;
             const R term = cs2 + vA2;
             const R cf = sqrt(0.5 * (term + sqrt(mx(0.0, term * term - 4 * cs2 * vAn2))));

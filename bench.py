@@ -24,6 +24,11 @@ sys.path.insert(0, HERE)
 
 PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 
+# HBM bytes per launch moved by a kernel on a workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by
+# the same counters on a 1 GiB device copy (x2.00 and x1.00); the command and the per-kernel table are in
+# profiles/hall_step_shell_8m_hbm_counters.md.  Counter runs cannot share a process with this measurement.
+PMC_TRAFFIC = {("shell_8m_mixed", "edge_apply_f32"): 2213052089}
+
 
 def kernel_bytes(mesh, name, precision_bytes=8, ns=2):
     """Compulsory HBM bytes of one launch of a kernel: every array it reads or writes counted once (ns ion species)."""
@@ -55,7 +60,8 @@ def kernel_bytes(mesh, name, precision_bytes=8, ns=2):
         "d1_f32": 4 * (F + 1) + 4 * nf2e + 4 * E + 4 * E + E + 4 * F,
         "bcell_f32": 4 * (C + 1) + 4 * nc2f + 12 * nc2f + 4 * F + 12 * C,
         "hall_g_f32": 4 * C + 12 * C + 4 * (C + 1) + 4 * nc2f + 8 * F + 12 * F + 4 * C + 12 * C + 12 * C,
-        "edge_apply_f32": E + 4 * (E + 1) + 4 * nf2e + 4 * F + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 4,
+        # dir, e2f ptr, (face, signed star2) pairs, phi, edge vec, ec ptr, (cell, weight) pairs, g, m_e, pv, v, out
+        "edge_apply_f32": E + 4 * (E + 1) + 8 * nf2e + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 4,
     }
     return table.get(name)
 
@@ -182,10 +188,11 @@ def main():
         if profile:
             dev.profile(False)
             breakdown = dev.profile_report()
-            dom = max(breakdown, key=lambda k: breakdown[k][1] if kernel_bytes(mesh, k, ns=ns) else -1)
             dev.profile(None)
-            dev.profile_only(dom)
-            dev.profile(True)
+            if breakdown:
+                dom = max(breakdown, key=lambda k: breakdown[k][1] if kernel_bytes(mesh, k, ns=ns) else -1)
+                dev.profile_only(dom)
+            dev.profile(True)  # without profiled warmup steps: every kernel in the timed region
         t = time.perf_counter()
         its, res, dts = run_steps(dev, args.steps, prec, dt_max, restore)
         dev.sync()
@@ -195,6 +202,9 @@ def main():
             dev.profile(False)
             timed = dev.profile_report()
             dev.profile_only(None)
+            if not breakdown:
+                breakdown = timed
+                dom = max(timed, key=lambda k: timed[k][1] if kernel_bytes(mesh, k, ns=ns) else -1)
         return el, its, res, (breakdown, dom, timed), dts
 
     # main measurement; the timed region carries events around the dominant kernel's launches only
@@ -221,8 +231,9 @@ def main():
         byt = kernel_bytes(mesh, dom, ns=ns)
         avg_s = ms / n / 1e3
         achieved = byt / avg_s
+        traffic = PMC_TRAFFIC.get((args.workload, dom)) if not args.net else None
         roof = dict(kernel=dom, bound="hbm", achieved=round(achieved / 1e9, 1), peak=PEAK_HBM / 1e9, unit="GB/s",
-                    frac=round(achieved / PEAK_HBM, 4), traffic=None, bytes_per_launch=int(byt),
+                    frac=round(achieved / PEAK_HBM, 4), traffic=traffic, bytes_per_launch=int(byt),
                     avg_launch_us=round(avg_s * 1e6, 2), launches=n)
 
     # the run's own consecutive steps after the warmup (no restore): dt collapses as the polar field grows

@@ -15,6 +15,8 @@ struct DevMesh32
     float *alpha, *B0;
     const float *aV;          // per (cell, face) entry: s A_f / V_c (3)
     const int32_t *nbr;       // per (cell, face) entry: the cell across the face, -1 on the boundary
+    const int2 *ecp;          // per edge-cell entry: (cell, bits of the FP32 weight), one 8-byte load
+    const int2 *e2fp;         // per edge-face entry: (face, bits of the FP32 signed star2 of the face)
     int32_t cdeg, fdeg;       // faces per cell / edges per face when uniform (tets: 4, 3), else 0 (ptr arrays)
 };
 

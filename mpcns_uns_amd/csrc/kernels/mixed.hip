@@ -69,13 +69,17 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, 
     if (resistive)
         for (int32_t k = f0; k < f1; ++k)
         {
-            const int32_t q = d.e2f[k], f = pid(q);
-            p0 += fsg(q) * (s.star2[f] * phi[f]);
+            const int2 q = s.e2fp[k];
+            p0 += __int_as_float(q.y) * phi[q.x];
         }
     if (hall)
     {
         const F3 t = f3(s.evec, e);
-        for (int32_t k = d.ec_ptr[e]; k < d.ec_ptr[e + 1]; ++k) p1 += s.ecw[k] * fdot(f3(g, d.ec_cell[k]), t);
+        for (int32_t k = d.ec_ptr[e]; k < d.ec_ptr[e + 1]; ++k)
+        {
+            const int2 q = s.ecp[k];
+            p1 += __int_as_float(q.y) * fdot(f3(g, q.x), t);
+        }
     }
     out[e] = (pv[e] - v[e]) + dt * (s.m_e[e] * p0 + p1);
 }
@@ -498,6 +502,20 @@ __global__ void __launch_bounds__(256) k_update_mixed(const float *const *V, con
     for (int32_t i = 0; i < j; ++i) s += y[i] * static_cast<double>(V[i][e]);
     delta[e] += s * static_cast<double>(invdiag[e]);
 }
+__global__ void __launch_bounds__(256) k_pack_ec(const int32_t *cell, const float *w, int64_t n, int2 *out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_int2(cell[i], __float_as_int(w[i]));
+}
+__global__ void __launch_bounds__(256) k_pack_e2f(const int32_t *e2f, const float *star2, int64_t n, int2 *out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        const int32_t q = e2f[i], f = pid(q);
+        out[i] = make_int2(f, __float_as_int(fsg(q) * star2[f]));
+    }
+}
 __global__ void __launch_bounds__(256) k_f64_to_f32(const double *x, int64_t n, float *out)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -557,6 +575,14 @@ void mhdg_ctx::mixed_build()
     M.s.m_e = f32(d.m_e, E);
     M.s.recon = f32(d.recon, 3 * nc2f);
     M.s.ecw = f32(d.ecw, nec);
+    {
+        int2 *ecp = static_cast<int2 *>(alloc(nec * sizeof(int2)));
+        hipLaunchKernelGGL(k_pack_ec, dim3(nb(nec)), dim3(256), 0, stream, d.ec_cell, M.s.ecw, int64_t(nec), ecp);
+        M.s.ecp = ecp;
+        int2 *e2fp = static_cast<int2 *>(alloc(e2f_nnz * sizeof(int2)));
+        hipLaunchKernelGGL(k_pack_e2f, dim3(nb(e2f_nnz)), dim3(256), 0, stream, d.e2f, M.s.star2, int64_t(e2f_nnz), e2fp);
+        M.s.e2fp = e2fp;
+    }
     M.s.evec = f32(d.evec, 3 * E);
     {
         float *aV = static_cast<float *>(alloc(3 * nc2f * sizeof(float)));

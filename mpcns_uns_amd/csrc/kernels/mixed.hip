@@ -30,6 +30,14 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 } // namespace
 
+// Block index that gives each XCD one contiguous range of blocks (the hardware runs block b on XCD b % 8), so that
+// neighbouring edges share an L2 for their gathers: 4 % faster for the edge kernel, slower for the cell kernels
+__device__ __forceinline__ int32_t xcd_block()
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x, per = nb / 8;
+    if (b >= 8 * per) return int32_t(b);
+    return int32_t((b % 8) * per + b / 8);
+}
 // Row range of a CSR table: fixed degree (deg > 0, every row the same length) or the ptr array
 __device__ __forceinline__ void rows(int32_t deg, const int32_t *ptr, int32_t r, int32_t &k0, int32_t &k1)
 {
@@ -60,7 +68,7 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, 
                                                         const float *g, const float *v, const float *pv, const int8_t *dir, float dt,
                                                         float *out)
 {
-    const int32_t e = gid();
+    const int32_t e = xcd_block() * int32_t(blockDim.x) + int32_t(threadIdx.x);
     if (e >= d.ne) return;
     if (dir[e]) { out[e] = 0.0f; return; }
     const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];

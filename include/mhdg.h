@@ -83,6 +83,10 @@ typedef struct mhdg_config {
      * created at rest with temperature src_T (energy q_s src_T / (gamma - 1)). */
     double src_rate[10];
     double src_scale_height, src_r0, src_T;
+    /* Scales::induce_coeff (Scales.h:23; 0 in Scales::simple): with the Lorentz force on, the step applies the
+     * exact gyration of the two ions' relative velocity about B as two half steps around the fluid update
+     * (mercury::gyrate, MercuryUns/physics/Gyration.cpp:17-51, Solver.cpp:237-241, 272).  Two-ion fluids. */
+    double induce_coeff;
 } mhdg_config;
 
 /* Fills `cfg` with the reference defaults (Config.h:19-96 initialisers, Scales::simple(5/3, 1, 1)). */

@@ -49,6 +49,7 @@ class Config(C.Structure):
         ("ion_fs", (C.c_double * 5) * 10),
         ("src_rate", C.c_double * 10),
         ("src_scale_height", C.c_double), ("src_r0", C.c_double), ("src_T", C.c_double),
+        ("induce_coeff", C.c_double),
     ]
 
 

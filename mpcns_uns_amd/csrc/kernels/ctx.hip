@@ -160,6 +160,9 @@ static void make_params(const mhdg_config &c, Params &P)
     for (int s = 0; s < P.ns; ++s) { P.src_rate[s] = c.src_rate[s]; src = src || c.src_rate[s] != 0.0; }
     if (src && !(c.src_scale_height > 0)) throw std::runtime_error("src_scale_height must be positive with ion production");
     P.src_H = c.src_scale_height; P.src_r0 = c.src_r0; P.src_T = c.src_T;
+    if (c.induce_coeff < 0) throw std::runtime_error("induce_coeff must be >= 0");
+    if (c.induce_coeff > 0 && P.ns != 2) throw std::runtime_error("the gyration split needs the two-ion fluid");
+    P.induce_coeff = c.induce_coeff;
     // E_inf = -1 * (u_inf x B_imf) (MercuryUns/driver/Solver.cpp:162)
     const V3 u(c.u_inf[0], c.u_inf[1], c.u_inf[2]), B(c.B_imf[0], c.B_imf[1], c.B_imf[2]);
     const V3 E = -1.0 * cross(u, B);
@@ -400,6 +403,15 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     hipEvent_t ev;
     hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
     hip_check(hipEventRecord(ev, stream), "event");
+    // Strang split of the ions' gyration (Solver.cpp:237-241): half a step, then prepare again
+    const bool split = P.induce_coeff > 0 && P.lorentz;
+    if (split)
+    {
+        LAUNCH("gyrate", k_gyrate, dim3(nb), dim3(256), 0, stream, d, P, U[0], U[1], static_cast<const double *>(Btot),
+               static_cast<const double *>(dt_dev), 0.5);
+        LAUNCH("prepare", k_prepare<double>, dim3(nb), dim3(256), 0, stream, d, P, Uc, static_cast<const double *>(Phi),
+               static_cast<const double *>(Badd), Ww, Bind, Btot, uplus, alpha, dt_part);
+    }
     SpC Ws{}, none{}, gradsC{};
     SpW grads{}, Us{}, noneW{};
     for (int s = 0; s < P.ns; ++s)
@@ -447,6 +459,9 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     else
         hip_check(hipMemcpyAsync(E_last, E_expl, size_t(ne) * 8, hipMemcpyDeviceToDevice, stream), "copy");
     LAUNCH("faraday", k_faraday<double>, dim3(nblk(nf)), dim3(256), 0, stream, d, Euse, nullptr, dt_dev, 0.0, Phi);
+    if (split) // the second half of the gyration, with the field of the step's prepare (Solver.cpp:272)
+        LAUNCH("gyrate", k_gyrate, dim3(nb), dim3(256), 0, stream, d, P, U[0], U[1], static_cast<const double *>(Btot),
+               static_cast<const double *>(dt_dev), 0.5);
     check_launch("faraday");
     if (prof_on) prof_flush();
     return gdt;
@@ -833,6 +848,7 @@ int mhdg_explicit_tangent(mhdg_ctx *c, int n_steps, double dt_max, const double 
     MHDG_TRY(c, {
         if (n_steps < 0) throw std::runtime_error("explicit_tangent: n_steps >= 0");
         if (c->P.ns != 2) throw std::runtime_error("explicit_tangent: two-species (H+, Na+) contexts only");
+        if (c->P.induce_coeff > 0) throw std::runtime_error("explicit_tangent: the gyration split has no tangent");
         c->tangent_alloc();
         const size_t C = c->nc, F = c->nf;
         auto seed = [&](const double *val, const double *host_d, size_t n, int slot, Dual *out) {
@@ -870,6 +886,7 @@ int mhdg_step_vjp(mhdg_ctx *c, int n_steps, double dt_max, double r0, double r1,
     MHDG_TRY(c, {
         if (n_steps < 0) throw std::runtime_error("step_vjp: n_steps >= 0");
         if (c->P.ns != 2) throw std::runtime_error("step_vjp: two-species (H+, Na+) contexts only");
+        if (c->P.induce_coeff > 0) throw std::runtime_error("step_vjp: the gyration split has no adjoint");
         const size_t C = c->nc, F = c->nf;
         const double v = c->step_vjp(n_steps, dt_max, r0, r1, gfs, gE_inf);
         if (J) *J = v;

@@ -66,6 +66,7 @@ struct Params
     // exponential-atmosphere ion production q_s = rate_s exp(-(|x| - r0) / H), new ions at rest at temperature T
     double src_rate[kMaxSpecies];
     double src_H, src_r0, src_T;
+    double induce_coeff; // gyration split (0: off)
 };
 
 // Per-species array pointers passed by value (kernel arguments)

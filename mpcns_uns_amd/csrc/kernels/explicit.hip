@@ -186,6 +186,44 @@ template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh
 }
 
 
+// mercury::gyrate_n (MercuryUns/physics/Gyration.cpp:17-51) for the two ions over dt = scale * dt_dev[0]: the relative
+// velocity's component normal to B rotates by theta = (a_H chi_Na + a_Na chi_H) |B| dt, a_s = induce_coeff M_ref / M_s,
+// shared by the ions in proportion to a_H chi_Na and a_Na chi_H; momentum and kinetic energy follow, thermal energy
+// stays.  The operation order is the reference's; cos and sin are the device library's.
+__global__ void __launch_bounds__(256) k_gyrate(DevMesh d, Params P, double *UH, double *UNa, const double *Btot, const double *dt_dev,
+                                                double scale)
+{
+    const int32_t c = gid();
+    if (c >= d.nc || d.mat[c] != 0) return;
+    const double aH = P.induce_coeff * P.M_ref / P.M_s[0], aN = P.induce_coeff * P.M_ref / P.M_s[1];
+    double *uh = UH + 5 * size_t(c), *un = UNa + 5 * size_t(c);
+    const Prim wh = to_prim(uh, P.gamma, P.rho_floor, P.p_floor), wn = to_prim(un, P.gamma, P.rho_floor, P.p_floor);
+    // Scales::numinfo on the conservative densities
+    const double nH = mx(uh[0], 0.0) / P.M_s[0] * P.M_ref, nN = mx(un[0], 0.0) / P.M_s[1] * P.M_ref;
+    const double ne = nH + nN;
+    const double chiH = ne > 1e-300 ? nH / ne : 1.0, chiN = ne > 1e-300 ? nN / ne : 0.0;
+    const V3 b = L3(Btot, c);
+    const double bm = norm(b);
+    const double denom = aH * chiN + aN * chiH;
+    if (bm <= 0 || denom <= 0) return;
+    const V3 bh = (1.0 / bm) * b;
+    const V3 uH(wh.u, wh.v, wh.w), uN(wn.u, wn.v, wn.w);
+    const V3 w = uH - uN;
+    const V3 w_par = dot(w, bh) * bh, w_perp = w - w_par;
+    const double th = denom * bm * (scale * dt_dev[0]);
+    const V3 dw = (cos(th) - 1.0) * w_perp + sin(th) * cross(w_perp, bh);
+    const double fH = aH * chiN / denom, fN = aN * chiH / denom;
+    const V3 dH = fH * dw, dN = -fN * dw;
+    const V3 uH2 = uH + dH, uN2 = uN + dN;
+    auto set = [](double *u, const V3 &v0, const V3 &v1) {
+        const double rho = u[0];
+        u[1] = rho * v1.x; u[2] = rho * v1.y; u[3] = rho * v1.z;
+        u[4] += 0.5 * rho * (dot(v1, v1) - dot(v0, v0));
+    };
+    set(uh, uH, uH2);
+    set(un, uN, uN2);
+}
+
 namespace
 {
 // Rusanov flux F*|A| of species slot s (species sp) through non-skipped face f with at least one fluid cell

@@ -918,6 +918,43 @@ void enforce_floors(orc_ctx &o)
 
 Prim fs_of(const double *p) { return Prim{p[0], p[1], p[2], p[3], p[4]}; }
 
+// MercuryUns/physics/Gyration.cpp:17-51 (gyrate_n, dt_cell = null) for the two ions
+void gyrate(orc_ctx &o, double dt)
+{
+    const mhdg_config &c = o.cfg;
+    const double aH = c.induce_coeff * c.M_ref / o.M_s[0], aN = c.induce_coeff * c.M_ref / o.M_s[1];
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < o.nc; ++i)
+    {
+        if (o.mat[i] == 1) continue;
+        double *uh = &o.U[0][5 * size_t(i)], *un = &o.U[1][5 * size_t(i)];
+        const Prim wh = to_prim(uh, c.gamma, c.rho_floor, c.p_floor), wn = to_prim(un, c.gamma, c.rho_floor, c.p_floor);
+        const double nH = std::max(uh[0], 0.0) / o.M_s[0] * c.M_ref, nN = std::max(un[0], 0.0) / o.M_s[1] * c.M_ref;
+        const double ne = nH + nN;
+        const double chiH = ne > 1e-300 ? nH / ne : 1.0, chiN = ne > 1e-300 ? nN / ne : 0.0;
+        const V3 b = L3(o.Btot, i);
+        const double bm = norm(b);
+        const double denom = aH * chiN + aN * chiH;
+        if (bm <= 0 || denom <= 0) continue;
+        const V3 bh = (1.0 / bm) * b;
+        const V3 uH(wh.u, wh.v, wh.w), uN(wn.u, wn.v, wn.w);
+        const V3 w = uH - uN;
+        const V3 w_par = dot(w, bh) * bh, w_perp = w - w_par;
+        const double th = denom * bm * dt;
+        const V3 dw = (std::cos(th) - 1.0) * w_perp + std::sin(th) * cross(w_perp, bh);
+        const double fH = aH * chiN / denom, fN = aN * chiH / denom;
+        const V3 dH = fH * dw, dN = -fN * dw;
+        const V3 uH2 = uH + dH, uN2 = uN + dN;
+        auto set = [](double *u, const V3 &v0, const V3 &v1) {
+            const double rho = u[0];
+            u[1] = rho * v1.x; u[2] = rho * v1.y; u[3] = rho * v1.z;
+            u[4] += 0.5 * rho * (dot(v1, v1) - dot(v0, v0));
+        };
+        set(uh, uH, uH2);
+        set(un, uN, uN2);
+    }
+}
+
 // MercuryUns/driver/Solver.cpp:228-277 (gyration split inactive: induce_coeff = 0)
 double step(orc_ctx &o, double dt_max, int *iters, double *res)
 {
@@ -929,6 +966,13 @@ double step(orc_ctx &o, double dt_max, int *iters, double *res)
     double gdt = local_dt(o, Wall, o.Btot.data());
     gdt = std::min(gdt, dt_max);
     if (!std::isfinite(gdt) || gdt <= 0) throw std::runtime_error("invalid time step");
+    // Strang split of the gyration (Solver.cpp:237-241, 272)
+    const bool split = c.induce_coeff > 0 && c.lorentz_force;
+    if (split)
+    {
+        gyrate(o, 0.5 * gdt);
+        prepare(o);
+    }
     for (int k = 0; k < o.ns; ++k)
     {
         if (c.second_order) limited_gradients(o, o.W[k].data(), k, fs_of(o.fs_s[k]), true, nullptr, o.grad[k].data());
@@ -972,6 +1016,7 @@ double step(orc_ctx &o, double dt_max, int *iters, double *res)
         *iters = ind_solve(o, o.Phi.data(), o.E.data(), dir.data(), gdt, Eout.data(), res);
     }
     faraday(o, Eout.data(), nullptr, o.Phi.data(), gdt);
+    if (split) gyrate(o, 0.5 * gdt);
     o.E = Eout;
     return gdt;
 }

@@ -67,6 +67,7 @@ Config make_config(const mhdg_config &c, int dipole, const double *m, const doub
     k.sc.M_ref = c.M_ref; k.sc.M_H = c.M_H; k.sc.M_Na = c.M_Na;
     k.sc.ne_floor = c.ne_floor;
     k.sc.hall_coef = c.hall_coef;
+    k.sc.induce_coeff = c.induce_coeff;
     k.sc.inver_Rem = c.inver_Rem;
     k.cfl = c.cfl;
     k.second_order = c.second_order != 0;

@@ -365,3 +365,25 @@ def test_prepare_and_solution_files(small_explicit, tmp_path):
         b.write(m, solution_fields(orc, m), step, t)
     for name in ("sol_0000002_r00000.vtu", "sol_0000002.pvtu", "solution.pvd"):
         assert (tmp_path / "dev" / name).read_bytes() == (tmp_path / "orc" / name).read_bytes(), name
+
+
+@pytest.mark.parametrize("hall", [False, True])
+def test_gyration_split_step(hall):
+    """The fused step with the gyration split (induce_coeff > 0) against the oracle, which matches the reference bit for
+    bit (tests/test_oracle_reference.py).  The rotation uses the device's cos and sin, so agreement is to round-off."""
+    from mpcns_uns_amd import Device, workloads
+    from oracle.oracle import Oracle
+
+    w = workloads.build(net=(12, 48, 3), hall=hall, induce_coeff=40.0)
+    dev, orc = Device(w["mesh"], w["cfg"]), Oracle(w["mesh"], w["cfg"])
+    for x in (dev, orc):
+        x.set_background(*w["background"])
+        x.set_state(*w["state"])
+    for _ in range(3):
+        dd, _, rd = dev.step()
+        do, _, ro = orc.step()
+        assert abs(dd - do) <= 1e-14 * do
+    for a, b in zip(dev.get_state(), orc.get_state()):
+        assert maxrel(a, b) < 1e-12
+    with pytest.raises(RuntimeError):
+        dev.step_vjp(1, 1.2, 1.5)

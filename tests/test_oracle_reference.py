@@ -156,3 +156,27 @@ def test_implicit_ideal_and_edge_dt_bitwise():
         Er, itr, rr = ref.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, dt_edge=dt_edge)
         assert ito == itr and ro == rr and np.array_equal(Eo, Er)
         assert rr < w["cfg"].ind_rtol
+
+
+@pytest.mark.parametrize("hall", [False, True])
+def test_gyration_split_steps_bitwise(hall):
+    """step_first_order with the gyration split (Scales::induce_coeff > 0, MercuryUns/physics/Gyration.cpp:17-51,
+    Solver.cpp:237-241, 272): identical states after 3 steps, and the split changed them."""
+    from oracle.oracle import Oracle
+
+    w, ref = _shell(hall, induce_coeff=40.0)
+    orc = Oracle(w["mesh"], w["cfg"])
+    orc.set_background(*w["background"])
+    orc.set_state(*w["state"])
+    ref.solver_init()
+    ref.set_state(*w["state"])
+    w0, _ = _shell(hall)
+    plain = Oracle(w0["mesh"], w0["cfg"])
+    plain.set_background(*w0["background"])
+    plain.set_state(*w0["state"])
+    for _ in range(3):
+        assert ref.step() == orc.step()[0]
+        plain.step()
+        for a, b in zip(ref.get_state(), orc.get_state()):
+            assert np.array_equal(a, b)
+    assert not np.array_equal(orc.get_state()[0], plain.get_state()[0])

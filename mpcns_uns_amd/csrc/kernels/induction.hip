@@ -132,11 +132,26 @@ __global__ void __launch_bounds__(256) k_d1(DevMesh d, const double *v, const do
     const int32_t f = gid();
     if (f >= d.nf) return;
     double s = 0.0;
-    for (int32_t k = d.f2e_ptr[f]; k < d.f2e_ptr[f + 1]; ++k)
+    const int32_t k0 = d.f2e_ptr[f], k1 = d.f2e_ptr[f + 1];
+    for (int32_t k = k0; k < k1; k += kRowUnroll)
     {
-        const int32_t q = d.f2e[k], e = pid(q);
-        const double x = invdiag ? (dir[e] ? 0.0 : v[e] * invdiag[e]) : v[e];
-        s += dte ? psg(q) * dte[e] * x : psg(q) * x;
+        // loads of the group first, sums in order
+        int32_t q[kRowUnroll];
+        double x[kRowUnroll], t[kRowUnroll];
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u)
+            if (k + u < k1) q[u] = d.f2e[k + u];
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u)
+            if (k + u < k1)
+            {
+                const int32_t e = pid(q[u]);
+                x[u] = invdiag ? (dir[e] ? 0.0 : v[e] * invdiag[e]) : v[e];
+                if (dte) t[u] = dte[e];
+            }
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u)
+            if (k + u < k1) s += dte ? psg(q[u]) * t[u] * x[u] : psg(q[u]) * x[u];
     }
     phi[f] = s;
 }
@@ -147,10 +162,20 @@ __global__ void __launch_bounds__(256) k_bcell(DevMesh d, const double *phi, dou
     const int32_t c = gid();
     if (c >= d.nc) return;
     double bx = 0.0, by = 0.0, bz = 0.0;
-    for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+    const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1];
+    for (int32_t k = k0; k < k1; k += kRowUnroll)
     {
-        const double v = phi[pid(d.c2f[k])];
-        bx += d.recon[3 * size_t(k)] * v; by += d.recon[3 * size_t(k) + 1] * v; bz += d.recon[3 * size_t(k) + 2] * v;
+        double x[kRowUnroll];
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u)
+            if (k + u < k1) x[u] = phi[pid(d.c2f[k + u])];
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u)
+            if (k + u < k1)
+            {
+                const size_t r = 3 * size_t(k + u);
+                bx += d.recon[r] * x[u]; by += d.recon[r + 1] * x[u]; bz += d.recon[r + 2] * x[u];
+            }
     }
     b[3 * size_t(c)] = bx; b[3 * size_t(c) + 1] = by; b[3 * size_t(c) + 2] = bz;
 }
@@ -166,13 +191,33 @@ __global__ void __launch_bounds__(256) k_hall_g(DevMesh d, const double *b, cons
     {
         V3 sum;
         const V3 bc = L3(b, c);
-        for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+        const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1];
+        for (int32_t k = k0; k < k1; k += kRowUnroll)
         {
-            const int32_t v = d.c2f[k], f = pid(v);
-            V3 bf = bc;
-            const int32_t l = d.fL[f], r = d.fR[f];
-            if (l >= 0 && r >= 0) bf = 0.5 * (bc + L3(b, l == c ? r : l));
-            sum += cross(psg(v) * L3(d.farea, f), bf);
+            int32_t q[kRowUnroll], o[kRowUnroll];
+            V3 A[kRowUnroll], bo[kRowUnroll];
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u)
+                if (k + u < k1) q[u] = d.c2f[k + u];
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u)
+                if (k + u < k1)
+                {
+                    const int32_t f = pid(q[u]), l = d.fL[f], r = d.fR[f];
+                    o[u] = (l >= 0 && r >= 0) ? (l == c ? r : l) : -1;
+                    A[u] = L3(d.farea, f);
+                }
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u)
+                if (k + u < k1 && o[u] >= 0) bo[u] = L3(b, o[u]);
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u)
+                if (k + u < k1)
+                {
+                    V3 bf = bc;
+                    if (o[u] >= 0) bf = 0.5 * (bc + bo[u]);
+                    sum += cross(psg(q[u]) * A[u], bf);
+                }
         }
         const V3 J = (1.0 / d.cvol[c]) * sum;
         out = a * cross(J, L3(B0, c));
@@ -191,23 +236,51 @@ __global__ void __launch_bounds__(256) k_edge_apply(DevMesh d, int32_t resistive
 {
     const int32_t e = gid();
     if (e >= d.ne) return;
-    if (d.e2f_ptr[e + 1] - d.e2f_ptr[e] > kLightMax) return; // k_edge_apply_heavy
+    const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
+    if (f1 - f0 > kLightMax) return; // k_edge_apply_heavy
     double p0 = 0.0, p1 = 0.0;
+    // each group's index loads, then its gathers, then the sums in entry order
     if (resistive)
-        for (int32_t k = d.e2f_ptr[e]; k < d.e2f_ptr[e + 1]; ++k)
+        for (int32_t k = f0; k < f1; k += kEdgeUnroll)
         {
-            const int32_t q = d.e2f[k], f = pid(q);
-            const double val = d.star2[f] * phi[f];
-            p0 += psg(q) * val;
+            int32_t q[kEdgeUnroll];
+            double st[kEdgeUnroll], x[kEdgeUnroll];
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < f1) q[u] = d.e2f[k + u];
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < f1) { st[u] = d.star2[pid(q[u])]; x[u] = phi[pid(q[u])]; }
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < f1)
+                {
+                    const double val = st[u] * x[u];
+                    p0 += psg(q[u]) * val;
+                }
         }
     if (hall)
     {
         const V3 t = L3(d.evec, e);
-        for (int32_t k = d.ec_ptr[e]; k < d.ec_ptr[e + 1]; ++k)
+        const int32_t c0 = d.ec_ptr[e], c1 = d.ec_ptr[e + 1];
+        for (int32_t k = c0; k < c1; k += kEdgeUnroll)
         {
-            const double w = d.ecw[k];
-            const V3 wt = w * t;
-            p1 += dot(L3(g, d.ec_cell[k]), wt);
+            int32_t q[kEdgeUnroll];
+            double w[kEdgeUnroll];
+            V3 x[kEdgeUnroll];
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < c1) { q[u] = d.ec_cell[k + u]; w[u] = d.ecw[k + u]; }
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < c1) x[u] = L3(g, q[u]);
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < c1)
+                {
+                    const V3 wt = w[u] * t;
+                    p1 += dot(x[u], wt);
+                }
         }
     }
     const double p2 = ide ? (imask[e] ? ide[e] : 0.0) : 0.0;

@@ -6,6 +6,10 @@ namespace mhdg
 {
 constexpr int32_t kHeavyMax = 1024; // largest edge valence handled by k_edge_emf_heavy (LDS arrays)
 constexpr int32_t kLightMax = 32;   // edges up to this valence use one thread each
+// Gather loops issue the index loads of this many table entries together, then their gathers, then the sums in
+// entry order (a loop of one entry at a time waits on each index load before its gather)
+constexpr int kEdgeUnroll = 8; // entries of an edge row (faces or cells around the edge)
+constexpr int kRowUnroll = 4;  // entries of a face or cell row (edges of a face, faces of a cell)
 // LDS arrays of k_edge_emf_heavy: half the valence for the 16-byte Dual
 template <class R> constexpr int32_t kHeavyShared() { return sizeof(R) == sizeof(double) ? kHeavyMax : kHeavyMax / 2; }
 

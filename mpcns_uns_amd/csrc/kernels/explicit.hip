@@ -25,8 +25,9 @@ template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, P
             R *w = Ws.p[s] + 5 * size_t(c);
             w[0] = a.rho; w[1] = a.u; w[2] = a.v; w[3] = a.w; w[4] = a.p;
         }
-        SpCT<R> Wc;
-        for (int s = 0; s < kMaxSpecies; ++s) Wc.p[s] = Ws.p[s];
+        // the species rows through the kernel argument itself (a local copy of the pointer table, indexed by a
+        // run-time species number, went to scratch memory)
+        const SpCT<R> &Wc = reinterpret_cast<const SpCT<R> &>(Ws);
         const V3T<R> bi = cell_vector(d, c, Phi);
         const V3T<R> ba = L3(Badd, c);
         const V3T<R> bt(bi.x + ba.x, bi.y + ba.y, bi.z + ba.z);
@@ -439,33 +440,60 @@ template <class R> __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, 
     if (nf > vmax) return;
     R out = E_in ? E_in[e] : 0.0;
     const int32_t c0 = d.ec_ptr[e], c1 = d.ec_ptr[e + 1];
+    const int32_t nc = c1 - c0;
+    const bool fast = nf <= kEmfFast && nc <= kEmfFast;
+    // common valence: the row indices are loaded once, together, before their gathers
+    int32_t fid[kEmfFast], cid[kEmfFast];
     bool any_fluid = false;
-    for (int32_t k = c0; k < c1 && !any_fluid; ++k) any_fluid = d.mat[d.ec_cell[k]] == 0;
+    if (fast)
+    {
+#pragma unroll
+        for (int i = 0; i < kEmfFast; ++i)
+        {
+            if (i < nf) fid[i] = pid(d.e2f[f0 + i]);
+            if (i < nc) cid[i] = d.ec_cell[c0 + i];
+        }
+#pragma unroll
+        for (int k = 0; k < kEmfFast; ++k)
+            if (k < nc) any_fluid |= d.mat[cid[k]] == 0;
+    }
+    else
+        for (int32_t k = c0; k < c1 && !any_fluid; ++k) any_fluid = d.mat[d.ec_cell[k]] == 0;
     if (nf > 0 && d.elen[e] > 0 && any_fluid)
     {
         const V3 t = L3(d.evec, e);
         const double L = d.elen[e];
         const V3 th = (1.0 / L) * t;
         R Ec = 0;
-        for (int32_t i = f0; i < f1; ++i) Ec += d.efw[i] * dot(L3(Ef, pid(d.e2f[i])), t);
+        if (fast)
+        {
+            V3T<R> ef[kEmfFast];
+#pragma unroll
+            for (int i = 0; i < kEmfFast; ++i)
+                if (i < nf) ef[i] = L3(Ef, fid[i]);
+#pragma unroll
+            for (int i = 0; i < kEmfFast; ++i)
+                if (i < nf) Ec += d.efw[f0 + i] * dot(ef[i], t);
+        }
+        else
+            for (int32_t i = f0; i < f1; ++i) Ec += d.efw[i] * dot(L3(Ef, pid(d.e2f[i])), t);
         V3 e1, e2;
         plane_basis(th, e1, e2);
         R M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0, Eu = 0;
-        const int32_t nc = c1 - c0;
-        if (nf <= kEmfFast && nc <= kEmfFast)
+        if (fast)
         {
-            // common valence: the edge's cell velocities and the per-face terms stay in registers, so the
-            // second pass re-reads nothing (same operations in the same order as the general path)
+            // the edge's cell velocities and the per-face terms stay in registers, so the second pass re-reads
+            // nothing (same operations in the same order as the general path)
             V3T<R> uc[kEmfFast];
 #pragma unroll
             for (int k = 0; k < kEmfFast; ++k)
-                if (k < nc) uc[k] = L3(u_cell, d.ec_cell[c0 + k]);
+                if (k < nc) uc[k] = L3(u_cell, cid[k]);
             R fqa[kEmfFast], fqb[kEmfFast], fw[kEmfFast], faef[kEmfFast], fsig[kEmfFast], fpf[kEmfFast];
 #pragma unroll
             for (int i = 0; i < kEmfFast; ++i)
                 if (i < nf)
                 {
-                    const int32_t f = pid(d.e2f[f0 + i]);
+                    const int32_t f = fid[i];
                     V3 m, nrm;
                     face_dirs(d, f, e, th, m, nrm);
                     R aef = 0.0;

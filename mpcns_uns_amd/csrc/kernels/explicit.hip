@@ -66,11 +66,44 @@ template <class R> __global__ void __launch_bounds__(256) k_dt_final(const R *pa
     if (threadIdx.x == 0) out[0] = mn(v, dt_max);
 }
 
+// Stencil neighbour state of a boundary entry (nb < 0, face f = -(nb + 1), c2f entry v): the ghost table row, or
+// ghost_state (Common/fluid/Reconstruction.cpp:9-24) with the cell value wc inside.
+template <class R>
+__device__ __forceinline__ void stencil_ghost(const DevMesh &d, int32_t sp, const R *gh, const R *wc, const R *fs, int32_t v, int32_t nb,
+                                              R *wn)
+{
+    const int32_t f = -(nb + 1);
+    if (gh)
+    {
+        for (int m = 0; m < 5; ++m) wn[m] = gh[5 * size_t(f) + m];
+        return;
+    }
+    const int8_t kind = d.kind[f];
+    for (int m = 0; m < 5; ++m) wn[m] = wc[m];
+    if (kind == kWall)
+    {
+        if (sp == 1) { wn[0] = 1e-12; wn[1] = 0; wn[2] = 0; wn[3] = 0; wn[4] = 1e-12; }
+        else
+        {
+            const V3T<R> n = (psg(v) / d.famag[f]) * L3(d.farea, f);
+            const R un = wc[1] * n.x + wc[2] * n.y + wc[3] * n.z;
+            wn[1] -= 2 * un * n.x; wn[2] -= 2 * un * n.y; wn[3] -= 2 * un * n.z;
+        }
+    }
+    else if (kind == kInflow)
+        for (int m = 0; m < 5; ++m) wn[m] = fs[m];
+}
+
+constexpr int kGradFast = 4; // cells with up to this many faces keep their rows in registers (tetrahedra)
+
 // Stencil::limited_gradients (Common/fluid/Reconstruction.cpp:80-142) for `ns` species and cell_current
 // (Common/induction/Current.cpp:9-31, by the blocks of grid row 0).  ghost: optional per-face table of 5
 // primitives per species.
 // Each block stages its cells' 15-value gradient rows in LDS and writes them as one contiguous range (the
 // per-thread 120-byte rows written directly left partially written cache lines behind: 2.2x write traffic).
+// Cells of at most kGradFast faces load their face and stencil rows once, together, before the gathers, and
+// the limiter's face offsets once for every species; the operations and their order are those of the general
+// path.
 template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads,
                                                       int32_t limit, const R *Bind, R *J)
 {
@@ -80,22 +113,60 @@ template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh
     const int32_t n_here = min(int32_t(blockDim.x), d.nc - c_first);
     const bool act = c < d.nc;
     const int32_t k0 = act ? d.c2f_ptr[c] : 0, k1 = act ? d.c2f_ptr[c + 1] : 0;
+    const int32_t nk = k1 - k0;
+    const bool fast = act && nk <= kGradFast;
+    int32_t vk[kGradFast];
+    if (fast)
+#pragma unroll
+        for (int i = 0; i < kGradFast; ++i)
+            if (i < nk) vk[i] = d.c2f[k0 + i];
     if (act && J && blockIdx.y == 0)
     {
         V3T<R> sum;
         const V3T<R> bc = L3(Bind, c);
-        for (int32_t k = k0; k < k1; ++k)
+        if (fast)
         {
-            const int32_t v = d.c2f[k], f = pid(v);
-            V3T<R> bf = bc;
-            const int32_t l = d.fL[f], r = d.fR[f];
-            if (l >= 0 && r >= 0) bf = 0.5 * (bc + L3(Bind, l == c ? r : l));
-            sum += cross(psg(v) * L3(d.farea, f), bf);
+            int32_t o[kGradFast];
+            V3 A[kGradFast];
+            V3T<R> bo[kGradFast];
+#pragma unroll
+            for (int i = 0; i < kGradFast; ++i)
+                if (i < nk)
+                {
+                    const int32_t f = pid(vk[i]), l = d.fL[f], r = d.fR[f];
+                    o[i] = (l >= 0 && r >= 0) ? (l == c ? r : l) : -1;
+                    A[i] = L3(d.farea, f);
+                }
+#pragma unroll
+            for (int i = 0; i < kGradFast; ++i)
+                if (i < nk) bo[i] = L3(Bind, o[i] >= 0 ? o[i] : c);
+#pragma unroll
+            for (int i = 0; i < kGradFast; ++i)
+                if (i < nk)
+                {
+                    V3T<R> bf = bc;
+                    if (o[i] >= 0) bf = 0.5 * (bc + bo[i]);
+                    sum += cross(psg(vk[i]) * A[i], bf);
+                }
         }
+        else
+            for (int32_t k = k0; k < k1; ++k)
+            {
+                const int32_t v = d.c2f[k], f = pid(v);
+                V3T<R> bf = bc;
+                const int32_t l = d.fL[f], r = d.fR[f];
+                if (l >= 0 && r >= 0) bf = 0.5 * (bc + L3(Bind, l == c ? r : l));
+                sum += cross(psg(v) * L3(d.farea, f), bf);
+            }
         const R invV = 1.0 / d.cvol[c];
         st3(J, c, V3T<R>(sum.x * invV, sum.y * invV, sum.z * invV));
     }
     const V3T<R> xc = act ? L3(d.ccen, c) : V3T<R>();
+    V3T<R> dxk[kGradFast];
+    if (fast && limit)
+#pragma unroll
+        for (int i = 0; i < kGradFast; ++i)
+            if (i < nk) dxk[i] = L3(d.fcen, pid(vk[i])) - xc;
     // species s of the block: blockIdx.y, blockIdx.y + gridDim.y, ... (one grid row, every species, measured
     // faster than one row per species)
     for (int32_t s = int32_t(blockIdx.y); s < ns; s += int32_t(gridDim.y))
@@ -112,37 +183,7 @@ template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh
         for (int m = 0; m < 5; ++m) fs[m] = seeded<R>(P.fs[sp][m], P.dfs[sp][m]);
         R wc[5], r[5][3], lo[5], hi[5];
         for (int m = 0; m < 5; ++m) { wc[m] = W[5 * size_t(c) + m]; lo[m] = hi[m] = wc[m]; r[m][0] = r[m][1] = r[m][2] = 0.0; }
-        for (int32_t k = k0; k < k1; ++k)
-        {
-            const int32_t nb = d.snb[k];
-            if (nb == kNoStencil) continue;
-            R wn[5];
-            if (nb >= 0)
-                for (int m = 0; m < 5; ++m) wn[m] = W[5 * size_t(nb) + m];
-            else
-            {
-                const int32_t f = -(nb + 1);
-                if (gh)
-                    for (int m = 0; m < 5; ++m) wn[m] = gh[5 * size_t(f) + m];
-                else
-                {
-                    // ghost_state (Common/fluid/Reconstruction.cpp:9-24) with the cell value inside
-                    const int8_t kind = d.kind[f];
-                    for (int m = 0; m < 5; ++m) wn[m] = wc[m];
-                    if (kind == kWall)
-                    {
-                        if (sp == 1) { wn[0] = 1e-12; wn[1] = 0; wn[2] = 0; wn[3] = 0; wn[4] = 1e-12; }
-                        else
-                        {
-                            const V3T<R> n = (psg(d.c2f[k]) / d.famag[f]) * L3(d.farea, f);
-                            const R un = wc[1] * n.x + wc[2] * n.y + wc[3] * n.z;
-                            wn[1] -= 2 * un * n.x; wn[2] -= 2 * un * n.y; wn[3] -= 2 * un * n.z;
-                        }
-                    }
-                    else if (kind == kInflow)
-                        for (int m = 0; m < 5; ++m) wn[m] = fs[m];
-                }
-            }
+        auto accumulate = [&](int32_t k, const R *wn) {
             const R w = d.sw[k];
             const V3T<R> dd = L3(d.sd, k);
             for (int m = 0; m < 5; ++m)
@@ -152,29 +193,83 @@ template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh
                 lo[m] = mn(lo[m], wn[m]);
                 hi[m] = mx(hi[m], wn[m]);
             }
+        };
+        if (fast)
+        {
+            int32_t nbk[kGradFast];
+#pragma unroll
+            for (int i = 0; i < kGradFast; ++i)
+                if (i < nk) nbk[i] = d.snb[k0 + i];
+#pragma unroll
+            for (int i = 0; i < kGradFast; ++i)
+                if (i < nk && nbk[i] != kNoStencil)
+                {
+                    R wn[5];
+                    if (nbk[i] >= 0)
+                        for (int m = 0; m < 5; ++m) wn[m] = W[5 * size_t(nbk[i]) + m];
+                    else
+                        stencil_ghost(d, sp, gh, wc, fs, vk[i], nbk[i], wn);
+                    accumulate(k0 + i, wn);
+                }
         }
+        else
+            for (int32_t k = k0; k < k1; ++k)
+            {
+                const int32_t nb = d.snb[k];
+                if (nb == kNoStencil) continue;
+                R wn[5];
+                if (nb >= 0)
+                    for (int m = 0; m < 5; ++m) wn[m] = W[5 * size_t(nb) + m];
+                else
+                    stencil_ghost(d, sp, gh, wc, fs, d.c2f[k], nb, wn);
+                accumulate(k, wn);
+            }
         const double *I6 = d.sinv + 6 * size_t(c);
         const R I[9] = {I6[0], I6[1], I6[2], I6[1], I6[3], I6[4], I6[2], I6[4], I6[5]};
         for (int m = 0; m < 5; ++m)
         {
             const R g[3] = {I[0] * r[m][0] + I[1] * r[m][1] + I[2] * r[m][2], I[3] * r[m][0] + I[4] * r[m][1] + I[5] * r[m][2],
                                  I[6] * r[m][0] + I[7] * r[m][1] + I[8] * r[m][2]};
+            const bool rel = limit && P.rel_limit > 0 && (m == 0 || m == 4);
             R psi = 1.0;
-            if (limit)
-                for (int32_t k = k0; k < k1; ++k)
-                {
-                    const V3T<R> dx = L3(d.fcen, pid(d.c2f[k])) - xc;
-                    const R dl = g[0] * dx.x + g[1] * dx.y + g[2] * dx.z;
-                    if (dl > 1e-300) psi = mn(psi, (hi[m] - wc[m]) / dl);
-                    else if (dl < -1e-300) psi = mn(psi, (lo[m] - wc[m]) / dl);
-                }
-            if (limit && P.rel_limit > 0 && (m == 0 || m == 4))
-                for (int32_t k = k0; k < k1; ++k)
-                {
-                    const V3T<R> dx = L3(d.fcen, pid(d.c2f[k])) - xc;
-                    const R dl = fabs(g[0] * dx.x + g[1] * dx.y + g[2] * dx.z);
-                    if (dl > 1e-300) psi = mn(psi, P.rel_limit * wc[m] / dl);
-                }
+            if (fast)
+            {
+                if (limit)
+#pragma unroll
+                    for (int i = 0; i < kGradFast; ++i)
+                        if (i < nk)
+                        {
+                            const R dl = g[0] * dxk[i].x + g[1] * dxk[i].y + g[2] * dxk[i].z;
+                            if (dl > 1e-300) psi = mn(psi, (hi[m] - wc[m]) / dl);
+                            else if (dl < -1e-300) psi = mn(psi, (lo[m] - wc[m]) / dl);
+                        }
+                if (rel)
+#pragma unroll
+                    for (int i = 0; i < kGradFast; ++i)
+                        if (i < nk)
+                        {
+                            const R dl = fabs(g[0] * dxk[i].x + g[1] * dxk[i].y + g[2] * dxk[i].z);
+                            if (dl > 1e-300) psi = mn(psi, P.rel_limit * wc[m] / dl);
+                        }
+            }
+            else
+            {
+                if (limit)
+                    for (int32_t k = k0; k < k1; ++k)
+                    {
+                        const V3T<R> dx = L3(d.fcen, pid(d.c2f[k])) - xc;
+                        const R dl = g[0] * dx.x + g[1] * dx.y + g[2] * dx.z;
+                        if (dl > 1e-300) psi = mn(psi, (hi[m] - wc[m]) / dl);
+                        else if (dl < -1e-300) psi = mn(psi, (lo[m] - wc[m]) / dl);
+                    }
+                if (rel)
+                    for (int32_t k = k0; k < k1; ++k)
+                    {
+                        const V3T<R> dx = L3(d.fcen, pid(d.c2f[k])) - xc;
+                        const R dl = fabs(g[0] * dx.x + g[1] * dx.y + g[2] * dx.z);
+                        if (dl > 1e-300) psi = mn(psi, P.rel_limit * wc[m] / dl);
+                    }
+            }
             psi = mx(0.0, mn(1.0, psi));
             out[3 * m] = psi * g[0]; out[3 * m + 1] = psi * g[1]; out[3 * m + 2] = psi * g[2];
         }

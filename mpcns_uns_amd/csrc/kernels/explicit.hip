@@ -28,7 +28,34 @@ template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, P
         // the species rows through the kernel argument itself (a local copy of the pointer table, indexed by a
         // run-time species number, went to scratch memory)
         const SpCT<R> &Wc = reinterpret_cast<const SpCT<R> &>(Ws);
-        const V3T<R> bi = cell_vector(d, c, Phi);
+        // up to kGradFast faces: the face rows are loaded once, together, for the cell vector and the time step
+        const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1], nk = k1 - k0;
+        const bool fast = nk <= kGradFast;
+        double Ak[kGradFast];
+        V3 ak[kGradFast];
+        V3T<R> bi;
+        if (fast)
+        {
+            int32_t vk[kGradFast];
+            R fl[kGradFast];
+#pragma unroll
+            for (int i = 0; i < kGradFast; ++i)
+                if (i < nk) vk[i] = d.c2f[k0 + i];
+#pragma unroll
+            for (int i = 0; i < kGradFast; ++i)
+                if (i < nk)
+                {
+                    const int32_t f = pid(vk[i]);
+                    Ak[i] = d.famag[f]; ak[i] = L3(d.farea, f); fl[i] = Phi[f];
+                }
+            R M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, r[3] = {0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < kGradFast; ++i)
+                if (i < nk) cell_vector_add(M, r, vk[i], R(Ak[i]), ak[i], fl[i]);
+            bi = cell_vector_solve(M, r);
+        }
+        else
+            bi = cell_vector(d, c, Phi);
         const V3T<R> ba = L3(Badd, c);
         const V3T<R> bt(bi.x + ba.x, bi.y + ba.y, bi.z + ba.z);
         st3(Bind, c, bi);
@@ -47,7 +74,7 @@ template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, P
                 else up = V3T<R>(up.x + chi * w[1], up.y + chi * w[2], up.z + chi * w[3]);
             }
             al = (P.hall && ne_eff > 0) ? d.taper[c] * P.hall_coef / ne_eff : 0.0;
-            dtc = cell_dt(d, P, c, Wc, bt, size_t(c));
+            dtc = fast ? cell_dt_rows<R, kGradFast>(P, d.cvol[c], Wc, bt, size_t(c), nk, Ak, ak) : cell_dt(d, P, c, Wc, bt, size_t(c));
         }
         st3(uplus, c, up);
         alpha[c] = al;
@@ -93,8 +120,6 @@ __device__ __forceinline__ void stencil_ghost(const DevMesh &d, int32_t sp, cons
     else if (kind == kInflow)
         for (int m = 0; m < 5; ++m) wn[m] = fs[m];
 }
-
-constexpr int kGradFast = 4; // cells with up to this many faces keep their rows in registers (tetrahedra)
 
 // Stencil::limited_gradients (Common/fluid/Reconstruction.cpp:80-142) for `ns` species and cell_current
 // (Common/induction/Current.cpp:9-31, by the blocks of grid row 0).  ghost: optional per-face table of 5
@@ -465,6 +490,14 @@ template <class R> __global__ void __launch_bounds__(256) k_cell_update(DevMesh 
     const R V = d.cvol[c];
     // n_e of the cell's ions, once for every species' source
     const R ne_c = (sources && fluid) ? electron_density(P, Wall.p, c) : R(0.0);
+    // up to kGradFast faces: the face rows are loaded once, and each species' flux rows together before the sums
+    const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1], nk = k1 - k0;
+    const bool fast = nk <= kGradFast;
+    int32_t vk[kGradFast];
+    if (fast && fluid && FA)
+#pragma unroll
+        for (int i = 0; i < kGradFast; ++i)
+            if (i < nk) vk[i] = d.cfs[k0 + i];
     for (int32_t s = 0; s < ns; ++s)
     {
         const int32_t sp = s0 + s;
@@ -473,8 +506,23 @@ template <class R> __global__ void __launch_bounds__(256) k_cell_update(DevMesh 
             for (int m = 0; m < 5; ++m) rhs[m] = rhs_in.p[s][5 * size_t(c) + m];
         if (fluid)
         {
-            if (FA)
-            for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+            if (FA && fast)
+            {
+                R fa[kGradFast][5];
+#pragma unroll
+                for (int i = 0; i < kGradFast; ++i)
+                    if (i < nk)
+                        for (int m = 0; m < 5; ++m) fa[i][m] = FA[size_t(5 * ns) * pid(vk[i]) + 5 * s + m];
+#pragma unroll
+                for (int i = 0; i < kGradFast; ++i)
+                    if (i < nk)
+                    {
+                        if (vk[i] >= 0) for (int m = 0; m < 5; ++m) rhs[m] -= fa[i][m] / V;
+                        else for (int m = 0; m < 5; ++m) rhs[m] += fa[i][m] / V;
+                    }
+            }
+            else if (FA)
+            for (int32_t k = k0; k < k1; ++k)
             {
                 const int32_t v = d.cfs[k], f = pid(v);
                 const R *fa = FA + size_t(5 * ns) * f + 5 * s;

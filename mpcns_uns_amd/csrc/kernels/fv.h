@@ -8,24 +8,24 @@ namespace mhdg
 {
 namespace
 {
-// src/uns/geometry/Reconstruct.cpp:11-45 for one cell
-template <class R> __device__ V3T<R> cell_vector(const DevMesh &d, int32_t c, const R *flux)
+constexpr int kGradFast = 4; // cells with up to this many faces keep their face rows in registers (tetrahedra)
+
+// src/uns/geometry/Reconstruct.cpp:11-45 for one cell: the face term (c2f entry v, |A|, area vector, flux) and
+// the 3x3 solve
+template <class R> __device__ __forceinline__ void cell_vector_add(R (&M)[3][3], R (&r)[3], int32_t v, const R &A, const V3 &area, const R &flux)
 {
-    R M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, r[3] = {0, 0, 0};
-    for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+    if (A <= 0) return;
+    const V3T<R> n = (psg(v) / A) * area;
+    const R nv[3] = {n.x, n.y, n.z};
+    const R phi = psg(v) * flux;
+    for (int i = 0; i < 3; ++i)
     {
-        const int32_t v = d.c2f[k], f = pid(v);
-        const R A = d.famag[f];
-        if (A <= 0) continue;
-        const V3T<R> n = (psg(v) / A) * L3(d.farea, f);
-        const R nv[3] = {n.x, n.y, n.z};
-        const R phi = psg(v) * flux[f];
-        for (int i = 0; i < 3; ++i)
-        {
-            r[i] += nv[i] * phi;
-            for (int j = 0; j < 3; ++j) M[i][j] += A * nv[i] * nv[j];
-        }
+        r[i] += nv[i] * phi;
+        for (int j = 0; j < 3; ++j) M[i][j] += A * nv[i] * nv[j];
     }
+}
+template <class R> __device__ V3T<R> cell_vector_solve(const R (&M)[3][3], const R (&r)[3])
+{
     const R det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
                        M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
     const R sc = mx(mx(mx(M[0][0], M[1][1]), M[2][2]), 1e-300);
@@ -42,6 +42,16 @@ template <class R> __device__ V3T<R> cell_vector(const DevMesh &d, int32_t c, co
     I[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) / det;
     return {I[0][0] * r[0] + I[0][1] * r[1] + I[0][2] * r[2], I[1][0] * r[0] + I[1][1] * r[1] + I[1][2] * r[2],
             I[2][0] * r[0] + I[2][1] * r[1] + I[2][2] * r[2]};
+}
+template <class R> __device__ V3T<R> cell_vector(const DevMesh &d, int32_t c, const R *flux)
+{
+    R M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, r[3] = {0, 0, 0};
+    for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+    {
+        const int32_t v = d.c2f[k], f = pid(v);
+        cell_vector_add(M, r, v, R(d.famag[f]), L3(d.farea, f), flux[f]);
+    }
+    return cell_vector_solve(M, r);
 }
 
 // MercuryUns/fluid/MercurySources.cpp:104-145 for one fluid cell (+inf if no face restricts it)
@@ -71,6 +81,45 @@ template <class R> __device__ R cell_dt(const DevMesh &d, const Params &P, int32
             denom += (fabs(w[1] * n.x + w[2] * n.y + w[3] * n.z) + cf) * A;
         }
         if (denom > 0) dt = mn(dt, P.cfl * d.cvol[c] / denom);
+    }
+    return dt;
+}
+
+// cell_dt for a cell whose N or fewer faces are given as (|A|, area vector) rows: the face normal and the normal
+// Alfven term, which do not depend on the species, are formed once (same operations as cell_dt)
+template <class R, int N>
+__device__ R cell_dt_rows(const Params &P, double vol, const SpCT<R> &Ws, const V3T<R> &b, size_t row, int nk, const double (&A)[N],
+                          const V3 (&area)[N])
+{
+    R dt = __builtin_inf();
+    const R B2 = dot(b, b);
+    R rho_sum = Ws.p[0][5 * row];
+    for (int s = 1; s < P.ns; ++s) rho_sum = rho_sum + Ws.p[s][5 * row];
+    const R rho_tot = mx(rho_sum, P.rho_floor_dt);
+    const R vA2 = P.inver_MA2 * B2 / rho_tot;
+    V3T<R> n[N];
+    R vAn2[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        if (i < nk)
+        {
+            n[i] = (1.0 / A[i]) * area[i];
+            vAn2[i] = P.inver_MA2 * dot(b, n[i]) * dot(b, n[i]) / rho_tot;
+        }
+    for (int s = 0; s < P.ns; ++s)
+    {
+        const R *w = Ws.p[s] + 5 * row;
+        const R cs2 = P.gamma * w[4] / mx(w[0], 1e-30);
+        R denom = 0;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (i < nk)
+            {
+                const R term = cs2 + vA2;
+                const R cf = sqrt(0.5 * (term + sqrt(mx(0.0, term * term - 4 * cs2 * vAn2[i]))));
+                denom += (fabs(w[1] * n[i].x + w[2] * n[i].y + w[3] * n[i].z) + cf) * A[i];
+            }
+        if (denom > 0) dt = mn(dt, P.cfl * vol / denom);
     }
     return dt;
 }

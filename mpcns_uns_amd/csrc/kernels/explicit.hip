@@ -435,23 +435,7 @@ template <class R> __global__ void __launch_bounds__(256) k_face(DevMesh d, Para
     const int32_t l = act ? d.fL[f] : -1, r = act ? d.fR[f] : -1;
     const R A = act ? d.famag[f] : 1.0;
     const V3T<R> n = act ? (1.0 / A) * L3(d.farea, f) : V3T<R>();
-    if (FA && act)
-    {
-        const int8_t kind = d.kind[f];
-        const bool any = (l >= 0 && d.mat[l] == 0) || (r >= 0 && d.mat[r] == 0);
-        R *out = staged ? stage + 5 * ns * threadIdx.x : FA + size_t(5 * ns) * f;
-        if (kind == kSkip || !any)
-            for (int q = 0; q < 5 * ns; ++q) out[q] = 0.0;
-        else
-            for (int32_t s = 0; s < ns; ++s) face_species_flux(d, P, f, l, r, kind, A, n, s, s0 + s, Ws, grads, ghosts, out + 5 * s);
-    }
-    if (staged)
-    {
-        __syncthreads();
-        const int32_t f_first = blockIdx.x * blockDim.x, n_here = min(int32_t(blockDim.x), d.nf - f_first);
-        R *dst = FA + size_t(5 * ns) * f_first;
-        for (int32_t i = threadIdx.x; i < 5 * ns * n_here; i += blockDim.x) dst[i] = stage[i];
-    }
+    // the EMF candidate first: its gathers need only l and r
     if (Ef && act)
     {
         const R Bn_tot = (PhiAdd ? Phi[f] + PhiAdd[f] : Phi[f]) / A;
@@ -471,6 +455,23 @@ template <class R> __global__ void __launch_bounds__(256) k_face(DevMesh d, Para
             F = dot(u, n) * L3(Btot, c) - Bn_tot * u;
         }
         st3(Ef, f, -1.0 * cross(n, F));
+    }
+    if (FA && act)
+    {
+        const int8_t kind = d.kind[f];
+        const bool any = (l >= 0 && d.mat[l] == 0) || (r >= 0 && d.mat[r] == 0);
+        R *out = staged ? stage + 5 * ns * threadIdx.x : FA + size_t(5 * ns) * f;
+        if (kind == kSkip || !any)
+            for (int q = 0; q < 5 * ns; ++q) out[q] = 0.0;
+        else
+            for (int32_t s = 0; s < ns; ++s) face_species_flux(d, P, f, l, r, kind, A, n, s, s0 + s, Ws, grads, ghosts, out + 5 * s);
+    }
+    if (staged)
+    {
+        __syncthreads();
+        const int32_t f_first = blockIdx.x * blockDim.x, n_here = min(int32_t(blockDim.x), d.nf - f_first);
+        R *dst = FA + size_t(5 * ns) * f_first;
+        for (int32_t i = threadIdx.x; i < 5 * ns * n_here; i += blockDim.x) dst[i] = stage[i];
     }
 }
 

@@ -30,7 +30,7 @@ template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, P
         const SpCT<R> &Wc = reinterpret_cast<const SpCT<R> &>(Ws);
         // up to kGradFast faces: the face rows are loaded once, together, for the cell vector and the time step
         const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1], nk = k1 - k0;
-        const bool fast = nk <= kGradFast;
+        const bool fast = kRowsInRegisters<R> && nk <= kGradFast;
         double Ak[kGradFast];
         V3 ak[kGradFast];
         V3T<R> bi;
@@ -139,7 +139,7 @@ template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh
     const bool act = c < d.nc;
     const int32_t k0 = act ? d.c2f_ptr[c] : 0, k1 = act ? d.c2f_ptr[c + 1] : 0;
     const int32_t nk = k1 - k0;
-    const bool fast = act && nk <= kGradFast;
+    const bool fast = kRowsInRegisters<R> && act && nk <= kGradFast;
     int32_t vk[kGradFast];
     if (fast)
 #pragma unroll
@@ -493,7 +493,7 @@ template <class R> __global__ void __launch_bounds__(256) k_cell_update(DevMesh 
     const R ne_c = (sources && fluid) ? electron_density(P, Wall.p, c) : R(0.0);
     // up to kGradFast faces: the face rows are loaded once, and each species' flux rows together before the sums
     const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1], nk = k1 - k0;
-    const bool fast = nk <= kGradFast;
+    const bool fast = kRowsInRegisters<R> && nk <= kGradFast;
     int32_t vk[kGradFast];
     if (fast && fluid && FA)
 #pragma unroll

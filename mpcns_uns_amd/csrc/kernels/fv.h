@@ -9,6 +9,8 @@ namespace mhdg
 namespace
 {
 constexpr int kGradFast = 4; // cells with up to this many faces keep their face rows in registers (tetrahedra)
+// Only the double instantiations do: the Dual (forward-mode) ones would spill the rows to scratch memory
+template <class R> constexpr bool kRowsInRegisters = sizeof(R) == sizeof(double);
 
 // src/uns/geometry/Reconstruct.cpp:11-45 for one cell: the face term (c2f entry v, |A|, area vector, flux) and
 // the 3x3 solve

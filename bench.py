@@ -147,10 +147,16 @@ def main():
         n = int(args.copy_calibration * 2 ** 30) // 4
         x = torch.ones(n, device="cuda", dtype=torch.float32)
         y = torch.empty_like(x)
-        for _ in range(5):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        ev[0].record()
+        for i in range(5):
             y.copy_(x)
+            ev[i + 1].record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "repeats": 5}))
+        us = sorted(ev[i].elapsed_time(ev[i + 1]) * 1e3 for i in range(5))
+        # the practical HBM ceiling of a streaming kernel on this device: read + write bytes over the median copy
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "repeats": 5, "median_us": round(us[2], 1),
+                          "copy_GBps": round(8 * n / (us[2] * 1e-6) / 1e9, 1)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the path shards by mesh partition (see DESIGN.md), "

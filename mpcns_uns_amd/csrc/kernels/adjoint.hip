@@ -806,7 +806,7 @@ __global__ void __launch_bounds__(256) k_adj_dt_argmin(DevMesh d, Params P, SpC 
     const int32_t c = gid();
     double v = __builtin_inf();
     int32_t ci = INT32_MAX;
-    if (c < d.nc && d.mat[c] == 0) { v = cell_dt(d, P, c, Ws, L3(Btot, c), size_t(c)); ci = c; }
+    if (c < d.nc && d.mat[c] == 0) { v = cell_dt(d, P, c, Ws.p, L3(Btot, c), size_t(c)); ci = c; }
     const int t = threadIdx.x;
     sv[t] = v; sc[t] = ci;
     __syncthreads();
@@ -832,7 +832,7 @@ __global__ void k_adj_dt_cell(DevMesh d, Params P, int32_t c, SpC Ws, const doub
             for (int m = 0; m < 5; ++m) wd[5 * s + m] = Dual(Ws.p[s][5 * size_t(c) + m], j == 5 * s + m ? 1.0 : 0.0);
         const int o = j - 5 * kAdjNs;
         const V3D bd(Dual(b.x, o == 0 ? 1.0 : 0.0), Dual(b.y, o == 1 ? 1.0 : 0.0), Dual(b.z, o == 2 ? 1.0 : 0.0));
-        const Dual t = cell_dt(d, P, c, sp, bd, 0);
+        const Dual t = cell_dt(d, P, c, sp.p, bd, 0);
         const double g = dtb * t.d;
         if (o < 0) Wb.p[j / 5][5 * size_t(c) + j % 5] += g;
         else Btb[3 * size_t(c) + o] += g;

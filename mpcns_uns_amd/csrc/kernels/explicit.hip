@@ -25,9 +25,9 @@ template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, P
             R *w = Ws.p[s] + 5 * size_t(c);
             w[0] = a.rho; w[1] = a.u; w[2] = a.v; w[3] = a.w; w[4] = a.p;
         }
-        // the species rows through the kernel argument itself (a local copy of the pointer table, indexed by a
-        // run-time species number, went to scratch memory)
-        const SpCT<R> &Wc = reinterpret_cast<const SpCT<R> &>(Ws);
+        // the species rows through the kernel argument's pointer table itself (a local copy of the table, indexed by
+        // a run-time species number, went to scratch memory)
+        const R *const *Wc = Ws.p;
         // up to kGradFast faces: the face rows are loaded once, together, for the cell vector and the time step
         const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1], nk = k1 - k0;
         const bool fast = kRowsInRegisters<R> && nk <= kGradFast;
@@ -64,11 +64,11 @@ template <class R> __global__ void __launch_bounds__(256) k_prepare(DevMesh d, P
         R al = 0.0;
         if (d.mat[c] == 0)
         {
-            const R ne = electron_density(P, Wc.p, c);
+            const R ne = electron_density(P, Wc, c);
             const R ne_eff = sqrt(ne * ne + P.ne_floor * P.ne_floor);
             for (int s = 0; s < P.ns; ++s)
             {
-                const R *w = Wc.p[s] + 5 * size_t(c);
+                const R *w = Wc[s] + 5 * size_t(c);
                 const R chi = ne > 1e-300 ? ion_density(P, s, w[0]) / ne : (s == 0 ? R(1.0) : R(0.0));
                 if (s == 0) up = V3T<R>(chi * w[1], chi * w[2], chi * w[3]);
                 else up = V3T<R>(up.x + chi * w[1], up.y + chi * w[2], up.z + chi * w[3]);
@@ -861,7 +861,7 @@ __global__ void __launch_bounds__(256) k_local_dt(DevMesh d, Params P, SpC Ws, c
     __shared__ double sh[256];
     const int32_t c = gid();
     double dtc = __builtin_inf();
-    if (c < d.nc && d.mat[c] == 0) dtc = cell_dt(d, P, c, Ws, L3(Btot, c), size_t(c));
+    if (c < d.nc && d.mat[c] == 0) dtc = cell_dt(d, P, c, Ws.p, L3(Btot, c), size_t(c));
     const double m = block_min(dtc, sh);
     if (threadIdx.x == 0) dt_part[blockIdx.x] = m;
 }

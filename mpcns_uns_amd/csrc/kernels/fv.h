@@ -58,18 +58,18 @@ template <class R> __device__ V3T<R> cell_vector(const DevMesh &d, int32_t c, co
 
 // MercuryUns/fluid/MercurySources.cpp:104-145 for one fluid cell (+inf if no face restricts it)
 // (every ion species; the total density of the fast speed summed in species order)
-// (the species rows of cell c are read at row `row` of Ws: c itself, or 0 for a one-cell copy)
-template <class R> __device__ R cell_dt(const DevMesh &d, const Params &P, int32_t c, const SpCT<R> &Ws, const V3T<R> &b, size_t row)
+// (the species rows of cell c are read at row `row` of the species arrays Ws: c itself, or 0 for a one-cell copy)
+template <class R> __device__ R cell_dt(const DevMesh &d, const Params &P, int32_t c, const R *const *Ws, const V3T<R> &b, size_t row)
 {
     R dt = __builtin_inf();
     const R B2 = dot(b, b);
-    R rho_sum = Ws.p[0][5 * row];
-    for (int s = 1; s < P.ns; ++s) rho_sum = rho_sum + Ws.p[s][5 * row];
+    R rho_sum = Ws[0][5 * row];
+    for (int s = 1; s < P.ns; ++s) rho_sum = rho_sum + Ws[s][5 * row];
     const R rho_tot = mx(rho_sum, P.rho_floor_dt);
     const R vA2 = P.inver_MA2 * B2 / rho_tot; // the same for every face and species
     for (int s = 0; s < P.ns; ++s)
     {
-        const R *w = Ws.p[s] + 5 * row;
+        const R *w = Ws[s] + 5 * row;
         const R cs2 = P.gamma * w[4] / mx(w[0], 1e-30);
         R denom = 0;
         for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
@@ -90,13 +90,13 @@ template <class R> __device__ R cell_dt(const DevMesh &d, const Params &P, int32
 // cell_dt for a cell whose N or fewer faces are given as (|A|, area vector) rows: the face normal and the normal
 // Alfven term, which do not depend on the species, are formed once (same operations as cell_dt)
 template <class R, int N>
-__device__ R cell_dt_rows(const Params &P, double vol, const SpCT<R> &Ws, const V3T<R> &b, size_t row, int nk, const double (&A)[N],
+__device__ R cell_dt_rows(const Params &P, double vol, const R *const *Ws, const V3T<R> &b, size_t row, int nk, const double (&A)[N],
                           const V3 (&area)[N])
 {
     R dt = __builtin_inf();
     const R B2 = dot(b, b);
-    R rho_sum = Ws.p[0][5 * row];
-    for (int s = 1; s < P.ns; ++s) rho_sum = rho_sum + Ws.p[s][5 * row];
+    R rho_sum = Ws[0][5 * row];
+    for (int s = 1; s < P.ns; ++s) rho_sum = rho_sum + Ws[s][5 * row];
     const R rho_tot = mx(rho_sum, P.rho_floor_dt);
     const R vA2 = P.inver_MA2 * B2 / rho_tot;
     V3T<R> n[N];
@@ -110,7 +110,7 @@ __device__ R cell_dt_rows(const Params &P, double vol, const SpCT<R> &Ws, const 
         }
     for (int s = 0; s < P.ns; ++s)
     {
-        const R *w = Ws.p[s] + 5 * row;
+        const R *w = Ws[s] + 5 * row;
         const R cs2 = P.gamma * w[4] / mx(w[0], 1e-30);
         R denom = 0;
 #pragma unroll

@@ -62,6 +62,8 @@ struct mhdg_ctx
     int restart = 2;
     mhdg::RedScratch rs{};
     mhdg::MixedSolver *mixed = nullptr;
+    // pinned host memory: the step's time step at offset 0, small read-backs staged at kPinStageOffset
+    static constexpr size_t kPinBytes = 4096, kPinStageOffset = 2048, kPinStageBytes = 2048;
     void *host_pin = nullptr;
     std::vector<void *> owned;
     std::map<int, std::pair<void *, size_t>> staging;

@@ -74,6 +74,16 @@ double *mhdg_ctx::stage_in(int slot, const void *host, size_t n_doubles)
 }
 void mhdg_ctx::d2h(void *host, const void *dev, size_t bytes)
 {
+    if (host_pin && bytes <= kPinStageBytes)
+    {
+        // small read-backs (reduction results, Hessenberg columns: one per Krylov step) through pinned memory,
+        // which saves the runtime's staging of a copy into pageable memory
+        char *pin = static_cast<char *>(host_pin) + kPinStageOffset;
+        hip_check(hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, stream), "D2H");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        std::memcpy(host, pin, bytes);
+        return;
+    }
     hip_check(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream), "D2H");
     hip_check(hipStreamSynchronize(stream), "sync");
 }
@@ -256,7 +266,7 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
     hip_check(hipMemcpy(Vptrs, V.data(), sizeof(double *) * V.size(), hipMemcpyHostToDevice), "upload");
     hcol = D(restart + 2);
     ydev = D(restart + 1);
-    hip_check(hipHostMalloc(&host_pin, 4096), "hipHostMalloc");
+    hip_check(hipHostMalloc(&host_pin, kPinBytes), "hipHostMalloc");
     hip_check(hipDeviceSynchronize(), "sync");
 }
 

@@ -304,9 +304,10 @@ void mhdg_ctx::apply_op(const double *phi_in, const double *v, const double *inv
         if (dte_cur) dt = 1.0; // d1 already carries dt_e: y = x + F (ImplicitInduction.cpp:196)
     }
     const bool ideal_part = ideal_cfg && u_ideal;
-    if (op_hall || ideal_part) LAUNCH("bcell", k_bcell, dim3(nblk(nc)), dim3(256), 0, stream, d, phi, bcell);
+    const int64_t lanes = kCellLanes64 * int64_t(nc);
+    if (op_hall || ideal_part) LAUNCH("bcell", k_bcell_g<kCellLanes64>, dim3(nblk(lanes)), dim3(256), 0, stream, d, phi, bcell);
     if (op_hall && lin_b) LAUNCH("hall_g_tangent", k_hall_g_tangent, dim3(nblk(nc)), dim3(256), 0, stream, d, lin_b, bcell, alpha, Badd, gcell);
-    else if (op_hall) LAUNCH("hall_g", k_hall_g, dim3(nblk(nc)), dim3(256), 0, stream, d, bcell, alpha, B0, gcell);
+    else if (op_hall) LAUNCH("hall_g", k_hall_g_g<kCellLanes64>, dim3(nblk(lanes)), dim3(256), 0, stream, d, bcell, alpha, B0, gcell);
     if (ideal_part)
     {
         // add_ideal_emf of (u, b, phi) with b as total and induced field (ImplicitInduction.cpp:150-157)

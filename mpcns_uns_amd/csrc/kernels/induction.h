@@ -20,6 +20,9 @@ __global__ void k_div_scalar(const double *r, const double *beta, int32_t n, dou
 __global__ void k_d1(DevMesh d, const double *v, const double *invdiag, const int8_t *dir, const double *dte, double *phi);
 __global__ void k_bcell(DevMesh d, const double *phi, double *b);
 __global__ void k_hall_g(DevMesh d, const double *b, const double *alpha, const double *B0, double *g);
+constexpr int kCellLanes64 = 4; // lanes per cell of the FP64 cell kernels of apply_F
+template <int G> __global__ void k_bcell_g(DevMesh d, const double *phi, double *b);
+template <int G> __global__ void k_hall_g_g(DevMesh d, const double *b, const double *alpha, const double *B0, double *g);
 __global__ void k_edge_apply(DevMesh d, int32_t resistive, int32_t hall, const double *phi, const double *g, const double *ide,
                              const int8_t *imask, const double *v, const double *invdiag, const int8_t *dir, double dt, int32_t mode,
                              int32_t mask, const double *bvec, double *out);

@@ -225,6 +225,73 @@ __global__ void __launch_bounds__(256) k_hall_g(DevMesh d, const double *b, cons
     st3(g, c, out);
 }
 
+// k_bcell and k_hall_g with G lanes per cell: lane q forms the terms of entries base + q (the lanes' table and
+// gather loads side by side), and every lane adds the group's terms in entry order through shuffles, so each sum
+// runs in the order of the one-thread kernels and the results are the same bit for bit.
+template <int G> __global__ void __launch_bounds__(256) k_bcell_g(DevMesh d, const double *phi, double *b)
+{
+    const int32_t t = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    const int32_t c = t / G, lane = t % G;
+    const bool ok = c < d.nc;
+    const int32_t k0 = ok ? d.c2f_ptr[c] : 0, k1 = ok ? d.c2f_ptr[c + 1] : 0;
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    for (int32_t base = k0; base < k1; base += G)
+    {
+        const int32_t k = base + lane;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        if (k < k1)
+        {
+            const double v = phi[pid(d.c2f[k])];
+            px = d.recon[3 * size_t(k)] * v; py = d.recon[3 * size_t(k) + 1] * v; pz = d.recon[3 * size_t(k) + 2] * v;
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+        {
+            const double qx = __shfl(px, q, G), qy = __shfl(py, q, G), qz = __shfl(pz, q, G);
+            if (base + q < k1) { bx += qx; by += qy; bz += qz; }
+        }
+    }
+    if (ok && lane == 0) { b[3 * size_t(c)] = bx; b[3 * size_t(c) + 1] = by; b[3 * size_t(c) + 2] = bz; }
+}
+template <int G> __global__ void __launch_bounds__(256) k_hall_g_g(DevMesh d, const double *b, const double *alpha, const double *B0, double *g)
+{
+    const int32_t t = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    const int32_t c = t / G, lane = t % G;
+    const bool ok = c < d.nc;
+    const double a = ok ? alpha[c] : 0.0;
+    V3 out;
+    if (ok && a != 0.0)
+    {
+        V3 sum;
+        const V3 bc = L3(b, c);
+        const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1];
+        for (int32_t base = k0; base < k1; base += G)
+        {
+            const int32_t k = base + lane;
+            V3 t3;
+            if (k < k1)
+            {
+                const int32_t v = d.c2f[k], f = pid(v);
+                V3 bf = bc;
+                const int32_t l = d.fL[f], r = d.fR[f];
+                if (l >= 0 && r >= 0) bf = 0.5 * (bc + L3(b, l == c ? r : l));
+                t3 = cross(psg(v) * L3(d.farea, f), bf);
+            }
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+            {
+                const V3 tq(__shfl(t3.x, q, G), __shfl(t3.y, q, G), __shfl(t3.z, q, G));
+                if (base + q < k1) sum += tq;
+            }
+        }
+        const V3 J = (1.0 / d.cvol[c]) * sum;
+        out = a * cross(J, L3(B0, c));
+    }
+    if (ok && lane == 0) st3(g, c, out);
+}
+template __global__ void k_bcell_g<kCellLanes64>(DevMesh, const double *, double *);
+template __global__ void k_hall_g_g<kCellLanes64>(DevMesh, const double *, const double *, const double *, double *);
+
 // Edge assembly of apply_F (ImplicitInduction.cpp:139-182): part0 = sum_f f2e star2 Phi (faces ascending),
 // part1 = sum_c g_c . (w_ec t_e) (cells ascending), part2 = imask ? ide : 0 (implicit ideal part, ide nullable),
 // F = m part0 + part1 + part2.

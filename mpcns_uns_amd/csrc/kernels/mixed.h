@@ -28,9 +28,7 @@ constexpr int kCellLanes = 4; // lanes per cell of the FP32 cell kernels
 template <int G> __global__ void k_bcell_f32_g(DevMesh d, DevMesh32 s, const float *phi, float *b);
 template <int G> __global__ void k_hall_g_f32_g(DevMesh d, DevMesh32 s, const float *b, float *g);
 __global__ void k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi, const float *g, const float *v,
-                                 const float *pv, const int8_t *dir, float dt, float *out);
-__global__ void k_edge_apply_f32_heavy(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const int32_t *heavy, const float *phi,
-                                       const float *g, const float *v, const float *pv, const int8_t *dir, float dt, float *out);
+                                 const float *pv, const int8_t *dir, float dt, float *out, const int32_t *heavy, int32_t n_light);
 __global__ void k_tables_f32(DevMesh d, float *aV, int32_t *nbr);
 __global__ void k_multidot_f32(const float *const *V, int32_t i0, int32_t nv, const float *w, int32_t n, double *part, int32_t with_ww,
                                RedScratch rs, double *out);

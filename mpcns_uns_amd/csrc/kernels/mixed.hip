@@ -32,9 +32,9 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // Block index that gives each XCD one contiguous range of blocks (the hardware runs block b on XCD b % 8), so that
 // neighbouring edges share an L2 for their gathers: 4 % faster for the edge kernel, slower for the cell kernels
-__device__ __forceinline__ int32_t xcd_block()
+__device__ __forceinline__ int32_t xcd_block(uint32_t nb)
 {
-    const uint32_t nb = gridDim.x, b = blockIdx.x, per = nb / 8;
+    const uint32_t b = blockIdx.x, per = nb / 8;
     if (b >= 8 * per) return int32_t(b);
     return int32_t((b % 8) * per + b / 8);
 }
@@ -74,20 +74,54 @@ __global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, DevMesh32 s, const fl
     phi[f] = sum;
 }
 
+// One high-valence edge (the polar axes) per block: terms in parallel, tree sums (order-free in FP32)
+__device__ __forceinline__ void edge_apply_f32_heavy(const DevMesh &d, const DevMesh32 &s, int32_t resistive, int32_t hall, int32_t e,
+                                                     const float *phi, const float *g, const float *v, const float *pv, const int8_t *dir,
+                                                     float dt, float *out)
+{
+    __shared__ float sh[2][4];
+    float p0 = 0.0f, p1 = 0.0f;
+    if (resistive)
+        for (int32_t k = d.e2f_ptr[e] + threadIdx.x; k < d.e2f_ptr[e + 1]; k += 256)
+        {
+            const int32_t q = d.e2f[k], f = pid(q);
+            p0 += fsg(q) * (s.star2[f] * phi[f]);
+        }
+    if (hall)
+    {
+        const F3 t = f3(s.evec, e);
+        for (int32_t k = d.ec_ptr[e] + threadIdx.x; k < d.ec_ptr[e + 1]; k += 256) p1 += s.ecw[k] * fdot(f3(g, d.ec_cell[k]), t);
+    }
+    for (int o = 32; o > 0; o >>= 1) { p0 += __shfl_down(p0, o, 64); p1 += __shfl_down(p1, o, 64); }
+    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = p0; sh[1][threadIdx.x >> 6] = p1; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        const float P0 = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3], P1 = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+        out[e] = dir[e] ? 0.0f : (pv[e] - v[e]) + dt * (s.m_e[e] * P0 + P1);
+    }
+}
+
 // out = dir ? 0 : (A P - I) v = (pv - v) + dt F(Phi') with F = m_e sum_f s star2_f Phi'_f + sum_c w_ec g_c . t_e
 // (the shifted operator: same Krylov space as A P, and the O(1) identity part never enters the FP32 vector)
 // Each thread issues the (index, weight) loads of up to kEdgeUnroll entries together and then their gathers
 // together (the gather depends on the index, so a loop of one entry at a time keeps about one load in flight per
 // wave: 403 against 288 us per launch on the 8M shell); the sums still run in entry order.
+// Blocks from n_light on take one high-valence edge each (heavy[block - n_light]) in the same launch.
 __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi,
                                                         const float *g, const float *v, const float *pv, const int8_t *dir, float dt,
-                                                        float *out)
+                                                        float *out, const int32_t *heavy, int32_t n_light)
 {
-    const int32_t e = xcd_block() * int32_t(blockDim.x) + int32_t(threadIdx.x);
+    if (int32_t(blockIdx.x) >= n_light)
+    {
+        edge_apply_f32_heavy(d, s, resistive, hall, heavy[blockIdx.x - n_light], phi, g, v, pv, dir, dt, out);
+        return;
+    }
+    const int32_t e = xcd_block(uint32_t(n_light)) * int32_t(blockDim.x) + int32_t(threadIdx.x);
     if (e >= d.ne) return;
     if (dir[e]) { out[e] = 0.0f; return; }
     const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
-    if (f1 - f0 > kLightMax) return; // k_edge_apply_f32_heavy
+    if (f1 - f0 > kLightMax) return; // one of the heavy blocks
     float p0 = 0.0f, p1 = 0.0f;
     if (resistive)
         for (int32_t k = f0; k < f1; k += kEdgeUnroll)
@@ -199,34 +233,6 @@ template __global__ void k_bcell_f32_g<4>(DevMesh, DevMesh32, const float *, flo
 template __global__ void k_hall_g_f32_g<4>(DevMesh, DevMesh32, const float *, float *);
 
 // The same for the high-valence edges, one block per edge (tree sums: order-free in FP32)
-__global__ void __launch_bounds__(256) k_edge_apply_f32_heavy(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const int32_t *heavy,
-                                                              const float *phi, const float *g, const float *v, const float *pv,
-                                                              const int8_t *dir, float dt, float *out)
-{
-    __shared__ float sh[2][4];
-    const int32_t e = heavy[blockIdx.x];
-    float p0 = 0.0f, p1 = 0.0f;
-    if (resistive)
-        for (int32_t k = d.e2f_ptr[e] + threadIdx.x; k < d.e2f_ptr[e + 1]; k += 256)
-        {
-            const int32_t q = d.e2f[k], f = pid(q);
-            p0 += fsg(q) * (s.star2[f] * phi[f]);
-        }
-    if (hall)
-    {
-        const F3 t = f3(s.evec, e);
-        for (int32_t k = d.ec_ptr[e] + threadIdx.x; k < d.ec_ptr[e + 1]; k += 256) p1 += s.ecw[k] * fdot(f3(g, d.ec_cell[k]), t);
-    }
-    for (int o = 32; o > 0; o >>= 1) { p0 += __shfl_down(p0, o, 64); p1 += __shfl_down(p1, o, 64); }
-    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = p0; sh[1][threadIdx.x >> 6] = p1; }
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        const float P0 = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3], P1 = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
-        out[e] = dir[e] ? 0.0f : (pv[e] - v[e]) + dt * (s.m_e[e] * P0 + P1);
-    }
-}
-
 // Operator table of the FP32 path: per (cell, face) s A_f / V_c and the neighbour across the face
 __global__ void __launch_bounds__(256) k_tables_f32(DevMesh d, float *aV, int32_t *nbr)
 {
@@ -721,11 +727,8 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
                 LAUNCH("bcell_f32", k_bcell_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.phi, M.b);
                 LAUNCH("hall_g_f32", k_hall_g_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.b, M.g);
             }
-            LAUNCH("edge_apply_f32", k_edge_apply_f32, dim3(nb(ne)), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall), M.phi,
-                   M.g, M.V[j], M.pv, dir, dtf, M.w);
-            if (n_heavy)
-                LAUNCH("edge_apply_f32_heavy", k_edge_apply_f32_heavy, dim3(n_heavy), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall),
-                       static_cast<const int32_t *>(heavy), M.phi, M.g, M.V[j], M.pv, dir, dtf, M.w);
+            LAUNCH("edge_apply_f32", k_edge_apply_f32, dim3(nb(ne) + n_heavy), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall),
+                   M.phi, M.g, M.V[j], M.pv, dir, dtf, M.w, static_cast<const int32_t *>(heavy), int32_t(nb(ne)));
             // classical Gram-Schmidt against V_0..V_j; the update also normalises V_{j+1} unless the DGKS test asks
             // for a second projection
             for (int i0 = 0; i0 <= j; i0 += kDotChunk)

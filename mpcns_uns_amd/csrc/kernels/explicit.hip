@@ -632,7 +632,7 @@ template <class R> __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, 
 #pragma unroll
             for (int k = 0; k < kEmfFast; ++k)
                 if (k < nc) uc[k] = L3(u_cell, cid[k]);
-            R fqa[kEmfFast], fqb[kEmfFast], fw[kEmfFast], faef[kEmfFast], fsig[kEmfFast], fpf[kEmfFast];
+            R fqa[kEmfFast], fqb[kEmfFast], faef[kEmfFast], fsig[kEmfFast];
 #pragma unroll
             for (int i = 0; i < kEmfFast; ++i)
                 if (i < nf)
@@ -652,16 +652,17 @@ template <class R> __global__ void __launch_bounds__(256) k_edge_emf(DevMesh d, 
                     M22 += w * qb * qb;
                     g1 += w * qa;
                     g2 += w * qb;
-                    fqa[i] = qa; fqb[i] = qb; fw[i] = w; faef[i] = aef; fsig[i] = sig; fpf[i] = phi_ind[f] / d.famag[f];
+                    fqa[i] = qa; fqb[i] = qb; faef[i] = aef; fsig[i] = sig;
                 }
             R l1, l2;
             uct_solve(M11, M12, M22, g1, g2, l1, l2);
+            // the face weight and flux are loaded again (cache hits) rather than held across the solve
 #pragma unroll
             for (int i = 0; i < kEmfFast; ++i)
                 if (i < nf)
                 {
-                    const R omega = fw[i] * (1.0 - (fqa[i] * l1 + fqb[i] * l2));
-                    Eu += omega * faef[i] * fsig[i] * fpf[i];
+                    const R omega = d.efw[f0 + i] * (1.0 - (fqa[i] * l1 + fqb[i] * l2));
+                    Eu += omega * faef[i] * fsig[i] * (phi_ind[fid[i]] / d.famag[fid[i]]);
                 }
         }
         else

@@ -129,7 +129,7 @@ __device__ __forceinline__ void stencil_ghost(const DevMesh &d, int32_t sp, cons
 // Cells of at most kGradFast faces load their face and stencil rows once, together, before the gathers, and
 // the limiter's face offsets once for every species; the operations and their order are those of the general
 // path.
-template <class R> __global__ void __launch_bounds__(256) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads,
+template <class R> __global__ void __launch_bounds__(256, 3) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads,
                                                       int32_t limit, const R *Bind, R *J)
 {
     __shared__ R stage[256 * 15];

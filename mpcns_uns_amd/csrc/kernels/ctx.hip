@@ -454,8 +454,12 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     (void)hipEventDestroy(ev);
     const double gdt = dth[0];
     if (!std::isfinite(gdt) || gdt <= 0) throw std::runtime_error("invalid time step");
-    LAUNCH("cell_update", k_cell_update<double>, dim3(nb), dim3(256), 0, stream, d, P, P.ns, 0, static_cast<const double *>(FA), none, Wc,
-           static_cast<const double *>(Btot), static_cast<const double *>(J), static_cast<const double *>(dt_dev), 0.0, 1, 1, Us, noneW);
+    if (P.ns > 2)
+        LAUNCH("cell_update", k_cell_update_many<double>, dim3(nb), dim3(256), 0, stream, d, P, P.ns, static_cast<const double *>(FA), Wc,
+               static_cast<const double *>(Btot), static_cast<const double *>(J), static_cast<const double *>(dt_dev), Us);
+    else
+        LAUNCH("cell_update", k_cell_update<double>, dim3(nb), dim3(256), 0, stream, d, P, P.ns, 0, static_cast<const double *>(FA), none, Wc,
+               static_cast<const double *>(Btot), static_cast<const double *>(J), static_cast<const double *>(dt_dev), 0.0, 1, 1, Us, noneW);
     launch_edge_emf(Ef, uplus, Phi, P.cuct, nullptr, E_expl, 1);
     *iters = 0;
     *res = 0.0;

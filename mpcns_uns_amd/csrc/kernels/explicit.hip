@@ -571,6 +571,74 @@ template <class R> __global__ void __launch_bounds__(256) k_cell_update(DevMesh 
 }
 
 
+// k_cell_update of the fused step for more than two ions (rhs_in, rhs_out none, s0 = 0, sources and update on):
+// every species' flux sum is gathered in one pass over the cell's faces, so each face row (5 ns values) is read
+// once; with one pass per species the rows' lines were fetched again for every species (3x the algorithmic bytes
+// for ten ions).  Each species' sum runs over the faces in the same ascending order, so the results are those of
+// k_cell_update bit for bit.
+template <class R> __global__ void __launch_bounds__(256) k_cell_update_many(DevMesh d, Params P, int32_t ns, const R *FA, SpCT<R> Wall,
+                                                                            const R *Btot, const R *J, const R *dt_dev, SpWT<R> Us)
+{
+    const int32_t c = gid();
+    if (c >= d.nc) return;
+    const bool fluid = d.mat[c] == 0;
+    const R dt = dt_dev[0];
+    const R V = d.cvol[c];
+    const R ne_c = fluid ? electron_density(P, Wall.p, c) : R(0.0);
+    R rhs[kMaxSpecies][5];
+#pragma unroll
+    for (int s = 0; s < kMaxSpecies; ++s)
+        for (int m = 0; m < 5; ++m) rhs[s][m] = 0;
+    if (fluid)
+        for (int32_t k = d.c2f_ptr[c]; k < d.c2f_ptr[c + 1]; ++k)
+        {
+            const int32_t v = d.cfs[k];
+            const R *fa = FA + size_t(5 * ns) * pid(v);
+#pragma unroll
+            for (int s = 0; s < kMaxSpecies; ++s)
+                if (s < ns)
+                {
+                    if (v >= 0) for (int m = 0; m < 5; ++m) rhs[s][m] -= fa[5 * s + m] / V;
+                    else for (int m = 0; m < 5; ++m) rhs[s][m] += fa[5 * s + m] / V;
+                }
+        }
+#pragma unroll
+    for (int s = 0; s < kMaxSpecies; ++s)
+        if (s < ns && fluid)
+        {
+            R *r = rhs[s];
+            // mercury::add_sources for ion s (MercurySources.cpp:20-63), as in k_cell_update
+            const R *W = Wall.p[s];
+            const R ws = ion_density(P, s, W[5 * size_t(c)]) / sqrt(ne_c * ne_c + P.ne_floor * P.ne_floor);
+            const R rho = mx(W[5 * size_t(c)], 0.0);
+            const V3T<R> u(W[5 * size_t(c) + 1], W[5 * size_t(c) + 2], W[5 * size_t(c) + 3]);
+            V3T<R> S;
+            if (P.lorentz) S += (P.inver_MA2 * ws) * cross(L3(J, c), L3(Btot, c));
+            r[1] += S.x; r[2] += S.y; r[3] += S.z;
+            r[4] += dot(u, S) + (rho > 1e-30 ? 0.5 * dt * dot(S, S) / rho : 0.0);
+            if (P.src_rate[s] != 0.0 && d.atm)
+            {
+                const double q = P.src_rate[s] * d.atm[c];
+                r[0] += q;
+                r[4] += q * P.src_T / (P.gamma - 1.0);
+            }
+            // forward Euler with floors (Solver.cpp:246-252, 171-191), as in k_cell_update
+            R *u0 = Us.p[s] + 5 * size_t(c);
+            R q[5];
+            for (int m = 0; m < 5; ++m) q[m] = u0[m] + dt * r[m];
+            const R ke = q[0] > 0 ? 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0] : 0.0;
+            if (!(q[0] >= P.rho_floor)) to_cons(PrimT<R>{P.rho_floor, 0, 0, 0, P.p_floor}, P.gamma, q);
+            else if ((P.gamma - 1.0) * (q[4] - ke) < P.p_floor)
+            {
+                const PrimT<R> w = to_prim(q, P.gamma, P.rho_floor, P.p_floor);
+                to_cons(w, P.gamma, q);
+            }
+            for (int m = 0; m < 5; ++m) u0[m] = q[m];
+        }
+}
+template __global__ void k_cell_update_many<double>(DevMesh d, Params P, int32_t ns, const double *FA, SpCT<double> Wall, const double *Btot,
+                                                    const double *J, const double *dt_dev, SpWT<double> Us);
+
 // add_ideal_emf edge loop (Common/induction/IdealEMF.cpp:59-128), one thread per edge with valence at most
 // `vmax`; larger edges are left to k_edge_emf_heavy.  E_out[e] = E_in[e] + E_ideal (E_in nullable = 0);
 // apply_bc additionally applies the boundary EMF of Solver::rhs.

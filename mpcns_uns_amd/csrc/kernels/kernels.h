@@ -26,6 +26,8 @@ template <class R> __global__ void k_face(DevMesh d, Params P, int32_t ns, int32
 template <class R> __global__ void k_cell_update(DevMesh d, Params P, int32_t ns, int32_t s0, const R *FA, SpCT<R> rhs_in, SpCT<R> Wall,
                                                  const R *Btot, const R *J, const R *dt_dev, double dt_val, int32_t sources, int32_t update,
                                                  SpWT<R> Us, SpWT<R> rhs_out);
+template <class R> __global__ void k_cell_update_many(DevMesh d, Params P, int32_t ns, const R *FA, SpCT<R> Wall, const R *Btot, const R *J,
+                                                     const R *dt_dev, SpWT<R> Us);
 __global__ void k_cell_vector(DevMesh d, const double *flux, double *b);
 __global__ void k_local_dt(DevMesh d, Params P, SpC Ws, const double *Btot, double *dt_part);
 template <class R> __global__ void k_edge_emf(DevMesh d, Params P, const R *Ef, const R *u_cell, const R *phi_ind, double cuct, int32_t vmax,

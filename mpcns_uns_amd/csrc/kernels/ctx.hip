@@ -433,7 +433,11 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
         Us.p[s] = U[s];
     }
     if (P.second_order)
-        LAUNCH("grad_current", k_grad_current<double>, dim3(nb), dim3(256), 0, stream, d, P, P.ns, 0, Ws, none, grads, 1, Bind, J);
+    {
+        // many species: the stencil rows loaded once for all of them
+        auto kgrad = P.ns > 2 ? k_grad_current<double, true> : k_grad_current<double, false>;
+        LAUNCH("grad_current", kgrad, dim3(nb), dim3(256), 0, stream, d, P, P.ns, 0, Ws, none, grads, 1, Bind, J);
+    }
     else
     {
         for (int s = 0; s < P.ns; ++s) hip_check(hipMemsetAsync(grad[s], 0, 15 * size_t(nc) * 8, stream), "memset");

@@ -129,7 +129,7 @@ __device__ __forceinline__ void stencil_ghost(const DevMesh &d, int32_t sp, cons
 // Cells of at most kGradFast faces load their face and stencil rows once, together, before the gathers, and
 // the limiter's face offsets once for every species; the operations and their order are those of the general
 // path.
-template <class R> __global__ void __launch_bounds__(256, 3) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads,
+template <class R, bool HOIST> __global__ void __launch_bounds__(256, 3) k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads,
                                                       int32_t limit, const R *Bind, R *J)
 {
     __shared__ R stage[256 * 15];
@@ -192,6 +192,14 @@ template <class R> __global__ void __launch_bounds__(256, 3) k_grad_current(DevM
 #pragma unroll
         for (int i = 0; i < kGradFast; ++i)
             if (i < nk) dxk[i] = L3(d.fcen, pid(vk[i])) - xc;
+    // HOIST (many species): the stencil rows too are loaded once for every species
+    int32_t nbh[kGradFast];
+    double swh[kGradFast];
+    V3 sdh[kGradFast];
+    if (HOIST && fast)
+#pragma unroll
+        for (int i = 0; i < kGradFast; ++i)
+            if (i < nk) { nbh[i] = d.snb[k0 + i]; swh[i] = d.sw[k0 + i]; sdh[i] = L3(d.sd, k0 + i); }
     // species s of the block: blockIdx.y, blockIdx.y + gridDim.y, ... (one grid row, every species, measured
     // faster than one row per species)
     for (int32_t s = int32_t(blockIdx.y); s < ns; s += int32_t(gridDim.y))
@@ -209,8 +217,8 @@ template <class R> __global__ void __launch_bounds__(256, 3) k_grad_current(DevM
         R wc[5], r[5][3], lo[5], hi[5];
         for (int m = 0; m < 5; ++m) { wc[m] = W[5 * size_t(c) + m]; lo[m] = hi[m] = wc[m]; r[m][0] = r[m][1] = r[m][2] = 0.0; }
         auto accumulate = [&](int32_t k, const R *wn) {
-            const R w = d.sw[k];
-            const V3T<R> dd = L3(d.sd, k);
+            const R w = HOIST && fast ? R(swh[k - k0]) : R(d.sw[k]);
+            const V3T<R> dd = HOIST && fast ? V3T<R>(sdh[k - k0]) : V3T<R>(L3(d.sd, k));
             for (int m = 0; m < 5; ++m)
             {
                 const R dq = wn[m] - wc[m];
@@ -224,7 +232,7 @@ template <class R> __global__ void __launch_bounds__(256, 3) k_grad_current(DevM
             int32_t nbk[kGradFast];
 #pragma unroll
             for (int i = 0; i < kGradFast; ++i)
-                if (i < nk) nbk[i] = d.snb[k0 + i];
+                if (i < nk) nbk[i] = HOIST ? nbh[i] : d.snb[k0 + i];
 #pragma unroll
             for (int i = 0; i < kGradFast; ++i)
                 if (i < nk && nbk[i] != kNoStencil)
@@ -968,7 +976,8 @@ template __global__ void k_shell_energy<double>(DevMesh, const double *, const d
 template __global__ void k_shell_energy<Dual>(DevMesh, const Dual *, const Dual *, double, double, Dual *);
 template __global__ void k_prepare<double>(DevMesh d, Params P, SpCT<double> Us, const double *Phi, const double *Badd, SpWT<double> Ws, double *Bind, double *Btot, double *uplus, double *alpha, double *dt_part);
 template __global__ void k_dt_final<double>(const double *part, int32_t n, double dt_max, double *out);
-template __global__ void k_grad_current<double>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> ghosts, SpWT<double> grads, int32_t limit, const double *Bind, double *J);
+template __global__ void k_grad_current<double, false>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> ghosts, SpWT<double> grads, int32_t limit, const double *Bind, double *J);
+template __global__ void k_grad_current<double, true>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> ghosts, SpWT<double> grads, int32_t limit, const double *Bind, double *J);
 template __global__ void k_face_species<double>(DevMesh d, Params P, int32_t ns, SpCT<double> Ws, SpCT<double> grads, double *FA);
 template __global__ void k_face<double>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<double> Ws, SpCT<double> grads, SpCT<double> ghosts, double *FA, const double *uplus, const double *Btot, const double *Bind, const double *Phi, const double *PhiAdd, double *Ef);
 template __global__ void k_cell_update<double>(DevMesh d, Params P, int32_t ns, int32_t s0, const double *FA, SpCT<double> rhs_in, SpCT<double> Wall, const double *Btot, const double *J, const double *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<double> Us, SpWT<double> rhs_out);
@@ -977,7 +986,7 @@ template __global__ void k_edge_emf_heavy<double>(DevMesh d, Params P, const dou
 template __global__ void k_faraday<double>(DevMesh d, const double *E, const double *dte, const double *dt_dev, double dt_val, double *B);
 template __global__ void k_prepare<Dual>(DevMesh d, Params P, SpCT<Dual> Us, const Dual *Phi, const Dual *Badd, SpWT<Dual> Ws, Dual *Bind, Dual *Btot, Dual *uplus, Dual *alpha, Dual *dt_part);
 template __global__ void k_dt_final<Dual>(const Dual *part, int32_t n, double dt_max, Dual *out);
-template __global__ void k_grad_current<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<Dual> Ws, SpCT<Dual> ghosts, SpWT<Dual> grads, int32_t limit, const Dual *Bind, Dual *J);
+template __global__ void k_grad_current<Dual, false>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<Dual> Ws, SpCT<Dual> ghosts, SpWT<Dual> grads, int32_t limit, const Dual *Bind, Dual *J);
 template __global__ void k_face<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<Dual> Ws, SpCT<Dual> grads, SpCT<Dual> ghosts, Dual *FA, const Dual *uplus, const Dual *Btot, const Dual *Bind, const Dual *Phi, const Dual *PhiAdd, Dual *Ef);
 template __global__ void k_cell_update<Dual>(DevMesh d, Params P, int32_t ns, int32_t s0, const Dual *FA, SpCT<Dual> rhs_in, SpCT<Dual> Wall, const Dual *Btot, const Dual *J, const Dual *dt_dev, double dt_val, int32_t sources, int32_t update, SpWT<Dual> Us, SpWT<Dual> rhs_out);
 template __global__ void k_edge_emf<Dual>(DevMesh d, Params P, const Dual *Ef, const Dual *u_cell, const Dual *phi_ind, double cuct, int32_t vmax, const Dual *E_in, Dual *E_out, int32_t apply_bc);

@@ -17,7 +17,7 @@ template <class R> constexpr int32_t kHeavyShared() { return sizeof(R) == sizeof
 template <class R> __global__ void k_prepare(DevMesh d, Params P, SpCT<R> Us, const R *Phi, const R *Badd, SpWT<R> Ws, R *Bind, R *Btot,
                                              R *uplus, R *alpha, R *dt_part);
 template <class R> __global__ void k_dt_final(const R *part, int32_t n, double dt_max, R *out);
-template <class R> __global__ void k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads, int32_t limit,
+template <class R, bool HOIST = false> __global__ void k_grad_current(DevMesh d, Params P, int32_t ns, int32_t s0, SpCT<R> Ws, SpCT<R> ghosts, SpWT<R> grads, int32_t limit,
                                const R *Bind, R *J);
 __global__ void k_gyrate(DevMesh d, Params P, double *UH, double *UNa, const double *Btot, const double *dt_dev, double scale);
 template <class R> __global__ void k_face_species(DevMesh d, Params P, int32_t ns, SpCT<R> Ws, SpCT<R> grads, R *FA);

@@ -209,6 +209,16 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
     d.f2e_ptr = upload(t.f2e_ptr); d.f2e = upload(t.f2e);
     d.e2f_ptr = upload(t.e2f_ptr); d.e2f = upload(t.e2f);
     d.fL = upload(t.fL); d.fR = upload(t.fR);
+    {
+        std::vector<int32_t> c2n(t.c2f.size());
+        for (int32_t c = 0; c < nc; ++c)
+            for (int64_t k = t.c2f_ptr[c]; k < t.c2f_ptr[c + 1]; ++k)
+            {
+                const int32_t v = t.c2f[k], f = v >= 0 ? v : ~v, l = t.fL[f], r = t.fR[f];
+                c2n[k] = (l >= 0 && r >= 0) ? (l == c ? r : l) : -1;
+            }
+        d.c2n = upload(c2n);
+    }
     d.ec_ptr = upload(t.ec_ptr); d.ec_cell = upload(t.ec_cell);
     d.mat = upload(t.mat8);
     d.kind = upload(m.face_kind, F); d.emode = upload(m.edge_mode, E);

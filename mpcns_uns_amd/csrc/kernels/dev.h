@@ -33,6 +33,7 @@ struct DevMesh
     const int32_t *f2e_ptr, *f2e;   // face -> edges, loop order
     const int32_t *e2f_ptr, *e2f;   // edge -> faces ascending (sign of the f2e entry)
     const int32_t *fL, *fR;         // cell seeing the face outward (+1) / inward (-1); -1 if none
+    const int32_t *c2n;             // per c2f entry: the cell across that face (-1 unless both sides are cells)
     const int32_t *ec_ptr, *ec_cell;
     const int8_t *mat, *kind, *emode;
     const double *farea, *fcen, *famag, *ccen, *cvol, *evec, *emid, *elen, *ecw, *efw, *star2;

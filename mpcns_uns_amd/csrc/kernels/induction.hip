@@ -271,10 +271,10 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_g(DevMesh d, co
             V3 t3;
             if (k < k1)
             {
-                const int32_t v = d.c2f[k], f = pid(v);
+                // the neighbour from the per-entry table: its gather waits on one load, not on the face's f2c
+                const int32_t v = d.c2f[k], f = pid(v), o = d.c2n[k];
                 V3 bf = bc;
-                const int32_t l = d.fL[f], r = d.fR[f];
-                if (l >= 0 && r >= 0) bf = 0.5 * (bc + L3(b, l == c ? r : l));
+                if (o >= 0) bf = 0.5 * (bc + L3(b, o));
                 t3 = cross(psg(v) * L3(d.farea, f), bf);
             }
 #pragma unroll

@@ -158,9 +158,8 @@ template <class R, bool HOIST> __global__ void __launch_bounds__(256, 3) k_grad_
             for (int i = 0; i < kGradFast; ++i)
                 if (i < nk)
                 {
-                    const int32_t f = pid(vk[i]), l = d.fL[f], r = d.fR[f];
-                    o[i] = (l >= 0 && r >= 0) ? (l == c ? r : l) : -1;
-                    A[i] = L3(d.farea, f);
+                    o[i] = d.c2n[k0 + i];
+                    A[i] = L3(d.farea, pid(vk[i]));
                 }
 #pragma unroll
             for (int i = 0; i < kGradFast; ++i)

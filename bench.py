@@ -159,8 +159,8 @@ def main():
                           "copy_GBps": round(8 * n / (us[2] * 1e-6) / 1e9, 1)}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; the path shards by mesh partition (see DESIGN.md), "
-                                   "multi-GPU runs are not part of it", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark measures one GPU; the step would shard by mesh partition with a halo "
+                                   "exchange per step, which is not part of this benchmark", "n_gpus": args.gpus}))
         sys.exit(2)
 
     from mpcns_uns_amd import Device, workloads

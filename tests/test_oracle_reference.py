@@ -1,7 +1,7 @@
 """The oracle and the host mesh layer against the compiled reference (oracle/_ref), bit for bit.
 
 The reference library is built from the MPCNS_Uns sources by `make -C oracle ref` where those sources are
-present; these tests skip otherwise (the GPU box has the built library but the oracle is also pinned by
+present; these tests skip otherwise (a machine without the library still pins the oracle through
 tests/test_oracle_known_answers.py, which needs no reference).
 """
 import numpy as np

@@ -796,21 +796,51 @@ template <class R> __global__ void __launch_bounds__(256) k_edge_emf_heavy(DevMe
     const V3 th = (1.0 / L) * t;
     V3 e1, e2;
     plane_basis(th, e1, e2);
-    if (active)
-        for (int32_t i = threadIdx.x; i < nf; i += blockDim.x)
+    // The edge's cell velocities are staged in LDS (in s_qa, s_qb, s_sig, before those take the per-face terms):
+    // every face's max over the cells then reads LDS, where a dependent global gather per (face, cell) pair made
+    // this kernel latency-bound (n_lon^2 pairs per pole-axis edge).  Same values, same order.
+    const int32_t nc = c1 - c0;
+    const bool staged = active && nc <= kHS;
+    if (staged)
+        for (int32_t k = threadIdx.x; k < nc; k += blockDim.x)
         {
+            const V3T<R> u = L3(u_cell, d.ec_cell[c0 + k]);
+            s_qa[k] = u.x; s_qb[k] = u.y; s_sig[k] = u.z;
+        }
+    __syncthreads();
+    constexpr int kPer = (kHS + 255) / 256; // faces per thread (nf <= kHS, 256 threads)
+    R qa_r[kPer], qb_r[kPer], sig_r[kPer];
+    if (active)
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+        {
+            const int32_t i = int32_t(threadIdx.x) + j * int32_t(blockDim.x);
+            if (i >= nf) break;
             const int32_t f = pid(d.e2f[f0 + i]);
             s_w[i] = d.efw[f0 + i];
             s_ec[i] = dot(L3(Ef, f), t);
             V3 m, nrm;
             face_dirs(d, f, e, th, m, nrm);
-            const R aef = edge_amax(d, u_cell, c0, c1, m);
+            R aef = 0.0;
+            if (staged)
+                for (int32_t k = 0; k < nc; ++k) aef = mx(aef, fabs(dot(V3T<R>(s_qa[k], s_qb[k], s_sig[k]), m)));
+            else
+                aef = edge_amax(d, u_cell, c0, c1, m);
             const R sig = dot(th, cross(m, nrm));
             const V3T<R> qv = (aef * sig) * nrm;
-            s_qa[i] = dot(qv, e1);
-            s_qb[i] = dot(qv, e2);
-            s_sig[i] = sig;
+            qa_r[j] = dot(qv, e1);
+            qb_r[j] = dot(qv, e2);
+            sig_r[j] = sig;
             s_aef[i] = aef;
+        }
+    __syncthreads(); // every read of the staged velocities is done
+    if (active)
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+        {
+            const int32_t i = int32_t(threadIdx.x) + j * int32_t(blockDim.x);
+            if (i >= nf) break;
+            s_qa[i] = qa_r[j]; s_qb[i] = qb_r[j]; s_sig[i] = sig_r[j];
         }
     __syncthreads();
     R Ec = 0;

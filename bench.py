@@ -59,7 +59,8 @@ def kernel_bytes(mesh, name, precision_bytes=8, ns=2):
         # FP32 operator of the mixed solve (int32 incidence, FP32 values)
         "d1_f32": 4 * (F + 1) + 4 * nf2e + 4 * E + 4 * E + E + 4 * F,
         "bcell_f32": 4 * (C + 1) + 4 * nc2f + 12 * nc2f + 4 * F + 12 * C,
-        "hall_g_f32": 4 * C + 12 * C + 4 * (C + 1) + 4 * nc2f + 8 * F + 12 * F + 4 * C + 12 * C + 12 * C,
+        # alpha B0 (3C), b (3C), c2f ptr + idx, f2c, area, volume, g (3C) write
+        "hall_g_f32": 12 * C + 4 * (C + 1) + 4 * nc2f + 8 * F + 12 * F + 4 * C + 12 * C + 12 * C,
         # dir, e2f ptr, (face, signed star2) pairs, phi, edge vec, ec ptr, (cell, weight) pairs, g, m_e, pv, v, out
         "edge_apply_f32": E + 4 * (E + 1) + 8 * nf2e + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 4,
     }

@@ -12,7 +12,7 @@ namespace mhdg
 struct DevMesh32
 {
     const float *star2, *m_e, *recon, *ecw, *evec, *invdiag;
-    float *alpha, *B0;
+    float *aB0;               // per cell alpha_c B0_c of the frozen state (3)
     const float *aV;          // per (cell, face) entry: s A_f / V_c (3)
     const int32_t *nbr;       // per (cell, face) entry: the cell across the face, -1 on the boundary
     const int2 *ecp;          // per edge-cell entry: (cell, bits of the FP32 weight), one 8-byte load
@@ -38,7 +38,7 @@ __global__ void k_scale_f32(const float *w, const double *h, const float *invdia
 __global__ void k_update_mixed(const float *const *V, const double *y, int32_t j, const float *invdiag, const int8_t *dir, int32_t n,
                                double *delta);
 __global__ void k_f64_to_f32(const double *x, int64_t n, float *out);
-__global__ void k_alpha_f32(const double *alpha, const double *B0, int32_t nc, float *alpha32, float *B032);
+__global__ void k_alpha_f32(const double *alpha, const double *B0, int32_t nc, float *aB0);
 
 struct MixedSolver
 {

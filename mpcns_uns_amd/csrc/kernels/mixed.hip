@@ -188,17 +188,19 @@ template <int G> __global__ void __launch_bounds__(256) k_bcell_f32_g(DevMesh d,
     bx = group_sum<G>(bx); by = group_sum<G>(by); bz = group_sum<G>(bz);
     if (ok && lane == 0) { b[3 * size_t(c)] = bx; b[3 * size_t(c) + 1] = by; b[3 * size_t(c) + 2] = bz; }
 }
-// g_c = alpha_c J_c x B0_c with J_c = sum_k (s A_f / V_c)_k x b_f, b_f the face average with the neighbour
+// g_c = J_c x (alpha_c B0_c) with J_c = sum_k (s A_f / V_c)_k x b_f, b_f the face average with the neighbour;
+// zero where alpha_c B0_c is
 template <int G> __global__ void __launch_bounds__(256) k_hall_g_f32_g(DevMesh d, DevMesh32 s, const float *b, float *g)
 {
     const int32_t t = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
     const int32_t c = t / G, lane = t % G;
     const bool ok = c < d.nc;
-    const float a = ok ? s.alpha[c] : 0.0f;
+    const F3 ab = ok ? f3(s.aB0, c) : F3{0.0f, 0.0f, 0.0f};
+    const bool hall = ab.x != 0.0f || ab.y != 0.0f || ab.z != 0.0f;
     F3 J{0.0f, 0.0f, 0.0f};
     if (ok)
     {
-        // the table loads do not wait for alpha; the neighbour gathers do
+        // the table loads do not wait for alpha B0; the neighbour gathers do
         int32_t k0, k1;
         rows(s.cdeg, d.c2f_ptr, c, k0, k1);
         const F3 bc = f3(b, c);
@@ -206,7 +208,7 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_f32_g(DevMesh d
         {
             const int32_t nb = s.nbr[k];
             const F3 av = f3(s.aV, k);
-            if (a == 0.0f) continue;
+            if (!hall) continue;
             F3 bf = bc;
             if (nb >= 0)
             {
@@ -220,12 +222,7 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_f32_g(DevMesh d
     J.x = group_sum<G>(J.x); J.y = group_sum<G>(J.y); J.z = group_sum<G>(J.z);
     if (ok && lane == 0)
     {
-        F3 out{0.0f, 0.0f, 0.0f};
-        if (a != 0.0f)
-        {
-            const F3 q = fcross(J, f3(s.B0, c));
-            out = {a * q.x, a * q.y, a * q.z};
-        }
+        const F3 out = hall ? fcross(J, ab) : F3{0.0f, 0.0f, 0.0f};
         g[3 * size_t(c)] = out.x; g[3 * size_t(c) + 1] = out.y; g[3 * size_t(c) + 2] = out.z;
     }
 }
@@ -572,12 +569,12 @@ __global__ void __launch_bounds__(256) k_f64_to_f32(const double *x, int64_t n, 
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = static_cast<float>(x[i]);
 }
-__global__ void __launch_bounds__(256) k_alpha_f32(const double *alpha, const double *B0, int32_t nc, float *a32, float *b32)
+// alpha_c B0_c in FP32 (formed in FP64): the FP32 Hall kernel reads one 3-vector per cell instead of alpha and B0
+__global__ void __launch_bounds__(256) k_alpha_f32(const double *alpha, const double *B0, int32_t nc, float *aB0)
 {
     const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    a32[c] = static_cast<float>(alpha[c]);
-    for (int k = 0; k < 3; ++k) b32[3 * size_t(c) + k] = static_cast<float>(B0[3 * size_t(c) + k]);
+    for (int k = 0; k < 3; ++k) aB0[3 * size_t(c) + k] = static_cast<float>(alpha[c] * B0[3 * size_t(c) + k]);
 }
 } // namespace mhdg
 
@@ -648,8 +645,7 @@ void mhdg_ctx::mixed_build()
     }
     M.pv = static_cast<float *>(alloc(E * sizeof(float)));
     M.s.invdiag = static_cast<float *>(alloc(E * sizeof(float)));
-    M.s.alpha = static_cast<float *>(alloc(C * sizeof(float)));
-    M.s.B0 = static_cast<float *>(alloc(3 * C * sizeof(float)));
+    M.s.aB0 = static_cast<float *>(alloc(3 * C * sizeof(float)));
     M.phi = static_cast<float *>(alloc(F * sizeof(float)));
     M.b = static_cast<float *>(alloc(3 * C * sizeof(float)));
     M.g = static_cast<float *>(alloc(3 * C * sizeof(float)));
@@ -683,7 +679,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
         dte32 = M.dte32;
     }
     // frozen state and Jacobi diagonal in FP32
-    LAUNCH("alpha_f32", k_alpha_f32, dim3(nb(nc)), dim3(256), 0, stream, alpha, B0, nc, M.s.alpha, M.s.B0);
+    LAUNCH("alpha_f32", k_alpha_f32, dim3(nb(nc)), dim3(256), 0, stream, alpha, B0, nc, M.s.aB0);
     LAUNCH("solve_init", k_solve_init, dim3(nb(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt, dte_cur, ne, x0,
            invdiag);
     LAUNCH("f64_to_f32", k_f64_to_f32, dim3(nb(ne)), dim3(256), 0, stream, invdiag, int64_t(ne), const_cast<float *>(M.s.invdiag));

@@ -260,7 +260,8 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_g(DevMesh d, co
     const bool ok = c < d.nc;
     const double a = ok ? alpha[c] : 0.0;
     V3 out;
-    if (ok && a != 0.0)
+    // the sum's loads do not wait for alpha (a load behind the alpha test waited for it); only the result does
+    if (ok)
     {
         V3 sum;
         const V3 bc = L3(b, c);
@@ -284,8 +285,11 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_g(DevMesh d, co
                 if (base + q < k1) sum += tq;
             }
         }
-        const V3 J = (1.0 / d.cvol[c]) * sum;
-        out = a * cross(J, L3(B0, c));
+        if (a != 0.0)
+        {
+            const V3 J = (1.0 / d.cvol[c]) * sum;
+            out = a * cross(J, L3(B0, c));
+        }
     }
     if (ok && lane == 0) st3(g, c, out);
 }

@@ -145,8 +145,15 @@ __global__ void __launch_bounds__(256) k_d1(DevMesh d, const double *v, const do
         for (int u = 0; u < kRowUnroll; ++u)
             if (k + u < k1)
             {
+                // v and invdiag gathered alongside dir, not behind its test
                 const int32_t e = pid(q[u]);
-                x[u] = invdiag ? (dir[e] ? 0.0 : v[e] * invdiag[e]) : v[e];
+                const double ve = v[e];
+                if (invdiag)
+                {
+                    const double iv = invdiag[e];
+                    x[u] = dir[e] ? 0.0 : ve * iv;
+                }
+                else x[u] = ve;
                 if (dte) t[u] = dte[e];
             }
 #pragma unroll

@@ -122,11 +122,13 @@ def lib():
 
 
 def ptr(a):
-    """Data pointer of a contiguous numpy array (None passes NULL)."""
+    """ctypes pointer to a contiguous numpy array's data (None passes NULL).  The pointer object holds a
+    reference to the array, so a temporary made for the call (ptr(f64(x))) lives until the call returns."""
     if a is None:
         return None
-    assert a.flags["C_CONTIGUOUS"], "arrays crossing the C-ABI must be contiguous"
-    return a.ctypes.data
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays crossing the C-ABI must be contiguous")
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def f64(a):

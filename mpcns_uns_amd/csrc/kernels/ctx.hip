@@ -411,6 +411,10 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
 // mercury::Solver::step_first_order (MercuryUns/driver/Solver.cpp:228-277) on the resident state
 double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
 {
+    // refusals before any launch, so that a throw leaves the resident state as it was
+    if (precision < 0 || precision > 1) throw std::runtime_error("step: precision must be 0 (FP64) or 1 (mixed)");
+    if (precision != 0 && ideal_cfg && u_ideal)
+        throw std::runtime_error("mixed-precision solve: the implicit ideal part has no FP32 operator; use precision 0");
     const int nb = nblk(nc);
     SpC Uc{}, Wc{};
     SpW Ww{};
@@ -424,10 +428,17 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     hipEvent_t ev;
     hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
     hip_check(hipEventRecord(ev, stream), "event");
-    // Strang split of the ions' gyration (Solver.cpp:237-241): half a step, then prepare again
+    // Strang split of the ions' gyration (Solver.cpp:237-241): half a step, then prepare again.  The rotation changes
+    // U, so the time step is checked first, as the reference throws before it touches the state (Solver.cpp:236).
     const bool split = P.induce_coeff > 0 && P.lorentz;
     if (split)
     {
+        hip_check(hipEventSynchronize(ev), "event sync");
+        if (!std::isfinite(dth[0]) || dth[0] <= 0)
+        {
+            (void)hipEventDestroy(ev);
+            throw std::runtime_error("invalid time step");
+        }
         LAUNCH("gyrate", k_gyrate, dim3(nb), dim3(256), 0, stream, d, P, U[0], U[1], static_cast<const double *>(Btot),
                static_cast<const double *>(dt_dev), 0.5);
         LAUNCH("prepare", k_prepare<double>, dim3(nb), dim3(256), 0, stream, d, P, Uc, static_cast<const double *>(Phi),
@@ -467,6 +478,7 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     hip_check(hipEventSynchronize(ev), "event sync");
     (void)hipEventDestroy(ev);
     const double gdt = dth[0];
+    // nothing above wrote U or Phi (only derived fields), so the state is unchanged when this throws
     if (!std::isfinite(gdt) || gdt <= 0) throw std::runtime_error("invalid time step");
     if (P.ns > 2)
         LAUNCH("cell_update", k_cell_update_many<double>, dim3(nb), dim3(256), 0, stream, d, P, P.ns, static_cast<const double *>(FA), Wc,

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import lib, ptr, f64
+from ._lib import lib, ptr
 
 DERIVED = {"W_H": 0, "W_Na": 1, "B_ind": 2, "B_tot": 3, "J": 4, "u_plus": 5, "E": 6}
 
@@ -33,6 +33,28 @@ class Device:
         if rc:
             raise RuntimeError(lib().mhdg_last_error(self._h).decode())
 
+    # Inputs crossing the C-ABI: contiguous float64 (or int8) arrays of the exact size the entry point reads, so a
+    # float32, list or strided input is converted (and kept alive by ptr() for the call) and a wrong size raises
+    # here instead of letting the library read past the caller's buffer.
+    _SIZES = {"c": lambda d: d.nc, "c3": lambda d: 3 * d.nc, "c5": lambda d: 5 * d.nc, "c15": lambda d: 15 * d.nc,
+              "f": lambda d: d.nf, "f5": lambda d: 5 * d.nf, "e": lambda d: d.ne, "fs": lambda d: 5}
+
+    def _in(self, a, kind, dtype=np.float64):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dtype).ravel()
+        n = self._SIZES[kind](self)
+        if a.size != n:
+            raise ValueError(f"array of {a.size} values where {n} are expected")
+        return a
+
+    @staticmethod
+    def _sized(a, n):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if a.size != n:
+            raise ValueError(f"array of {a.size} values where {n} are expected")
+        return a
+
     @staticmethod
     def info():
         buf = C.create_string_buffer(256)
@@ -42,63 +64,64 @@ class Device:
     # ------------------------------------------------------------------ mhd_common calls
     def cell_vector(self, flux):
         out = np.zeros(3 * self.nc)
-        self._ok(lib().mhdg_cell_vector(self._h, ptr(f64(flux)), ptr(out)))
+        self._ok(lib().mhdg_cell_vector(self._h, ptr(self._in(flux, 'f')), ptr(out)))
         return out
 
     def cell_current(self, B):
         out = np.zeros(3 * self.nc)
-        self._ok(lib().mhdg_cell_current(self._h, ptr(f64(B)), ptr(out)))
+        self._ok(lib().mhdg_cell_current(self._h, ptr(self._in(B, 'c3')), ptr(out)))
         return out
 
     def limited_gradients(self, W, species, fs, limit=True, ghost=None):
         out = np.zeros(15 * self.nc)
-        g = None if ghost is None else f64(ghost)
-        self._ok(lib().mhdg_limited_gradients(self._h, ptr(f64(W)), species, ptr(f64(fs)), int(limit), ptr(g), ptr(out)))
+        g = self._in(ghost, 'f5')
+        self._ok(lib().mhdg_limited_gradients(self._h, ptr(self._in(W, 'c5')), species, ptr(self._in(fs, 'fs')), int(limit), ptr(g), ptr(out)))
         return out
 
     def flux_rhs(self, species, fs, W, grad, ghost=None):
         out = np.zeros(5 * self.nc)
-        g = None if ghost is None else f64(ghost)
-        self._ok(lib().mhdg_flux_rhs(self._h, species, ptr(f64(fs)), ptr(f64(W)), ptr(f64(grad)), ptr(g), ptr(out)))
+        g = self._in(ghost, 'f5')
+        self._ok(lib().mhdg_flux_rhs(self._h, species, ptr(self._in(fs, 'fs')), ptr(self._in(W, 'c5')), ptr(self._in(grad, 'c15')), ptr(g), ptr(out)))
         return out
 
     def ideal_emf(self, u, B_tot, B_ind, phi_tot, phi_ind, cuct=0.5, E=None):
-        E = np.zeros(self.ne) if E is None else f64(E).copy()
-        self._ok(lib().mhdg_ideal_emf(self._h, ptr(f64(u)), ptr(f64(B_tot)), ptr(f64(B_ind)), ptr(f64(phi_tot)),
-                                      ptr(f64(phi_ind)), cuct, ptr(E)))
+        E = np.zeros(self.ne) if E is None else self._in(E, 'e').copy()
+        self._ok(lib().mhdg_ideal_emf(self._h, ptr(self._in(u, 'c3')), ptr(self._in(B_tot, 'c3')), ptr(self._in(B_ind, 'c3')),
+                                      ptr(self._in(phi_tot, 'f')), ptr(self._in(phi_ind, 'f')), cuct, ptr(E)))
         return E
 
     def faraday(self, E, Phi, dt):
-        Phi = f64(Phi).copy()
-        self._ok(lib().mhdg_faraday(self._h, ptr(f64(E)), ptr(Phi), dt))
+        Phi = self._in(Phi, 'f').copy()
+        self._ok(lib().mhdg_faraday(self._h, ptr(self._in(E, 'e')), ptr(Phi), dt))
         return Phi
 
     def faraday_edge_dt(self, E, dt_edge, Phi):
-        Phi = f64(Phi).copy()
-        self._ok(lib().mhdg_faraday_edge_dt(self._h, ptr(f64(E)), ptr(f64(dt_edge)), ptr(Phi)))
+        Phi = self._in(Phi, 'f').copy()
+        self._ok(lib().mhdg_faraday_edge_dt(self._h, ptr(self._in(E, 'e')), ptr(self._in(dt_edge, 'e')), ptr(Phi)))
         return Phi
 
     def div_b(self, B, extra=None):
         a, r, w = C.c_double(), C.c_double(), C.c_int64()
-        self._ok(lib().mhdg_div_b(self._h, ptr(f64(B)), ptr(None if extra is None else f64(extra)), C.byref(a), C.byref(r), C.byref(w)))
+        self._ok(lib().mhdg_div_b(self._h, ptr(self._in(B, 'f')), ptr(self._in(extra, 'f')), C.byref(a), C.byref(r), C.byref(w)))
         return a.value, r.value, w.value
 
     def local_dt(self, W_H, W_Na, B_tot):
         dt = C.c_double()
-        self._ok(lib().mhdg_local_dt(self._h, ptr(f64(W_H)), ptr(f64(W_Na)), ptr(f64(B_tot)), C.byref(dt)))
+        self._ok(lib().mhdg_local_dt(self._h, ptr(self._in(W_H, 'c5')), ptr(self._in(W_Na, 'c5')), ptr(self._in(B_tot, 'c3')), C.byref(dt)))
         return dt.value
 
     def add_sources(self, species, W_H, W_Na, B_tot, J, dt, rhs):
-        rhs = f64(rhs).copy()
-        self._ok(lib().mhdg_add_sources(self._h, species, ptr(f64(W_H)), ptr(f64(W_Na)), ptr(f64(B_tot)), ptr(f64(J)), dt, ptr(rhs)))
+        rhs = self._in(rhs, 'c5').copy()
+        self._ok(lib().mhdg_add_sources(self._h, species, ptr(self._in(W_H, 'c5')), ptr(self._in(W_Na, 'c5')), ptr(self._in(B_tot, 'c3')),
+                                        ptr(self._in(J, 'c3')), dt, ptr(rhs)))
         return rhs
 
     def ind_set_frozen(self, B0, ne_eff):
-        self._ok(lib().mhdg_ind_set_frozen(self._h, ptr(f64(B0)), ptr(f64(ne_eff))))
+        self._ok(lib().mhdg_ind_set_frozen(self._h, ptr(self._in(B0, 'c3')), ptr(self._in(ne_eff, 'c'))))
 
     def ind_apply_F(self, phi):
         out = np.zeros(self.ne)
-        self._ok(lib().mhdg_ind_apply_F(self._h, ptr(f64(phi)), ptr(out)))
+        self._ok(lib().mhdg_ind_apply_F(self._h, ptr(self._in(phi, 'f')), ptr(out)))
         return out
 
     def ind_flags(self):
@@ -108,39 +131,39 @@ class Device:
         return bool(a.value), bool(i.value), bool(h.value)
 
     def ind_set_ideal_state(self, u, mask):
-        self._ok(lib().mhdg_ind_set_ideal_state(self._h, ptr(f64(u)), ptr(np.ascontiguousarray(mask, dtype=np.int8))))
+        self._ok(lib().mhdg_ind_set_ideal_state(self._h, ptr(self._in(u, 'c3')), ptr(self._in(np.asarray(mask) != 0, 'e', np.int8))))
 
     def ind_ideal_linear(self, phi):
         out = np.zeros(self.ne)
-        self._ok(lib().mhdg_ind_ideal_linear(self._h, ptr(f64(phi)), ptr(out)))
+        self._ok(lib().mhdg_ind_ideal_linear(self._h, ptr(self._in(phi, 'f')), ptr(out)))
         return out
 
     def ind_solve(self, Phi_n, E_expl, dirichlet, dt, precision=0, dt_edge=None):
         E = np.zeros(self.ne)
         it, rr = C.c_int32(), C.c_double()
-        d = np.ascontiguousarray(dirichlet, dtype=np.int8)
-        self._ok(lib().mhdg_ind_solve(self._h, ptr(f64(Phi_n)), ptr(f64(E_expl)), ptr(d), dt,
-                                      ptr(None if dt_edge is None else f64(dt_edge)), precision, ptr(E), C.byref(it), C.byref(rr)))
+        d = self._in(np.asarray(dirichlet) != 0, 'e', np.int8)
+        self._ok(lib().mhdg_ind_solve(self._h, ptr(self._in(Phi_n, 'f')), ptr(self._in(E_expl, 'e')), ptr(d), dt,
+                                      ptr(self._in(dt_edge, 'e')), precision, ptr(E), C.byref(it), C.byref(rr)))
         return E, it.value, rr.value
 
     def ind_apply_G(self, phi):
         """Induction operator with the Hall field B_add + R phi of phi itself (quadratic in phi)."""
         out = np.zeros(self.ne)
-        self._ok(lib().mhdg_ind_apply_G(self._h, ptr(f64(phi)), ptr(out)))
+        self._ok(lib().mhdg_ind_apply_G(self._h, ptr(self._in(phi, 'f')), ptr(out)))
         return out
 
     def ind_jvp(self, phi_lin, dphi):
         """Tangent-linear product G'(phi_lin) dphi."""
         out = np.zeros(self.ne)
-        self._ok(lib().mhdg_ind_jvp(self._h, ptr(f64(phi_lin)), ptr(f64(dphi)), ptr(out)))
+        self._ok(lib().mhdg_ind_jvp(self._h, ptr(self._in(phi_lin, 'f')), ptr(self._in(dphi, 'f')), ptr(out)))
         return out
 
     def ind_newton(self, Phi_n, E_expl, dirichlet, dt, E0=None, rtol=1e-8, max_newton=20, eta=1e-4):
         """Newton-Krylov solve of the nonlinear Hall induction step: (E, newton steps, GMRES iterations, rel. residual)."""
         E = np.zeros(self.ne)
         k, g, rr = C.c_int32(), C.c_int32(), C.c_double()
-        d = np.ascontiguousarray(dirichlet, dtype=np.int8)
-        self._ok(lib().mhdg_ind_newton(self._h, ptr(f64(Phi_n)), ptr(f64(E_expl)), ptr(d), dt, ptr(None if E0 is None else f64(E0)),
+        d = self._in(np.asarray(dirichlet) != 0, 'e', np.int8)
+        self._ok(lib().mhdg_ind_newton(self._h, ptr(self._in(Phi_n, 'f')), ptr(self._in(E_expl, 'e')), ptr(d), dt, ptr(self._in(E0, 'e')),
                                        rtol, max_newton, eta, ptr(E), C.byref(k), C.byref(g), C.byref(rr)))
         return E, k.value, g.value, rr.value
 
@@ -155,8 +178,8 @@ class Device:
         """Forward-mode derivative of n explicit steps from the resident state along the given seeds:
         (J, dJ) of the shell energy of the final state, and the final state tangents when `states`."""
         J, dJ = C.c_double(), C.c_double()
-        opt = lambda a: None if a is None else f64(a)
-        seeds = [opt(a) for a in (dU_H, dU_Na, dPhi, dPhiAdd, dBadd, dfs, dE_inf)]
+        seeds = [self._in(a, k) for a, k in ((dU_H, 'c5'), (dU_Na, 'c5'), (dPhi, 'f'), (dPhiAdd, 'f'), (dBadd, 'c3'))]
+        seeds += [None if dfs is None else self._sized(dfs, 10), None if dE_inf is None else self._sized(dE_inf, 3)]
         outs = [np.zeros(5 * self.nc), np.zeros(5 * self.nc), np.zeros(self.nf)] if states else [None, None, None]
         self._ok(lib().mhdg_explicit_tangent(self._h, n_steps, dt_max, *[ptr(a) for a in seeds], r0, r1, C.byref(J), C.byref(dJ),
                                              *[ptr(a) for a in outs]))
@@ -175,7 +198,7 @@ class Device:
 
     # ------------------------------------------------------------------ resident state
     def set_state(self, U_H, U_Na, Phi):
-        self._ok(lib().mhdg_set_state(self._h, ptr(f64(U_H)), ptr(f64(U_Na)), ptr(f64(Phi))))
+        self._ok(lib().mhdg_set_state(self._h, ptr(self._in(U_H, 'c5')), ptr(self._in(U_Na, 'c5')), ptr(self._in(Phi, 'f'))))
 
     def set_state_from_fields(self, path):
         """Resident state from a restart file in the layout of mercury::Solver::write_global_fields
@@ -195,7 +218,7 @@ class Device:
 
     def set_species_state(self, species, U):
         """Conservative state (5 per cell) of ion species `species` < n_species (ten-ion fluids)."""
-        self._ok(lib().mhdg_set_species_state(self._h, int(species), ptr(f64(U))))
+        self._ok(lib().mhdg_set_species_state(self._h, int(species), ptr(self._in(U, 'c5'))))
 
     def get_species_state(self, species):
         U = np.zeros(5 * self.nc)
@@ -209,7 +232,7 @@ class Device:
         return W
 
     def set_background(self, PhiAdd, BaddCell):
-        self._ok(lib().mhdg_set_background(self._h, ptr(f64(PhiAdd)), ptr(f64(BaddCell))))
+        self._ok(lib().mhdg_set_background(self._h, ptr(self._in(PhiAdd, 'f')), ptr(self._in(BaddCell, 'c3'))))
 
     def step(self, dt_max=1e9, precision=0):
         dt, it, rr = C.c_double(), C.c_int32(), C.c_double()

@@ -12,8 +12,9 @@
 //
 // Every function keeps the reference signature, argument meaning and error behaviour (a failing device call
 // throws std::runtime_error with the library's message).  The device context of a mesh is created at the
-// first call that needs it and is keyed by the mesh's CellComplex; it is rebuilt when a later call brings the
-// boundary info or the configuration it was built without (setup only).  Arrays stay host-side
+// first call that needs it and is keyed by the mesh's CellComplex together with its entity counts and table
+// addresses (a different mesh at a reused address gets a new context); it is rebuilt when a later call brings
+// the boundary info or the configuration it was built without (setup only).  Config::ind_ilu is refused.  Arrays stay host-side
 // std::vectors owned by the drivers: each call copies its inputs to the device and its outputs back.
 //
 // Scope: one rank (the Exchanger is the identity, every entity is owned); the fluid calls serve any species
@@ -77,6 +78,29 @@ mhdg_config to_abi(const mhd::Config &k)
     return c;
 }
 
+// What identifies the mesh a context was built from: the entity counts and the addresses of the arrays the
+// device tables were copied from.  A complex destroyed and another created at the same address (drivers that
+// new and delete their LocalMesh) differs in at least one of them unless it is the same allocation reused with
+// the same sizes and buffers, in which case the arrays it describes are again the ones uploaded.
+struct MeshKey
+{
+    int64_t n_cells = -1, n_faces = -1, n_edges = -1;
+    const void *c2f = nullptr, *f2e = nullptr, *area = nullptr, *ec = nullptr;
+    bool operator==(const MeshKey &o) const
+    {
+        return n_cells == o.n_cells && n_faces == o.n_faces && n_edges == o.n_edges && c2f == o.c2f && f2e == o.f2e && area == o.area &&
+               ec == o.ec;
+    }
+};
+MeshKey key_of(const uns::CellComplex &cx, const uns::Geometry *g)
+{
+    MeshKey k;
+    k.n_cells = cx.n_cells(); k.n_faces = cx.n_faces(); k.n_edges = cx.n_edges();
+    k.c2f = cx.c2f.idx.data(); k.f2e = cx.f2e.idx.data();
+    if (g) { k.area = g->face_area.data(); k.ec = g->ec_cell.data(); }
+    return k;
+}
+
 // One device context per mesh, with what it was built from
 struct MeshRec
 {
@@ -85,6 +109,7 @@ struct MeshRec
     const mhd::BoundaryInfo *bc = nullptr;
     std::unique_ptr<mhd::Config> cfg;
     mhdg_ctx *ctx = nullptr;
+    MeshKey key; // of the mesh the context was built from
     bool built_with_bc = false, built_with_cfg = false;
     std::vector<char> no_kind, no_mode; // placeholders until the boundary info is known
     ~MeshRec() { if (ctx) mhdg_destroy(ctx); }
@@ -94,7 +119,8 @@ std::map<const uns::CellComplex *, std::unique_ptr<MeshRec>> g_mesh;
 MeshRec &record(const uns::LocalMesh &lm, const uns::Geometry &geo)
 {
     auto &r = g_mesh[&lm.mesh.cx];
-    if (!r) r = std::make_unique<MeshRec>();
+    // a record left by a different mesh at this address is dropped with its context, bc and configuration
+    if (!r || (r->ctx && !(r->key == key_of(lm.mesh.cx, &geo)))) r = std::make_unique<MeshRec>();
     r->lm = &lm;
     r->geo = &geo;
     return *r;
@@ -134,6 +160,7 @@ mhdg_ctx *context(MeshRec &r)
     mhdg_ctx *ctx = nullptr;
     if (mhdg_create(&d, &c, &ctx) != 0) throw std::runtime_error(std::string("mhdg_create: ") + mhdg_last_error(nullptr));
     r.ctx = ctx;
+    r.key = key_of(cx, r.geo);
     r.built_with_bc = r.bc != nullptr;
     r.built_with_cfg = r.cfg != nullptr;
     return ctx;
@@ -144,7 +171,17 @@ mhdg_ctx *context(const uns::CellComplex &cx)
     auto it = g_mesh.find(&cx);
     if (it == g_mesh.end() || !it->second->lm)
         throw std::runtime_error("mhd_gpu_shim: no mesh geometry registered for this complex (call a function taking LocalMesh first)");
-    return context(*it->second);
+    MeshRec &r = *it->second;
+    // the calls that take only the complex cannot see the geometry: the counts and tables must still be the ones
+    // the context was built from
+    if (r.ctx)
+    {
+        MeshKey k = key_of(cx, nullptr), built = r.key;
+        built.area = built.ec = nullptr;
+        if (!(k == built))
+            throw std::runtime_error("mhd_gpu_shim: this complex is not the mesh its context was built from (call a function taking LocalMesh first)");
+    }
+    return context(r);
 }
 
 // GhostFn -> per-face ghost table (5 primitives per face; NaN-free copies of W for non-boundary faces)
@@ -288,10 +325,16 @@ ImplicitInduction::ImplicitInduction(const uns::LocalMesh &lm, const uns::Geomet
     : lm_(lm), geo_(geo), ex_(ex), resistive_(cfg.resistive && cfg.sc.inver_Rem > 0), hall_(cfg.hall && cfg.sc.hall_coef > 0),
       ideal_(cfg.implicit_ideal), ideal_prm_(cfg.ideal), rtol_(cfg.ind_rtol), max_it_(cfg.ind_max_it), restart_(std::max(2, cfg.ind_restart))
 {
+    // the block-Jacobi ILU(0) preconditioner (ImplicitInduction.cpp:209-295) is not on the device: refuse it rather than
+    // solve with Jacobi and a different iteration history
+    if (cfg.ind_ilu) throw std::runtime_error("mhd_gpu_shim: induction_ilu is not supported by the device solve (Jacobi only)");
     auto r = std::make_unique<MeshRec>();
     r->lm = &lm;
     r->geo = &geo;
     r->cfg = std::make_unique<Config>(cfg);
+    // an instance constructed at the address of a destroyed one replaces its record (and releases its context);
+    // the reference class declares no destructor for the shim to hook, so records of destroyed instances whose
+    // address is not reused stay until the process ends
     g_ind[this] = std::move(r);
     MeshRec &m = record(lm, geo); // the mesh's free functions use the same configuration
     if (!m.cfg) m.cfg = std::make_unique<Config>(cfg);

@@ -21,10 +21,12 @@ _P, _D = C.c_void_p, C.c_double
 
 
 def _ptr(a):
+    """ctypes pointer to a contiguous array's data; the pointer object holds a reference to the array, so a
+    converted temporary (_ptr(_f(x))) stays alive until the C call has returned."""
     if a is None:
         return None
     assert a.flags["C_CONTIGUOUS"]
-    return a.ctypes.data
+    return a.ctypes.data_as(_P)
 
 
 def _f(a):
@@ -232,6 +234,8 @@ class Reference:
                 "ref_background": [_P, _P, _P], "ref_cell_vector": [_P, _P, _P], "ref_cell_current": [_P, _P, _P],
                 "ref_limited_gradients": [_P, _P, C.c_int, _P, C.c_int, _P],
                 "ref_flux_rhs": [_P, C.c_int, _P, _P, _P, _P],
+                "ref_limited_gradients_ghost": [_P, _P, C.c_int, _P, C.c_int, _P, _P, _P],
+                "ref_flux_rhs_ghost": [_P, C.c_int, _P, _P, _P, _P, _P, _P], "ref_faraday_edge_dt": [_P, _P, _P, _P],
                 "ref_ideal_emf": [_P, _P, _P, _P, _P, _P, _D, _P], "ref_faraday": [_P, _P, _P, _D],
                 "ref_div_b": [_P, _P, _P, _P, _P], "ref_local_dt": [_P, _P, _P, _P],
                 "ref_add_sources": [_P, C.c_int, _P, _P, _P, _P, _P, _D, _P],
@@ -333,15 +337,28 @@ class Reference:
         self.lib().ref_cell_current(self._h, _ptr(_f(B)), _ptr(out))
         return out
 
-    def limited_gradients(self, W, species, fs, limit=True):
+    def limited_gradients(self, W, species, fs, limit=True, ghost=None):
+        """Stencil::limited_gradients; `ghost` (5 per face) is served through a GhostFn that returns the table entry.
+        With a table, also returns the largest |rho| + |p| of the `inside` states the GhostFn was handed."""
         out = np.zeros(15 * self.nc)
-        self.lib().ref_limited_gradients(self._h, _ptr(_f(W)), species, _ptr(_f(fs)), int(limit), _ptr(out))
-        return out
+        if ghost is None:
+            self.lib().ref_limited_gradients(self._h, _ptr(_f(W)), species, _ptr(_f(fs)), int(limit), _ptr(out))
+            return out
+        seen = C.c_double(0.0)
+        self.lib().ref_limited_gradients_ghost(self._h, _ptr(_f(W)), species, _ptr(_f(fs)), int(limit), _ptr(_f(ghost)), _ptr(out),
+                                               C.byref(seen))
+        return out, seen.value
 
-    def flux_rhs(self, species, fs, W, grad):
+    def flux_rhs(self, species, fs, W, grad, ghost=None):
+        """fluid_flux_rhs; `ghost` as in limited_gradients (then also returns the GhostFn's largest inside state)."""
         out = np.zeros(5 * self.nc)
-        self.lib().ref_flux_rhs(self._h, species, _ptr(_f(fs)), _ptr(_f(W)), _ptr(_f(grad)), _ptr(out))
-        return out
+        if ghost is None:
+            self.lib().ref_flux_rhs(self._h, species, _ptr(_f(fs)), _ptr(_f(W)), _ptr(_f(grad)), _ptr(out))
+            return out
+        seen = C.c_double(0.0)
+        self.lib().ref_flux_rhs_ghost(self._h, species, _ptr(_f(fs)), _ptr(_f(W)), _ptr(_f(grad)), _ptr(_f(ghost)), _ptr(out),
+                                      C.byref(seen))
+        return out, seen.value
 
     def ideal_emf(self, u, B_tot, B_ind, phi_tot, phi_ind, cuct=0.5, E=None):
         E = np.zeros(self.ne) if E is None else _f(E).copy()
@@ -352,6 +369,11 @@ class Reference:
     def faraday(self, E, Phi, dt):
         Phi = _f(Phi).copy()
         self.lib().ref_faraday(self._h, _ptr(_f(E)), _ptr(Phi), dt)
+        return Phi
+
+    def faraday_edge_dt(self, E, dt_edge, Phi):
+        Phi = _f(Phi).copy()
+        self.lib().ref_faraday_edge_dt(self._h, _ptr(_f(E)), _ptr(_f(dt_edge)), _ptr(Phi))
         return Phi
 
     def div_b(self, B):

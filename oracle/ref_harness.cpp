@@ -5,6 +5,7 @@
 // Used only by tests/ to pin the oracle restatement and the host setup against the reference.
 #include <mpi.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -220,6 +221,29 @@ void ref_flux_rhs(ref_ctx *r, int species, const double *fs, const double *W, co
 {
     fluid_flux_rhs(r->lm, r->geo, *r->bc, r->cfg, species, prim5(fs), W, grad, rhs);
 }
+// The same two calls with a GhostFn (Reconstruction.h:17-21) that returns a per-face table of 5 primitives: the
+// form of MarsSolver's cached wall state (MarsSolver.cpp:124-128, 148-167).  The callback also records the
+// largest |inside| it was handed, so a test can see that the call went through the GhostFn branch.
+static GhostFn table_ghost(const double *table, double *seen)
+{
+    return [table, seen](Id f, int, const Prim &in, const Vec3 &) {
+        if (seen) *seen = std::max(*seen, std::abs(in.rho) + std::abs(in.p));
+        const double *g = table + 5 * static_cast<std::size_t>(f);
+        return Prim{g[0], g[1], g[2], g[3], g[4]};
+    };
+}
+void ref_limited_gradients_ghost(ref_ctx *r, const double *W, int species, const double *fs, int limit, const double *table, double *grad,
+                                 double *seen)
+{
+    const GhostFn g = table_ghost(table, seen);
+    r->st->limited_gradients(r->lm, r->geo, *r->bc, W, species, prim5(fs), limit != 0, grad, &g);
+}
+void ref_flux_rhs_ghost(ref_ctx *r, int species, const double *fs, const double *W, const double *grad, const double *table, double *rhs,
+                        double *seen)
+{
+    const GhostFn g = table_ghost(table, seen);
+    fluid_flux_rhs(r->lm, r->geo, *r->bc, r->cfg, species, prim5(fs), W, grad, rhs, &g);
+}
 void ref_ideal_emf(ref_ctx *r, const double *u, const double *Bt, const double *Bi, const double *pt, const double *pi, double cuct, double *E)
 {
     IdealInputs in;
@@ -229,6 +253,10 @@ void ref_ideal_emf(ref_ctx *r, const double *u, const double *Bt, const double *
     add_ideal_emf(r->lm, r->geo, in, p, E);
 }
 void ref_faraday(ref_ctx *r, const double *E, double *Phi, double dt) { faraday_update(r->lm.mesh.cx, r->active, E, Phi, dt); }
+void ref_faraday_edge_dt(ref_ctx *r, const double *E, const double *dt_edge, double *Phi)
+{
+    faraday_update_edge_dt(r->lm.mesh.cx, r->active, E, dt_edge, Phi);
+}
 void ref_div_b(ref_ctx *r, const double *B, double *max_abs, double *max_rel, int64_t *worst)
 {
     const DivBReport d = div_b(r->lm.mesh.cx, B);

@@ -39,3 +39,46 @@ def box_mesh(nx, ny, nz, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), default="outflo
     m.boundary(tag_types or {}, default=default)
     m._cells = (xyz, cells, tags)
     return m
+
+
+def floored_state(w, n=40, seed=77):
+    """The workload's initial state with cells the step must floor (Solver.cpp:171-191): n seeded cells whose Na+
+    density is negative (the density branch) and n others whose H+ internal energy is negative (the pressure
+    branch).  Returns (U_H, U_Na, Phi) and the two index sets."""
+    UH, UNa, Phi = (np.array(a, copy=True) for a in w["state"])
+    nc = UH.size // 5
+    rng = np.random.default_rng(seed)
+    cells = rng.choice(nc, 2 * n, replace=False)
+    rho_cells, p_cells = cells[:n], cells[n:]
+    Na = UNa.reshape(-1, 5)
+    Na[rho_cells, 0] = -1e-3
+    H = UH.reshape(-1, 5)
+    ke = 0.5 * (H[p_cells, 1] ** 2 + H[p_cells, 2] ** 2 + H[p_cells, 3] ** 2) / H[p_cells, 0]
+    H[p_cells, 4] = ke - 0.1
+    return (UH, UNa, Phi), rho_cells, p_cells
+
+
+def rough_gradients(w, scale=50.0, seed=5):
+    """Gradients (15 per cell) large enough that the linear face states of many cells have rho or p <= 0, which
+    sends fluid_flux_rhs's face_state to its first-order fallback (FluxRHS.cpp:16-27)."""
+    rng = np.random.default_rng(seed)
+    return scale * rng.normal(size=15 * w["mesh"].n_cells)
+
+
+def fallback_faces(mesh, W, grad):
+    """Number of (face, side) pairs whose linear reconstruction has rho <= 0 or p <= 0 (the fallback branch)."""
+    c2f_ptr, c2f = mesh.array("c2f.ptr"), mesh.array("c2f.idx")
+    fc, cc = mesh.array("face_center").reshape(-1, 3), mesh.array("cell_center").reshape(-1, 3)
+    cell = np.repeat(np.arange(mesh.n_cells), np.diff(c2f_ptr))
+    dx = fc[c2f] - cc[cell]
+    G = grad.reshape(-1, 5, 3)[cell]
+    Wc = W.reshape(-1, 5)[cell]
+    rho = Wc[:, 0] + np.einsum("ij,ij->i", G[:, 0], dx)
+    p = Wc[:, 4] + np.einsum("ij,ij->i", G[:, 4], dx)
+    return int(np.count_nonzero((rho <= 0) | (p <= 0)))
+
+
+def d2_of(mesh, Phi):
+    """(d2 Phi)_c = sum over the cell's faces of the signed flux, by integer incidence (c2f)."""
+    ptr, idx, sgn = mesh.array("c2f.ptr"), mesh.array("c2f.idx"), mesh.array("c2f.sgn")
+    return np.add.reduceat(sgn * Phi[idx], ptr[:-1])

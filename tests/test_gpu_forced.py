@@ -1,0 +1,115 @@
+"""Branches the seeded states never take, forced from the host, on the device against the oracle (which matches the
+compiled reference on the same inputs: tests/test_oracle_branches.py): floors, the face-state fallback, the invalid
+time step, GMRES truncated at ind_max_it, and a GhostFn served through the drop-in shim.
+"""
+import numpy as np
+import pytest
+
+from conftest import maxrel
+from meshes import fallback_faces, floored_state, rough_gradients
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(w, state=None):
+    from mpcns_uns_amd import Device
+    from oracle.oracle import Oracle
+
+    dev, orc = Device(w["mesh"], w["cfg"]), Oracle(w["mesh"], w["cfg"])
+    for x in (dev, orc):
+        x.set_background(*w["background"])
+        x.set_state(*(state or w["state"]))
+    return dev, orc
+
+
+def test_floors_forced_bitwise(small_explicit):
+    w = small_explicit
+    state, rho_cells, p_cells = floored_state(w)
+    dev, orc = _pair(w, state)
+    for _ in range(2):
+        assert dev.step() == orc.step()
+        for a, b in zip(dev.get_state(), orc.get_state()):
+            assert np.array_equal(a, b)
+    cfg = w["cfg"]
+    Na = orc.get_state()[1].reshape(-1, 5)
+    assert np.any(np.all(Na[rho_cells] == [cfg.rho_floor, 0, 0, 0, cfg.p_floor / (cfg.gamma - 1.0)], axis=1))
+
+
+@pytest.mark.parametrize("species", [0, 1])
+def test_face_state_fallback_bitwise(small_explicit, species):
+    w = small_explicit
+    dev, orc = _pair(w)
+    orc.prepare()
+    W = orc.derived("W_H" if species == 0 else "W_Na")
+    fs = np.array(w["cfg"].fs[species][:])
+    g = rough_gradients(w, scale=50.0 if species == 0 else 0.05)
+    assert fallback_faces(w["mesh"], W, g) > 100
+    assert np.array_equal(dev.flux_rhs(species, fs, W, g), orc.flux_rhs(species, fs, W, g))
+
+
+@pytest.mark.parametrize("gyration", [False, True])
+def test_invalid_time_step_leaves_state(gyration):
+    """dt <= 0 raises "invalid time step" before the state changes (also with the gyration split, whose first half
+    rotates U); a NaN cap is no cap, as in the reference."""
+    from mpcns_uns_amd import workloads
+
+    w = workloads.build(net=(12, 48, 3), hall=True, **({"induce_coeff": 40.0} if gyration else {}))
+    dev, orc = _pair(w)
+    for dt_max in (-1.0, 0.0):
+        with pytest.raises(RuntimeError, match="invalid time step"):
+            dev.step(dt_max)
+        for a, b in zip(dev.get_state(), w["state"]):
+            assert np.array_equal(a, b)
+    dd, _, rd = dev.step(float("nan"))
+    do, _, ro = orc.step(float("nan"))
+    assert abs(dd - do) <= 1e-14 * do and rd < 1e-8
+    with pytest.raises(RuntimeError, match="precision"):
+        dev.step(precision=7)
+
+
+def test_gmres_truncated_at_max_it():
+    from mpcns_uns_amd import Device, workloads
+    from oracle.oracle import Oracle
+
+    w = workloads.build(net=(12, 48, 3), hall=True, ind_max_it=3)
+    m = w["mesh"]
+    rng = np.random.default_rng(31)
+    B0 = np.tile([0.1, -0.2, 0.3], m.n_cells) + 0.05 * rng.normal(size=3 * m.n_cells)
+    ne_eff = 1.0 + 0.1 * rng.uniform(size=m.n_cells)
+    dev, mix, orc = Device(m, w["cfg"]), Device(m, w["cfg"]), Oracle(m, w["cfg"])
+    for x in (dev, mix, orc):
+        x.ind_set_frozen(B0, ne_eff)
+    dirichlet = (m.array("edge_mode") != 0).astype(np.int8)
+    E_expl = rng.normal(size=m.n_edges) * 1e-3
+    Ed, itd, rd = dev.ind_solve(w["state"][2], E_expl, dirichlet, 5e-2, precision=0)
+    Eo, ito, ro = orc.ind_solve(w["state"][2], E_expl, dirichlet, 5e-2)
+    assert itd == ito == 3
+    assert rd > w["cfg"].ind_rtol and abs(rd - ro) <= 1e-10 * ro
+    assert maxrel(Ed, Eo) < 1e-12
+    Em, itm, rm = mix.ind_solve(w["state"][2], E_expl, dirichlet, 5e-2, precision=1)
+    assert itm == 3 and rm > w["cfg"].ind_rtol
+
+
+@pytest.mark.skipif(not __import__("oracle.oracle", fromlist=["x"]).have_ref_on_device(),
+                    reason="oracle/_ref/libmpcns_ref_gpu.so not built")
+@pytest.mark.parametrize("species", [0, 1])
+def test_shim_ghostfn_table(small_explicit, species):
+    """The reference's Stencil::limited_gradients / fluid_flux_rhs called with a GhostFn (a per-face table, the form
+    MarsSolver uses) through the drop-in shim on the device: equal to the oracle's ghost-table path."""
+    from mpcns_uns_amd import workloads
+    from oracle.oracle import reference_for_mesh
+
+    w = small_explicit
+    dev, orc = _pair(w)
+    orc.prepare()
+    W = orc.derived("W_H" if species == 0 else "W_Na")
+    fs = np.array(w["cfg"].fs[species][:])
+    table = np.abs(np.random.default_rng(3 + species).normal(1.0, 0.2, 5 * w["mesh"].n_faces))
+    hyb = reference_for_mesh(w["mesh"], w["cfg"], dipole_m=workloads.DIPOLE_M, on_device=True)
+    for limit in (True, False):
+        gh, seen = hyb.limited_gradients(W, species, fs, limit, ghost=table)
+        assert seen > 0
+        go = orc.limited_gradients(W, species, fs, limit, table)
+        assert np.array_equal(gh, go)
+        rh, _ = hyb.flux_rhs(species, fs, W, go, ghost=table)
+        assert np.array_equal(rh, orc.flux_rhs(species, fs, W, go, table))

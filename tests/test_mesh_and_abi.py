@@ -182,3 +182,29 @@ def test_create_rejects_bad_descriptors(defect, message):
     h = C.c_void_p()
     assert lib().mhdg_create(C.byref(d), C.byref(cfg), C.byref(h)) != 0
     assert message in lib().mhdg_last_error(None).decode()
+
+
+def test_converted_inputs_outlive_the_call():
+    """ptr() of a converted temporary keeps the converted array alive (the pointer holds a reference), so a float32 or
+    list input reaches C intact; Device checks every input's size before its pointer crosses the C-ABI."""
+    import ctypes as C
+
+    from mpcns_uns_amd import Device
+    from mpcns_uns_amd._lib import f64, ptr
+
+    q = ptr(f64([1.0, 2.0, 3.0, 4.0, 5.0]))
+    junk = [np.full(5, 9.0) for _ in range(50)]  # reuses freed buffers of the same size, if any
+    assert C.cast(q, C.POINTER(C.c_double))[0:5] == [1.0, 2.0, 3.0, 4.0, 5.0]
+    del junk
+    d = Device.__new__(Device)  # size checks only: no device context
+    d.nc, d.nf, d.ne = 4, 6, 7
+    assert d._in(np.arange(12, dtype=np.float32), "c3").dtype == np.float64
+    assert d._in(list(range(7)), "e").tolist() == list(range(7))
+    assert d._in(np.arange(24.0).reshape(6, 4)[:, 0], "f").flags["C_CONTIGUOUS"]
+    assert d._in([True, False] * 3 + [True], "e", np.int8).tolist() == [1, 0, 1, 0, 1, 0, 1]
+    with pytest.raises(ValueError):
+        d._in(np.zeros(11), "c3")
+    m = __import__("mpcns_uns_amd").Mesh.shell(6, 12, 2, seed=3)
+    flux = np.random.default_rng(0).normal(size=m.n_faces)
+    assert np.array_equal(m.cell_vector(flux.astype(np.float32)), m.cell_vector(flux.astype(np.float32).astype(np.float64)))
+    assert np.array_equal(m.cell_vector(list(flux)), m.cell_vector(flux))

@@ -52,6 +52,10 @@ typedef struct mhdg_mesh_desc {
     const double *ef_weight;        /* w_{e,f}, aligned with e2f */
     const double *star1;            /* E */
     const double *star2;            /* F */
+    /* Edge -> node incidence (CellComplex::e2n; sign -1 the start node, +1 the end node) and the node count.
+     * Optional (NULL ptr / 0): only the multigrid preconditioner (precision 2) reads them, to agglomerate nodes. */
+    mhdg_incidence e2n;
+    int32_t n_nodes;
 } mhdg_mesh_desc;
 
 /* Run configuration: the fields of mhd::Config / mhd::Scales the path reads
@@ -87,6 +91,16 @@ typedef struct mhdg_config {
      * exact gyration of the two ions' relative velocity about B as two half steps around the fluid update
      * (mercury::gyrate, MercuryUns/physics/Gyration.cpp:17-51, Solver.cpp:237-241, 272).  Two-ion fluids. */
     double induce_coeff;
+    /* Agglomeration multigrid preconditioner of the mixed-precision solve (precision 2), named after the
+     * reference's agglomeration-multigrid parameters (Config.h:56-63): mg_levels levels including the fine one;
+     * mg_agglom_passes heavy-edge matching passes per level over the node graph (aggregates of ~2^passes
+     * nodes); mg_pre / mg_post damped-Jacobi sweeps before / after the coarse correction, mg_coarse sweeps on the
+     * coarsest level; mg_relax the Jacobi damping; mg_fine_hall 1: the fine-level smoother applies the full
+     * operator (Hall and resistive), 0: its resistive part only.  Zero fields take the defaults 4, 3, 2, 2, 4,
+     * 0.7 and 1 (mhdg_config_defaults sets them). */
+    int32_t mg_levels, mg_agglom_passes, mg_pre, mg_post, mg_coarse;
+    double mg_relax;
+    int32_t mg_fine_hall;
 } mhdg_config;
 
 /* Fills `cfg` with the reference defaults (Config.h:19-96 initialisers, Scales::simple(5/3, 1, 1)). */
@@ -149,11 +163,20 @@ int mhdg_ind_ideal_linear(mhdg_ctx *ctx, const double *phi, double *out);
  * scalar dt: system (I + F d1 diag(dt_e)) E = b.  precision: 0 = FP64 GMRES(m) with Jacobi and modified
  * Gram-Schmidt (the reference iteration); 1 = mixed precision (FP32 operator and Krylov basis, classical
  * Gram-Schmidt with FP64 accumulation, inside FP64 iterative refinement to the same relative residual;
- * not available with the implicit ideal part).  The warm-start correction is kept in the context between
+ * not available with the implicit ideal part); 2 = the mixed solve with flexible GMRES preconditioned by one FP32
+ * agglomeration-multigrid V-cycle per step (mhdg_mg_apply; needs e2n in the mesh descriptor, scalar dt only).  The warm-start correction is kept in the context between
  * calls, as the reference keeps corr_. */
 int mhdg_ind_solve(mhdg_ctx *ctx, const double *Phi_n, const double *E_expl, const int8_t *dirichlet,
                    double dt, const double *dt_edge, int precision, double *E, int32_t *iterations,
                    double *rel_res);
+
+/* Multigrid preconditioner of precision 2 (no reference counterpart: the reference preconditions with Jacobi or a
+ * block-Jacobi ILU(0), ImplicitInduction.cpp:209-317).  The hierarchy (include/mhdg_mesh.h, mhdg_mg_build) is built on
+ * the first use for a Dirichlet set and kept while later calls bring the same set.  mhdg_mg_apply: one V-cycle z = M v
+ * (FP32 inside, FP64 in and out) with the frozen state of mhdg_ind_set_frozen and step dt; mhdg_mg_info: the number of
+ * levels and the unknowns of each (up to max_levels entries of n_per_level). */
+int mhdg_mg_apply(mhdg_ctx *ctx, const int8_t *dirichlet, double dt, const double *v, double *z);
+int mhdg_mg_info(mhdg_ctx *ctx, const int8_t *dirichlet, int32_t max_levels, int32_t *levels, int32_t *n_per_level);
 
 /* --------------------------------------------- nonlinear Hall induction: tangent-linear and Newton-Krylov */
 /* No counterpart in the reference (it freezes the Hall field at B_tot^n, ImplicitInduction.cpp:105-111).

@@ -117,6 +117,26 @@ int mhdg_write_pvtu(const char *path, int32_t n_pieces, const char *const *piece
                     const mhdg_vtk_array *cell_arrays, int32_t n_point_arrays, const mhdg_vtk_array *point_arrays);
 int mhdg_write_pvd(const char *path, int32_t n, const double *times, const char *const *files);
 
+/* Agglomeration hierarchy of the multigrid preconditioner (precision 2 of mhdg_ind_solve / mhdg_step), built on the
+ * host as the device context builds it: levels of node aggregates by heavy-edge matching (the reference's
+ * agglomeration, MercuryUns/multigrid/Multigrid.cpp:219-282, on the node graph), coarse unknowns = pairs of adjacent
+ * aggregates, prolongation +-1 per finer edge (0 inside an aggregate and on Dirichlet edges), Galerkin coarse
+ * operators M_l (diagonal) and K_l (CSR) of I + dt K, K the resistive curl-curl of ImplicitInduction::apply_F.
+ * cfg supplies mg_levels, mg_agglom_passes and the resistivity; the descriptor must carry e2n.  For tests and
+ * inspection; no reference counterpart (the reference has no induction multigrid). */
+typedef struct mhdg_mg mhdg_mg;
+int mhdg_mg_build(const mhdg_mesh_desc *mesh, const mhdg_config *cfg, const int8_t *dirichlet, mhdg_mg **out);
+void mhdg_mg_free(mhdg_mg *mg);
+int mhdg_mg_levels(const mhdg_mg *mg);
+/* n unknowns, n_nodes aggregates (level 0: the mesh's edges and nodes), k_nnz entries of K_l (0 on level 0) */
+int mhdg_mg_level_size(const mhdg_mg *mg, int level, int32_t *n, int32_t *n_nodes, int64_t *k_nnz);
+/* Level >= 1: map (n of the finer level: signed coarse index, v < 0 = -1 on ~v, INT32_MIN none), mass (n), K_l CSR
+ * (k_ptr n+1, k_col and k_val k_nnz).  Level 0: mass only.  Any output may be NULL. */
+int mhdg_mg_level_arrays(const mhdg_mg *mg, int level, int32_t *map, double *mass, int32_t *k_ptr, int32_t *k_col, double *k_val);
+/* End nodes of every unknown of a level (tail, head: n each; level 0 the mesh's edge orientation) and, level >= 1, the
+ * aggregate of every node of the finer level (agg: n_nodes of the finer level).  Any output may be NULL. */
+int mhdg_mg_level_nodes(const mhdg_mg *mg, int level, int32_t *tail, int32_t *head, int32_t *agg);
+
 #ifdef __cplusplus
 }
 #endif

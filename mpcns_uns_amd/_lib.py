@@ -28,6 +28,7 @@ class MeshDesc(C.Structure):
         ("ec_ptr", C.POINTER(C.c_int64)), ("ec_cell", C.POINTER(C.c_int32)),
         ("ec_weight", C.POINTER(C.c_double)), ("ef_weight", C.POINTER(C.c_double)),
         ("star1", C.POINTER(C.c_double)), ("star2", C.POINTER(C.c_double)),
+        ("e2n", Incidence), ("n_nodes", C.c_int32),
     ]
 
 
@@ -50,6 +51,8 @@ class Config(C.Structure):
         ("src_rate", C.c_double * 10),
         ("src_scale_height", C.c_double), ("src_r0", C.c_double), ("src_T", C.c_double),
         ("induce_coeff", C.c_double),
+        ("mg_levels", C.c_int32), ("mg_agglom_passes", C.c_int32), ("mg_pre", C.c_int32), ("mg_post", C.c_int32),
+        ("mg_coarse", C.c_int32), ("mg_relax", C.c_double), ("mg_fine_hall", C.c_int32),
     ]
 
 
@@ -94,6 +97,12 @@ def lib():
         L.mhdg_fields_read.argtypes = [C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64), C.POINTER(I32)]
         L.mhdg_fields_get.argtypes = [P, I32, P]
         L.mhdg_fields_free.argtypes = [P]
+        L.mhdg_mg_build.argtypes = [C.POINTER(MeshDesc), C.POINTER(Config), P, C.POINTER(P)]
+        L.mhdg_mg_free.argtypes = [P]
+        L.mhdg_mg_levels.argtypes = [P]
+        L.mhdg_mg_level_size.argtypes = [P, C.c_int, P, P, P]
+        L.mhdg_mg_level_arrays.argtypes = [P, C.c_int, P, P, P, P, P]
+        L.mhdg_mg_level_nodes.argtypes = [P, C.c_int, P, P, P]
         for name, args in {
             "mhdg_cell_vector": [P, P, P], "mhdg_cell_current": [P, P, P],
             "mhdg_limited_gradients": [P, P, C.c_int, P, C.c_int, P, P],
@@ -104,7 +113,7 @@ def lib():
             "mhdg_add_sources": [P, C.c_int, P, P, P, P, D, P],
             "mhdg_ind_set_frozen": [P, P, P], "mhdg_ind_apply_F": [P, P, P],
             "mhdg_ind_solve": [P, P, P, P, D, P, C.c_int, P, P, P],
-            "mhdg_ind_flags": [P, P, P, P], "mhdg_ind_set_ideal_state": [P, P, P], "mhdg_ind_ideal_linear": [P, P, P],
+            "mhdg_ind_flags": [P, P, P, P], "mhdg_mg_apply": [P, P, D, P, P], "mhdg_mg_info": [P, P, I32, P, P], "mhdg_ind_set_ideal_state": [P, P, P], "mhdg_ind_ideal_linear": [P, P, P],
             "mhdg_step_explicit": [P, D, P], "mhdg_set_species_state": [P, C.c_int, P], "mhdg_get_species_state": [P, C.c_int, P],
             "mhdg_get_species_prim": [P, C.c_int, P], "mhdg_ind_apply_G": [P, P, P], "mhdg_ind_jvp": [P, P, P, P],
             "mhdg_ind_newton": [P, P, P, P, D, P, D, C.c_int, D, P, P, P, P],

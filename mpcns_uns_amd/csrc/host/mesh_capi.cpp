@@ -1,4 +1,5 @@
 // Host C-ABI for mesh setup (include/mhdg_mesh.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -11,6 +12,8 @@
 #include "../../../include/mhdg_mesh.h"
 #include "complex.h"
 #include "geometry.h"
+#include "multigrid.h"
+#include "tables.h"
 #include "shell.h"
 #include "unsbin.h"
 #include "vtu.h"
@@ -367,6 +370,8 @@ int mhdg_mesh_desc_fill(const mhdg_mesh *m, mhdg_mesh_desc *d)
     d->edge_vec = g.edge_vec.data(); d->edge_mid = g.edge_mid.data(); d->edge_len = g.edge_len.data();
     d->ec_ptr = g.ec_ptr.data(); d->ec_cell = g.ec_cell.data(); d->ec_weight = g.ec_weight.data(); d->ef_weight = g.ef_weight.data();
     d->star1 = g.star1.data(); d->star2 = g.star2.data();
+    d->e2n = {cx.e2n.ptr.data(), cx.e2n.idx.data(), cx.e2n.sgn.data()};
+    d->n_nodes = cx.nn;
     return 0;
 }
 
@@ -530,5 +535,61 @@ int mhdg_write_pvd(const char *path, int32_t n, const double *times, const char 
         return 0;
     }
     catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+}
+
+// ---------------------------------------------------------------------------- multigrid hierarchy
+struct mhdg_mg
+{
+    mhdg::MgHierarchy h;
+};
+
+extern "C" {
+int mhdg_mg_build(const mhdg_mesh_desc *mesh, const mhdg_config *cfg, const int8_t *dirichlet, mhdg_mg **out)
+{
+    try
+    {
+        using namespace mhdg;
+        check_desc(*mesh);
+        const Tables t = build_tables(*mesh, *cfg, 32);
+        std::vector<int32_t> tail, head;
+        edge_nodes(*mesh, tail, head);
+        std::vector<int8_t> dir(size_t(mesh->n_edges), 0);
+        if (dirichlet) dir.assign(dirichlet, dirichlet + mesh->n_edges);
+        auto m = std::make_unique<mhdg_mg>();
+        m->h = build_mg_hierarchy(mesh->n_nodes, mesh->n_edges, tail.data(), head.data(), mesh->star1, mesh->n_faces, t.f2e_ptr.data(),
+                                  t.f2e.data(), mesh->star2, t.m_e.data(), dir.data(), cfg->mg_levels > 0 ? cfg->mg_levels : 4,
+                                  cfg->mg_agglom_passes > 0 ? cfg->mg_agglom_passes : 3);
+        *out = m.release();
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+void mhdg_mg_free(mhdg_mg *mg) { delete mg; }
+int mhdg_mg_levels(const mhdg_mg *mg) { return static_cast<int>(mg->h.lv.size()); }
+int mhdg_mg_level_size(const mhdg_mg *mg, int level, int32_t *n, int32_t *n_nodes, int64_t *k_nnz)
+{
+    if (level < 0 || level >= static_cast<int>(mg->h.lv.size())) { g_err = "multigrid level out of range"; return 1; }
+    const mhdg::MgLevel &L = mg->h.lv[size_t(level)];
+    if (n) *n = L.n;
+    if (n_nodes) *n_nodes = L.n_nodes;
+    if (k_nnz) *k_nnz = static_cast<int64_t>(L.k_col.size());
+    return 0;
+}
+int mhdg_mg_level_nodes(const mhdg_mg *mg, int level, int32_t *tail, int32_t *head, int32_t *agg)
+{
+    if (level < 0 || level >= static_cast<int>(mg->h.lv.size())) { g_err = "multigrid level out of range"; return 1; }
+    const mhdg::MgLevel &L = mg->h.lv[size_t(level)];
+    auto put = [](const auto &v, auto *o) { if (o) std::copy(v.begin(), v.end(), o); };
+    put(L.tail, tail); put(L.head, head); put(L.agg, agg);
+    return 0;
+}
+int mhdg_mg_level_arrays(const mhdg_mg *mg, int level, int32_t *map, double *mass, int32_t *k_ptr, int32_t *k_col, double *k_val)
+{
+    if (level < 0 || level >= static_cast<int>(mg->h.lv.size())) { g_err = "multigrid level out of range"; return 1; }
+    const mhdg::MgLevel &L = mg->h.lv[size_t(level)];
+    auto put = [](const auto &v, auto *o) { if (o) std::copy(v.begin(), v.end(), o); };
+    put(L.map, map); put(L.mass, mass); put(L.k_ptr, k_ptr); put(L.k_col, k_col); put(L.k_val, k_val);
+    return 0;
 }
 }

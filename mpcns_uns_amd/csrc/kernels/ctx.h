@@ -62,6 +62,16 @@ struct mhdg_ctx
     int restart = 2;
     mhdg::RedScratch rs{};
     mhdg::MixedSolver *mixed = nullptr;
+    // multigrid preconditioner (multigrid.hip): the host arrays the hierarchy is built from (kept when the mesh
+    // descriptor carries e2n), the device hierarchy, and the Dirichlet set it was built for
+    struct MgHost { int32_t n_nodes = 0; std::vector<int32_t> tail, head, f2e_ptr, f2e; std::vector<double> star1, star2, m_e; } mgh;
+    std::vector<int8_t> dir_mercury_h, mg_dir_h;
+    mhdg::MgDevice *mg = nullptr;
+    void mg_build(const int8_t *dir_host);
+    void mg_set_solve(const int8_t *dir_dev, float dt); // Dirichlet set (device) and step of the solve the cycles serve
+    void mg_prepare(double dt);
+    void mg_cycle(int l, const float *b, float *x);
+    void op32(int mode, int hall_on, const float *pv, const float *v, const float *b, float omega, float *out);
     // pinned host memory: the step's time step at offset 0, small read-backs staged at kPinStageOffset
     static constexpr size_t kPinBytes = 4096, kPinStageOffset = 2048, kPinStageBytes = 2048;
     void *host_pin = nullptr;
@@ -96,7 +106,10 @@ struct mhdg_ctx
                   double *out);
     int solve64(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
     void mixed_build();
-    int solve_mixed(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
+    // use_mg: the inner solve is flexible GMRES preconditioned by one V-cycle of the multigrid hierarchy built for dir_host
+    // (the host copy of dir); otherwise GMRES with the Jacobi preconditioner
+    int solve_mixed(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res,
+                    bool use_mg = false, const int8_t *dir_host = nullptr);
     double step(double dt_max, int precision, int *iters, double *res);
     void newton_alloc();
     void tangent_alloc();

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../../include/mhdg.h"
+#include "../host/multigrid.h"
 #include "../host/tables.h"
 #include "ctx.h"
 
@@ -133,6 +134,7 @@ void mhdg_ctx::prof_flush()
 mhdg_ctx::~mhdg_ctx()
 {
     delete mixed;
+    delete mg;
     for (double *p : aw.ck) (void)hipFree(p);
     for (hipEvent_t ev : prof_pool) (void)hipEventDestroy(ev);
     for (void *p : owned) (void)hipFree(p);
@@ -248,6 +250,18 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
     std::vector<int8_t> dir(E);
     for (size_t e = 0; e < E; ++e) dir[e] = c.wall_dirichlet ? m.edge_mode[e] != 0 : m.edge_mode[e] == 2;
     dir_mercury = upload(dir);
+    dir_mercury_h = dir;
+    if (m.e2n.ptr && m.e2n.idx && m.e2n.sgn && m.n_nodes > 0)
+    {
+        // what the multigrid hierarchy is built from, at its first use
+        edge_nodes(m, mgh.tail, mgh.head);
+        mgh.n_nodes = m.n_nodes;
+        mgh.star1.assign(m.star1, m.star1 + E);
+        mgh.star2.assign(m.star2, m.star2 + F);
+        mgh.f2e_ptr = t.f2e_ptr;
+        mgh.f2e = t.f2e;
+        mgh.m_e = t.m_e;
+    }
 
     auto D = [&](size_t n) { return static_cast<double *>(alloc(n * sizeof(double))); };
     Phi = D(F); PhiAdd = D(F); Badd = D(3 * C);
@@ -412,7 +426,8 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
 double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
 {
     // refusals before any launch, so that a throw leaves the resident state as it was
-    if (precision < 0 || precision > 1) throw std::runtime_error("step: precision must be 0 (FP64) or 1 (mixed)");
+    if (precision < 0 || precision > 2) throw std::runtime_error("step: precision must be 0 (FP64), 1 (mixed) or 2 (mixed with multigrid)");
+    if (precision == 2 && mgh.n_nodes <= 0) throw std::runtime_error("multigrid preconditioner: the mesh descriptor carried no e2n incidence");
     if (precision != 0 && ideal_cfg && u_ideal)
         throw std::runtime_error("mixed-precision solve: the implicit ideal part has no FP32 operator; use precision 0");
     const int nb = nblk(nc);
@@ -494,7 +509,7 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     {
         hip_check(hipMemcpyAsync(B0, Btot, 3 * size_t(nc) * 8, hipMemcpyDeviceToDevice, stream), "copy");
         if (precision == 0) *iters = solve64(Phi, E_expl, dir_mercury, gdt, E_last, res);
-        else *iters = solve_mixed(Phi, E_expl, dir_mercury, gdt, E_last, res);
+        else *iters = solve_mixed(Phi, E_expl, dir_mercury, gdt, E_last, res, precision == 2, dir_mercury_h.data());
         Euse = E_last;
     }
     else
@@ -535,6 +550,7 @@ void mhdg_config_defaults(mhdg_config *c)
     for (int m = 0; m < 5; ++m) { c->fs[0][m] = fs[m]; c->fs[1][m] = fs[m]; }
     c->fs[1][0] = 1e-6; c->fs[1][4] = 1e-6;
     c->u_inf[0] = -1; c->u_inf[1] = 0; c->u_inf[2] = 0;
+    c->mg_levels = 4; c->mg_agglom_passes = 3; c->mg_pre = 2; c->mg_post = 2; c->mg_coarse = 4; c->mg_relax = 0.7; c->mg_fine_hall = 1;
 }
 
 int mhdg_create(const mhdg_mesh_desc *mesh, const mhdg_config *cfg, mhdg_ctx **out)
@@ -831,7 +847,8 @@ int mhdg_ind_solve(mhdg_ctx *c, const double *Phi_n, const double *E_expl, const
         c->dte_cur = dt_edge ? c->stage_in(4, dt_edge, c->ne) : nullptr;
         double rr = 0;
         int it = 0;
-        try { it = precision == 0 ? c->solve64(p, ee, dd, dt, eo, &rr) : c->solve_mixed(p, ee, dd, dt, eo, &rr); }
+        if (precision < 0 || precision > 2) throw std::runtime_error("solve: precision must be 0 (FP64), 1 (mixed) or 2 (mixed with multigrid)");
+        try { it = precision == 0 ? c->solve64(p, ee, dd, dt, eo, &rr) : c->solve_mixed(p, ee, dd, dt, eo, &rr, precision == 2, dir); }
         catch (...) { c->dte_cur = nullptr; throw; }
         c->dte_cur = nullptr;
         c->d2h(E, eo, size_t(c->ne) * 8);

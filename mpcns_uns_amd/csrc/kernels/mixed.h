@@ -27,8 +27,11 @@ __global__ void k_d1_f32(DevMesh d, DevMesh32 s, const float *pv, const float *d
 constexpr int kCellLanes = 4; // lanes per cell of the FP32 cell kernels
 template <int G> __global__ void k_bcell_f32_g(DevMesh d, DevMesh32 s, const float *phi, float *b);
 template <int G> __global__ void k_hall_g_f32_g(DevMesh d, DevMesh32 s, const float *b, float *g);
+enum : int { kOpShifted = 0, kOpApply = 1, kOpSmooth = 2, kOpResid = 3 }; // output modes of k_edge_apply_f32 (mixed.hip)
+template <int MODE>
 __global__ void k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi, const float *g, const float *v,
-                                 const float *pv, const int8_t *dir, float dt, float *out, const int32_t *heavy, int32_t n_light);
+                                 const float *pv, const int8_t *dir, float dt, float *out, const int32_t *heavy, int32_t n_light, const float *b,
+                                 float omega);
 __global__ void k_tables_f32(DevMesh d, float *aV, int32_t *nbr);
 __global__ void k_multidot_f32(const float *const *V, int32_t i0, int32_t nv, const float *w, int32_t n, double *part, int32_t with_ww,
                                RedScratch rs, double *out);
@@ -39,13 +42,17 @@ __global__ void k_update_mixed(const float *const *V, const double *y, int32_t j
                                double *delta);
 __global__ void k_f64_to_f32(const double *x, int64_t n, float *out);
 __global__ void k_alpha_f32(const double *alpha, const double *B0, int32_t nc, float *aB0);
+// multigrid.hip
+__global__ void k_update_flex(const float *const *Z, const double *y, int32_t j, const int8_t *dir, int32_t n, double *delta);
+
+struct MgDevice; // multigrid.hip
 
 struct MixedSolver
 {
     DevMesh32 s{};
     float *phi = nullptr, *b = nullptr, *g = nullptr, *w = nullptr, *dte32 = nullptr, *pv = nullptr;
-    std::vector<float *> V;
-    float **Vptrs = nullptr;
+    std::vector<float *> V, Z;      // Krylov basis; Z_j = M V_j of the flexible (multigrid) inner solve
+    float **Vptrs = nullptr, **Zptrs = nullptr;
     double *h = nullptr, *part = nullptr, *part_axpy = nullptr;
     int m = 30;
     bool built = false;

@@ -74,10 +74,28 @@ __global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, DevMesh32 s, const fl
     phi[f] = sum;
 }
 
+// What the edge kernel writes from y = dt F(Phi'), F = m_e sum_f s star2_f Phi'_f + sum_c w_ec g_c . t_e (MODE, a template
+// argument, so the Krylov product is the same code as before):
+//   kOpShifted: (A P - I) v = (pv - v) + y       (pv = P v, the Jacobi-preconditioned Krylov vector)
+//   kOpApply:   A x = x + y                       (x = pv, zero on Dirichlet edges)
+//   kOpSmooth:  x + omega D^-1 (b - A x)          (a damped-Jacobi sweep of A x = b, D^-1 = invdiag)
+//   kOpResid:   b - A x
+// and 0 on Dirichlet edges in every mode.
+template <int MODE>
+__device__ __forceinline__ float edge_out(const DevMesh32 &s, int32_t e, float y, const float *v, const float *pv, const float *b, float omega)
+{
+    if (MODE == kOpShifted) return (pv[e] - v[e]) + y;
+    const float ax = pv[e] + y;
+    if (MODE == kOpApply) return ax;
+    if (MODE == kOpSmooth) return pv[e] + omega * s.invdiag[e] * (b[e] - ax);
+    return b[e] - ax;
+}
+
 // One high-valence edge (the polar axes) per block: terms in parallel, tree sums (order-free in FP32)
+template <int MODE>
 __device__ __forceinline__ void edge_apply_f32_heavy(const DevMesh &d, const DevMesh32 &s, int32_t resistive, int32_t hall, int32_t e,
                                                      const float *phi, const float *g, const float *v, const float *pv, const int8_t *dir,
-                                                     float dt, float *out)
+                                                     float dt, float *out, const float *b, float omega)
 {
     __shared__ float sh[2][4];
     float p0 = 0.0f, p1 = 0.0f;
@@ -98,7 +116,7 @@ __device__ __forceinline__ void edge_apply_f32_heavy(const DevMesh &d, const Dev
     if (threadIdx.x == 0)
     {
         const float P0 = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3], P1 = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
-        out[e] = dir[e] ? 0.0f : (pv[e] - v[e]) + dt * (s.m_e[e] * P0 + P1);
+        out[e] = dir[e] ? 0.0f : edge_out<MODE>(s, e, dt * (s.m_e[e] * P0 + P1), v, pv, b, omega);
     }
 }
 
@@ -108,13 +126,14 @@ __device__ __forceinline__ void edge_apply_f32_heavy(const DevMesh &d, const Dev
 // together (the gather depends on the index, so a loop of one entry at a time keeps about one load in flight per
 // wave: 403 against 288 us per launch on the 8M shell); the sums still run in entry order.
 // Blocks from n_light on take one high-valence edge each (heavy[block - n_light]) in the same launch.
+template <int MODE>
 __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi,
                                                         const float *g, const float *v, const float *pv, const int8_t *dir, float dt,
-                                                        float *out, const int32_t *heavy, int32_t n_light)
+                                                        float *out, const int32_t *heavy, int32_t n_light, const float *b, float omega)
 {
     if (int32_t(blockIdx.x) >= n_light)
     {
-        edge_apply_f32_heavy(d, s, resistive, hall, heavy[blockIdx.x - n_light], phi, g, v, pv, dir, dt, out);
+        edge_apply_f32_heavy<MODE>(d, s, resistive, hall, heavy[blockIdx.x - n_light], phi, g, v, pv, dir, dt, out, b, omega);
         return;
     }
     const int32_t e = xcd_block(uint32_t(n_light)) * int32_t(blockDim.x) + int32_t(threadIdx.x);
@@ -157,8 +176,20 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, 
                 if (k + u < c1) p1 += __int_as_float(q[u].y) * fdot(x[u], t);
         }
     }
-    out[e] = (pv[e] - v[e]) + dt * (s.m_e[e] * p0 + p1);
+    out[e] = edge_out<MODE>(s, e, dt * (s.m_e[e] * p0 + p1), v, pv, b, omega);
 }
+template __global__ void k_edge_apply_f32<kOpShifted>(DevMesh, DevMesh32, int32_t, int32_t, const float *, const float *, const float *,
+                                                      const float *, const int8_t *, float, float *, const int32_t *, int32_t, const float *,
+                                                      float);
+template __global__ void k_edge_apply_f32<kOpApply>(DevMesh, DevMesh32, int32_t, int32_t, const float *, const float *, const float *,
+                                                    const float *, const int8_t *, float, float *, const int32_t *, int32_t, const float *,
+                                                    float);
+template __global__ void k_edge_apply_f32<kOpSmooth>(DevMesh, DevMesh32, int32_t, int32_t, const float *, const float *, const float *,
+                                                     const float *, const int8_t *, float, float *, const int32_t *, int32_t, const float *,
+                                                     float);
+template __global__ void k_edge_apply_f32<kOpResid>(DevMesh, DevMesh32, int32_t, int32_t, const float *, const float *, const float *,
+                                                    const float *, const int8_t *, float, float *, const int32_t *, int32_t, const float *,
+                                                    float);
 
 // Cell kernels with G lanes per cell: the lanes read the cell's (cell, face) table entries side by side (coalesced
 // across the wavefront) and combine their partial sums with shuffles (G = 4 measured 1.2-1.3x faster than one
@@ -662,7 +693,8 @@ void mhdg_ctx::mixed_build()
     M.built = true;
 }
 
-int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *dir, double dt, double *Eout, double *rel_res)
+int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *dir, double dt, double *Eout, double *rel_res, bool use_mg,
+                          const int8_t *dir_host)
 {
     mixed_build();
     MixedSolver &M = *mixed;
@@ -671,6 +703,13 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
     const double rtol = cfg.ind_rtol;
     const double inner_rtol = kInnerRtol;
     if (ideal_cfg && u_ideal) throw std::runtime_error("mixed-precision solve: the implicit ideal part has no FP32 operator; use precision 0");
+    if (use_mg)
+    {
+        // the coarse operators are built for a scalar dt
+        if (dte_cur) throw std::runtime_error("multigrid preconditioner: per-edge time steps are not supported; use precision 1");
+        if (!dir_host) throw std::runtime_error("multigrid preconditioner: no host copy of the Dirichlet set");
+        mg_build(dir_host);
+    }
     const float *dte32 = nullptr;
     if (dte_cur)
     {
@@ -683,6 +722,11 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
     LAUNCH("solve_init", k_solve_init, dim3(nb(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt, dte_cur, ne, x0,
            invdiag);
     LAUNCH("f64_to_f32", k_f64_to_f32, dim3(nb(ne)), dim3(256), 0, stream, invdiag, int64_t(ne), const_cast<float *>(M.s.invdiag));
+    if (use_mg)
+    {
+        mg_set_solve(dir, static_cast<float>(dt));
+        mg_prepare(dt);
+    }
     apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp);
     LAUNCH("axpy", k_add, dim3(nb(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
     chk(hipMemsetAsync(delta, 0, size_t(ne) * 8, stream), "memset");
@@ -716,15 +760,25 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
         int j = 0;
         for (; j < m && total < max_it; ++j, ++total)
         {
-            // w = A P V_j in FP32
-            LAUNCH("d1_f32", k_d1_f32, dim3(nb(nf)), dim3(256), 0, stream, d, M.s, M.pv, dte32, M.phi);
-            if (op_hall)
+            if (use_mg)
             {
-                LAUNCH("bcell_f32", k_bcell_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.phi, M.b);
-                LAUNCH("hall_g_f32", k_hall_g_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.b, M.g);
+                // flexible: Z_j = M V_j (one V-cycle), w = A Z_j
+                mg_cycle(0, M.V[j], M.Z[j]);
+                op32(kOpApply, op_hall ? 1 : 0, M.Z[j], nullptr, nullptr, 0.0f, M.w);
             }
-            LAUNCH("edge_apply_f32", k_edge_apply_f32, dim3(nb(ne) + n_heavy), dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(op_hall),
-                   M.phi, M.g, M.V[j], M.pv, dir, dtf, M.w, static_cast<const int32_t *>(heavy), int32_t(nb(ne)));
+            else
+            {
+                // w = (A P - I) V_j in FP32
+                LAUNCH("d1_f32", k_d1_f32, dim3(nb(nf)), dim3(256), 0, stream, d, M.s, M.pv, dte32, M.phi);
+                if (op_hall)
+                {
+                    LAUNCH("bcell_f32", k_bcell_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.phi, M.b);
+                    LAUNCH("hall_g_f32", k_hall_g_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.b, M.g);
+                }
+                LAUNCH("edge_apply_f32", k_edge_apply_f32<kOpShifted>, dim3(nb(ne) + n_heavy), dim3(256), 0, stream, d, M.s, int32_t(op_res),
+                       int32_t(op_hall), M.phi, M.g, M.V[j], M.pv, dir, dtf, M.w, static_cast<const int32_t *>(heavy), int32_t(nb(ne)),
+                       static_cast<const float *>(nullptr), 0.0f);
+            }
             // classical Gram-Schmidt against V_0..V_j; the update also normalises V_{j+1} unless the DGKS test asks
             // for a second projection
             for (int i0 = 0; i0 <= j; i0 += kDotChunk)
@@ -753,7 +807,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
                 LAUNCH("scale", k_scale_f32, dim3(nb(ne)), dim3(256), 0, stream, M.w, M.h + j + 1, M.s.invdiag, dir, ne, M.V[j + 1], M.pv);
             }
             for (int i = 0; i <= j; ++i) H[i][j] = hc[i];
-            H[j][j] += 1.0; // Arnoldi ran on A P - I
+            if (!use_mg) H[j][j] += 1.0; // Arnoldi ran on A P - I
             H[j + 1][j] = hn;
             for (int i = 0; i < j; ++i)
             {
@@ -775,7 +829,10 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
             yv[i] = s / H[i][i];
         }
         chk(hipMemcpyAsync(ydev, yv.data(), sizeof(double) * std::max(j, 1), hipMemcpyHostToDevice, stream), "H2D");
-        LAUNCH("update_mixed", k_update_mixed, dim3(nb(ne)), dim3(256), 0, stream, Vp, ydev, j, M.s.invdiag, dir, ne, delta);
+        if (use_mg)
+            LAUNCH("update_flex", k_update_flex, dim3(nb(ne)), dim3(256), 0, stream, static_cast<const float *const *>(M.Zptrs), ydev, j, dir, ne, delta);
+        else
+            LAUNCH("update_mixed", k_update_mixed, dim3(nb(ne)), dim3(256), 0, stream, Vp, ydev, j, M.s.invdiag, dir, ne, delta);
     }
     LAUNCH("solve_final", k_solve_final, dim3(nb(ne)), dim3(256), 0, stream, x0, delta, Eex, dir, ne, Eout, corr);
     check_launch("mixed solve");

@@ -146,6 +146,21 @@ class Device:
                                       ptr(self._in(dt_edge, 'e')), precision, ptr(E), C.byref(it), C.byref(rr)))
         return E, it.value, rr.value
 
+    def mg_apply(self, v, dirichlet, dt):
+        """One V-cycle z = M v of the multigrid preconditioner (precision 2) with the frozen state of ind_set_frozen,
+        for the Dirichlet set and time step given (the cycle the flexible solve applies to each Krylov vector)."""
+        z = np.zeros(self.ne)
+        d = self._in(np.asarray(dirichlet) != 0, 'e', np.int8)
+        self._ok(lib().mhdg_mg_apply(self._h, ptr(d), float(dt), ptr(self._in(v, 'e')), ptr(z)))
+        return z
+
+    def mg_info(self, dirichlet):
+        """Unknowns per level of the multigrid hierarchy for this Dirichlet set (built on first use)."""
+        lv, n = C.c_int32(), np.zeros(16, np.int32)
+        d = self._in(np.asarray(dirichlet) != 0, 'e', np.int8)
+        self._ok(lib().mhdg_mg_info(self._h, ptr(d), 16, C.byref(lv), ptr(n)))
+        return n[:lv.value].tolist()
+
     def ind_apply_G(self, phi):
         """Induction operator with the Hall field B_add + R phi of phi itself (quadratic in phi)."""
         out = np.zeros(self.ne)

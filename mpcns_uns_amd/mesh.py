@@ -143,6 +143,38 @@ class Mesh:
             self._desc = d
         return self._desc
 
+    def mg_hierarchy(self, cfg, dirichlet=None):
+        """Levels of the multigrid hierarchy the device builds for this mesh and Dirichlet set (mhdg_mg_build): a list of
+        dicts with n, n_nodes, mass, the end nodes tail/head of every unknown and, from level 1 on, agg (the aggregate of
+        every finer node), map (per finer unknown: signed index, v < 0 = -1 on ~v, INT32_MIN none) and the CSR of K_l
+        (k_ptr, k_col, k_val)."""
+        d = None if dirichlet is None else np.ascontiguousarray(np.asarray(dirichlet) != 0, dtype=np.int8)
+        if d is not None and d.size != self.n_edges:
+            raise ValueError("dirichlet needs one entry per edge")
+        h = C.c_void_p()
+        if lib().mhdg_mg_build(C.byref(self.desc()), C.byref(cfg), ptr(d), C.byref(h)):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+        try:
+            out = []
+            for lvl in range(lib().mhdg_mg_levels(h)):
+                n, nn, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+                lib().mhdg_mg_level_size(h, lvl, C.byref(n), C.byref(nn), C.byref(nnz))
+                L = dict(n=n.value, n_nodes=nn.value, mass=np.zeros(n.value), tail=np.zeros(n.value, np.int32),
+                         head=np.zeros(n.value, np.int32))
+                if lvl > 0:
+                    L["agg"] = np.zeros(out[-1]["n_nodes"], np.int32)
+                lib().mhdg_mg_level_nodes(h, lvl, ptr(L["tail"]), ptr(L["head"]), ptr(L.get("agg")))
+                if lvl == 0:
+                    lib().mhdg_mg_level_arrays(h, lvl, None, ptr(L["mass"]), None, None, None)
+                else:
+                    L.update(map=np.zeros(out[-1]["n"], np.int32), k_ptr=np.zeros(n.value + 1, np.int32),
+                             k_col=np.zeros(nnz.value, np.int32), k_val=np.zeros(nnz.value))
+                    lib().mhdg_mg_level_arrays(h, lvl, ptr(L["map"]), ptr(L["mass"]), ptr(L["k_ptr"]), ptr(L["k_col"]), ptr(L["k_val"]))
+                out.append(L)
+            return out
+        finally:
+            lib().mhdg_mg_free(h)
+
     def background_dipoles(self, B_imf, positions, moments, cores=None, cutoff=0.0):
         """Background of the IMF plus a crustal field of point dipoles (MarsSolver::build_background_ recipe)."""
         pos, mom = f64(positions).reshape(-1, 3), f64(moments).reshape(-1, 3)

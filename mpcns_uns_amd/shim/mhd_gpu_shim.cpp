@@ -22,7 +22,7 @@
 // of the device call, with the first-order cell state as `inside` (exact for ghost functions that ignore the
 // inside state, which is the only use MarsSolver makes of it: MarsSolver.cpp:124-128, 148-167).
 // The implicit solve runs in FP64 (the reference iteration) unless MHDG_SOLVE_PRECISION=1 selects the
-// mixed-precision solve.
+// mixed-precision solve (2: mixed precision with the multigrid preconditioner).
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -154,6 +154,7 @@ mhdg_ctx *context(MeshRec &r)
     d.edge_vec = d3(g.edge_vec); d.edge_mid = d3(g.edge_mid); d.edge_len = g.edge_len.data();
     d.ec_ptr = g.ec_ptr.data(); d.ec_cell = g.ec_cell.data(); d.ec_weight = g.ec_weight.data(); d.ef_weight = g.ef_weight.data();
     d.star1 = g.star1.data(); d.star2 = g.star2.data();
+    d.e2n = inc(cx.e2n); d.n_nodes = cx.n_nodes();
     mhdg_config c;
     if (r.cfg) c = to_abi(*r.cfg);
     else mhdg_config_defaults(&c);
@@ -211,7 +212,8 @@ std::vector<double> ghost_table(const uns::LocalMesh &lm, const uns::Geometry &g
 int solve_precision()
 {
     const char *p = std::getenv("MHDG_SOLVE_PRECISION");
-    return (p && std::atoi(p) == 1) ? 1 : 0;
+    const int v = p ? std::atoi(p) : 0;
+    return (v == 1 || v == 2) ? v : 0;
 }
 
 // ImplicitInduction instances -> a context of their own (each instance keeps its own warm-start correction,

@@ -32,6 +32,16 @@ int mhdg_mesh_from_cells(int32_t n_nodes, const double *xyz, int64_t n_cells, co
                          int64_t n_tags, const int32_t *tag_ptr, const int32_t *tag_nodes, const int32_t *tag_id,
                          mhdg_mesh **out);
 void mhdg_mesh_free(mhdg_mesh *m);
+
+/* Partition-ordered numbering for one GPU.  mhdg_mesh_hilbert_order: the cells sorted by the Hilbert key of their
+ * centres (the reference's SFC partitioner, src/uns/partition/Partitioner.cpp:37-108: 20-bit coordinates in the
+ * bounding box of the centres, Skilling's transpose, ties by id): order[k] = the cell placed at k.
+ * mhdg_mesh_renumber_cells: the same mesh (nodes, orientation, tags) with its cells in that order and its faces and
+ * edges numbered by first appearance, the reference builder's rule (ComplexBuilder.cpp:45-134); every cell keeps
+ * its face loops, so a face keeps its loop and an edge its node pair: new c2f row k = old c2f row order[k] up to
+ * the face ids, with equal signs.  Tetrahedral shells (mhdg_mesh_shell) only. */
+int mhdg_mesh_hilbert_order(const mhdg_mesh *m, int32_t *order);
+int mhdg_mesh_renumber_cells(const mhdg_mesh *m, const int32_t *order, mhdg_mesh **out);
 const char *mhdg_mesh_error(void);
 
 /* counts[0..7] = nodes, edges, faces, cells, nnz(f2e), nnz(c2f), nnz(e2f), nnz(edge->cell) */

@@ -76,6 +76,8 @@ def lib():
         L.mhdg_mesh_shell.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D, D, C.c_uint64, C.POINTER(P)]
         L.mhdg_mesh_from_cells.argtypes = [I32, P, I64, P, P, P, P, I64, P, P, P, C.POINTER(P)]
         L.mhdg_mesh_free.argtypes = [P]
+        L.mhdg_mesh_hilbert_order.argtypes = [P, P]
+        L.mhdg_mesh_renumber_cells.argtypes = [P, P, C.POINTER(P)]
         L.mhdg_mesh_counts.argtypes = [P, P]
         L.mhdg_mesh_array.argtypes = [P, C.c_char_p, C.POINTER(P), C.POINTER(I64), C.POINTER(I32)]
         L.mhdg_mesh_boundary.argtypes = [P, P, I32, I32, P]

@@ -16,7 +16,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("MHDG_ARCH", "gfx950")
 
-HOST_SRC = ["host/complex.cpp", "host/geometry.cpp", "host/shell.cpp", "host/mesh_capi.cpp", "host/tables.cpp", "host/unsbin.cpp", "host/vtu.cpp", "host/multigrid.cpp"]
+HOST_SRC = ["host/complex.cpp", "host/geometry.cpp", "host/shell.cpp", "host/mesh_capi.cpp", "host/tables.cpp", "host/unsbin.cpp", "host/vtu.cpp", "host/multigrid.cpp", "host/sfc.cpp"]
 HIP_SRC = ["kernels/explicit.hip", "kernels/induction.hip", "kernels/mixed.hip", "kernels/tangent.hip", "kernels/adjoint.hip", "kernels/multigrid.hip", "kernels/ctx.hip"]
 FP = ["-ffp-contract=off", "-fno-fast-math"]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", *FP,

@@ -13,6 +13,7 @@
 #include "complex.h"
 #include "geometry.h"
 #include "multigrid.h"
+#include "sfc.h"
 #include "tables.h"
 #include "shell.h"
 #include "unsbin.h"
@@ -247,6 +248,53 @@ int mhdg_mesh_from_cells(int32_t n_nodes, const double *xyz, int64_t n_cells, co
 }
 
 void mhdg_mesh_free(mhdg_mesh *m) { delete m; }
+
+int mhdg_mesh_hilbert_order(const mhdg_mesh *m, int32_t *order)
+{
+    try
+    {
+        const std::vector<int32_t> o = mhdg::hilbert_order(m->g.cell_center.data(), m->cx.nc);
+        std::copy(o.begin(), o.end(), order);
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
+
+int mhdg_mesh_renumber_cells(const mhdg_mesh *m, const int32_t *order, mhdg_mesh **out)
+{
+    try
+    {
+        const int64_t nc = m->cx.nc;
+        if (m->tets.size() != size_t(4 * nc)) throw std::runtime_error("renumber: only tetrahedral meshes built by mhdg_mesh_shell");
+        std::vector<char> seen(size_t(nc), 0);
+        auto r = std::make_unique<mhdg_mesh>();
+        r->xyz = m->xyz;
+        r->tets.resize(m->tets.size());
+        for (int64_t k = 0; k < nc; ++k)
+        {
+            const int32_t c = order[k];
+            if (c < 0 || c >= nc || seen[size_t(c)]) throw std::runtime_error("renumber: order is not a permutation of the cells");
+            seen[size_t(c)] = 1;
+            std::copy(m->tets.begin() + 4 * int64_t(c), m->tets.begin() + 4 * int64_t(c) + 4, r->tets.begin() + 4 * k);
+        }
+        // tagged faces by their nodes: the builder finds them again in the new numbering
+        std::vector<int32_t> tags;
+        for (int32_t f = 0; f < m->cx.nf; ++f)
+        {
+            if (!m->cx.face_tag[size_t(f)]) continue;
+            int32_t nodes[8];
+            int n = 0;
+            m->cx.face_loop(f, nodes, n);
+            if (n != 3) throw std::runtime_error("renumber: tagged face is not a triangle");
+            tags.insert(tags.end(), {nodes[0], nodes[1], nodes[2], m->cx.face_tag[size_t(f)]});
+        }
+        r->cx = mhdg::build_tet_complex(m->cx.nn, r->tets.data(), nc, tags.data(), static_cast<int64_t>(tags.size() / 4));
+        r->tag_names = m->tag_names;
+        *out = finish_mesh(r.release());
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
 
 int mhdg_mesh_counts(const mhdg_mesh *m, int64_t c[8])
 {

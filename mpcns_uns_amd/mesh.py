@@ -37,6 +37,33 @@ class Mesh:
             raise RuntimeError(lib().mhdg_mesh_error().decode())
         return cls(h)
 
+    def hilbert_order(self):
+        """Cells in the order of the Hilbert key of their centres (mhdg_mesh_hilbert_order): order[k] = old cell id."""
+        order = np.zeros(self.n_cells, np.int32)
+        if lib().mhdg_mesh_hilbert_order(self._h, ptr(order)):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+        return order
+
+    def renumbered(self, order):
+        """The same mesh with its cells in `order` and faces and edges numbered by first appearance
+        (mhdg_mesh_renumber_cells), and the maps back: (mesh, cell_map, face_map, edge_map), x_map[new id] = old id.
+        The boundary classification is not carried over: call boundary() on the new mesh."""
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        if order.size != self.n_cells:
+            raise ValueError("order needs one entry per cell")
+        h = C.c_void_p()
+        if lib().mhdg_mesh_renumber_cells(self._h, ptr(order), C.byref(h)):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+        m = Mesh(h)
+        # a cell keeps its loops, a face its loop start: rows correspond entry by entry
+        face_map = np.empty(self.n_faces, np.int32)
+        old_c2f = self.array("c2f.idx").reshape(self.n_cells, -1)
+        face_map[m.array("c2f.idx").reshape(m.n_cells, -1)] = old_c2f[order]
+        edge_map = np.empty(self.n_edges, np.int32)
+        old_f2e = self.array("f2e.idx").reshape(self.n_faces, -1)
+        edge_map[m.array("f2e.idx").reshape(m.n_faces, -1)] = old_f2e[face_map]
+        return m, order.copy(), face_map, edge_map
+
     @classmethod
     def from_cells(cls, xyz, cell_loops, material=None, tags=()):
         """cell_loops: per cell a list of outward node loops; tags: iterable of (nodes, tag_id)."""

@@ -54,8 +54,16 @@ def physics_config(hall, **kw):
     return default_config(**base)
 
 
-def build(name=None, net=None, hall=None, seed=None, noise=0.01, **cfg_kw):
-    """Mesh, config, initial state and background of a workload (or of a custom net size)."""
+def build(name=None, net=None, hall=None, seed=None, noise=0.01, order=None, **cfg_kw):
+    """Mesh, config, initial state and background of a workload (or of a custom net size).
+
+    order="hilbert": the same workload on the mesh renumbered in the Hilbert order of the cell centres (cells in key
+    order, faces and edges by first appearance); the state and background are drawn on the generator's numbering and
+    permuted, so every entity carries the same values.  w["maps"] = (cell_map, face_map, edge_map), new id -> old."""
+    if order not in (None, "hilbert"):
+        raise ValueError("order must be None or 'hilbert'")
+    if order == "hilbert":
+        return renumber(build(name, net, hall, seed, noise, None, **cfg_kw))
     spec = dict(WORKLOADS[name]) if name else dict(index=0, net=net, hall=bool(hall))
     if net is not None:
         spec["net"] = net
@@ -105,6 +113,23 @@ def _build_mars(spec, seed, noise, **cfg_kw):
     states, Phi = mesh.seeded_species_state(cfg, fs, noise, seed, Badd)
     return dict(mesh=mesh, cfg=cfg, state=(states[0], states[1], Phi), species_state=states, background=(PhiAdd, Badd),
                 dipoles=(pos, mom), seed=seed, spec=spec)
+
+
+def renumber(w):
+    """Workload w on its mesh renumbered in Hilbert order (Mesh.hilbert_order, Mesh.renumbered): same boundary rule,
+    state and background permuted entity by entity."""
+    m = w["mesh"]
+    m2, cm, fm, em = m.renumbered(m.hilbert_order())
+    m2.boundary({1: "wall", 2: "farfield"}, u_inf=U_INF)
+    cells5 = lambda U: np.ascontiguousarray(U.reshape(-1, 5)[cm].ravel())
+    UH, UNa, Phi = w["state"]
+    PhiAdd, Badd = w["background"]
+    out = dict(w, mesh=m2, state=(cells5(UH), cells5(UNa), np.ascontiguousarray(Phi[fm])),
+               background=(np.ascontiguousarray(PhiAdd[fm]), np.ascontiguousarray(Badd.reshape(-1, 3)[cm].ravel())),
+               maps=(cm, fm, em), order="hilbert")
+    if "species_state" in w:
+        out["species_state"] = [cells5(U) for U in w["species_state"]]
+    return out
 
 
 def load(dev, w):

@@ -248,6 +248,7 @@ class Reference:
                 "ref_write_fields": [C.c_char_p, C.c_uint64, C.c_int, _P, _P, _P, _P, _P],
                 "ref_read_field": [C.c_char_p, C.c_char_p, _P, _P, _P, _P, _P],
                 "ref_dipole_edge_integrals": [C.c_int, _P, _P, _P, _D, C.c_int64, _P, _P, _P],
+                "ref_hilbert_order": [C.c_int32, _P, _P],
             }.items():
                 getattr(L, name).argtypes = args
             L.ref_ind_solve.restype = C.c_int
@@ -477,3 +478,12 @@ def reference_for_mesh(mesh, cfg, dipole_m=None, dipole_c=(0, 0, 0), soft=1e-3, 
         tags.append((nodes, int(ftag[f])))
     cls = ReferenceOnDevice if on_device else Reference
     return cls(xyz, tet_cell_loops(tets), cfg, tags=tags, dipole_m=dipole_m, dipole_c=dipole_c, soft=soft)
+
+
+def reference_hilbert_order(centres):
+    """The reference partitioner's Hilbert order of points (uns::partition_cells, one part per point)."""
+    c = _f(centres).reshape(-1, 3)
+    order = np.zeros(len(c), np.int32)
+    if Reference.lib().ref_hilbert_order(len(c), _ptr(c), _ptr(order)):
+        raise RuntimeError("ref_hilbert_order failed")
+    return order

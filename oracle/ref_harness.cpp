@@ -32,6 +32,7 @@
 #include "uns/geometry/Reconstruct.h"
 #include "uns/io/MeshIO.h"
 #include "uns/io/VtkWriter.h"
+#include "uns/partition/Partitioner.h"
 #include "MarsUns/physics/MarsDipoles.h"
 
 #include <cmath>
@@ -433,6 +434,34 @@ int ref_read_field(const char *path, const char *name, double *out, int64_t *cou
         return 3;
     }
     catch (const std::exception &e) { std::fprintf(stderr, "ref_read_field: %s\n", e.what()); return 1; }
+}
+
+// uns::partition_cells with the Hilbert method (src/uns/partition/Partitioner.cpp:37-108, 188-211) on n unit-weight
+// cells at the given centres, cut into n parts without refinement: the part of each cell is its position in the
+// Hilbert order, so order[part[c]] = c.
+int ref_hilbert_order(int32_t n, const double *centres, int32_t *order)
+{
+    try
+    {
+        CellGraph g;
+        g.xadj.assign(size_t(n) + 1, 0);
+        g.vw.assign(size_t(n), 1.0);
+        g.centre.resize(size_t(n));
+        for (int32_t i = 0; i < n; ++i) g.centre[size_t(i)] = {centres[3 * size_t(i)], centres[3 * size_t(i) + 1], centres[3 * size_t(i) + 2]};
+        PartOptions opt;
+        opt.method = PartMethod::Hilbert;
+        opt.refine_passes = 0;
+        const std::vector<int> part = partition_cells(g, n, opt);
+        std::vector<char> hit(size_t(n), 0);
+        for (int32_t c = 0; c < n; ++c)
+        {
+            if (part[size_t(c)] < 0 || part[size_t(c)] >= n || hit[size_t(part[size_t(c)])]) return 2;
+            hit[size_t(part[size_t(c)])] = 1;
+            order[part[size_t(c)]] = c;
+        }
+        return 0;
+    }
+    catch (const std::exception &e) { std::fprintf(stderr, "ref_hilbert_order: %s\n", e.what()); return 1; }
 }
 
 // MARS::DipoleModel (MarsUns/physics/MarsDipoles.cpp): the reference's crustal-dipole edge integrals, the model read

@@ -27,7 +27,7 @@ PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 # HBM bytes per launch moved by a kernel on a workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by
 # the same counters on a 1 GiB device copy (x2.00 and x1.00); the command and the per-kernel table are in
 # profiles/hall_step_shell_8m_hbm_counters.md.  Counter runs cannot share a process with this measurement.
-PMC_TRAFFIC = {("shell_8m_mixed", "edge_apply_f32"): 1376126198}
+PMC_TRAFFIC = {("shell_8m_mixed", "lat", "edge_apply_f32"): 1376126198}
 
 
 def kernel_bytes(mesh, name, precision_bytes=8, ns=2):
@@ -63,8 +63,48 @@ def kernel_bytes(mesh, name, precision_bytes=8, ns=2):
         "hall_g_f32": 12 * C + 4 * (C + 1) + 4 * nc2f + 8 * F + 12 * F + 4 * C + 12 * C + 12 * C,
         # dir, e2f ptr, (face, signed star2) pairs, phi, edge vec, ec ptr, (cell, weight) pairs, g, m_e, pv, v, out
         "edge_apply_f32": E + 4 * (E + 1) + 8 * nf2e + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 4,
+        # the same gather with the multigrid outputs: A x (x, out), a Jacobi sweep or a residual (x, b, invdiag, out)
+        "mg_apply_f32": E + 4 * (E + 1) + 8 * nf2e + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 3,
+        "mg_smooth_f32": E + 4 * (E + 1) + 8 * nf2e + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 5,
+        "mg_resid_f32": E + 4 * (E + 1) + 8 * nf2e + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 4,
     }
     return table.get(name)
+
+
+KERNEL_CLASSES = {
+    "explicit": ("prepare", "dt_final", "grad_current", "face", "face_emf", "cell_update", "edge_emf", "edge_emf_heavy", "faraday",
+                 "gyrate"),
+    "operator": ("d1", "bcell", "hall_g", "edge_apply", "edge_apply_heavy", "d1_f32", "bcell_f32", "hall_g_f32", "edge_apply_f32",
+                 "mg_apply_f32"),
+    "orthogonalisation": ("mgs", "finish", "dot", "multidot", "multidot_final", "multiaxpy", "scale"),
+    "multigrid": ("mg_first", "mg_smooth_f32", "mg_resid_f32", "mg_sweep", "mg_resid", "mg_restrict", "mg_prolong", "mg_dinv"),
+}
+
+
+def split_by_class(breakdown, steps):
+    """Milliseconds per step of each kernel class (explicit stage, Krylov operator, orthogonalisation, multigrid,
+    the rest) from a per-kernel breakdown over `steps` steps."""
+    out = {k: 0.0 for k in (*KERNEL_CLASSES, "other")}
+    for name, (_, ms) in (breakdown or {}).items():
+        cls = next((k for k, names in KERNEL_CLASSES.items() if name in names), "other")
+        out[cls] += ms / max(steps, 1)
+    return {k: round(v, 3) for k, v in out.items()}
+
+
+def stream_calibration(gib=2.0):
+    """The device's streaming rates (GB/s) from the library's read, copy and triad kernels over `gib` GiB arrays."""
+    import ctypes as C
+
+    from mpcns_uns_amd._lib import lib
+
+    out = {}
+    for kind, name in enumerate(("read", "copy", "triad")):
+        us, moved = C.c_double(), C.c_double()
+        if lib().mhdg_stream_calibration(kind, int(gib * 2 ** 30), 5, C.byref(us), C.byref(moved)):
+            return None
+        out[name + "_GBps"] = round(moved.value / (us.value * 1e-6) / 1e9, 1)
+    out["array_GiB"] = gib
+    return out
 
 
 def run_steps(dev, steps, precision, dt_max, restore=True):
@@ -120,6 +160,10 @@ def differentiation(args):
                       "adjoint_vs_tangent_rel": agree}))
 
 
+PRECISIONS = {"fp64": 0, "mixed": 1, "mg": 2}
+PRECONDITIONER = {0: "jacobi", 1: "jacobi", 2: "multigrid"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,12 +171,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="shell_8m_mixed")
     ap.add_argument("--net", type=int, nargs=3, default=None, help="override the lat-lon net (n_lat n_lon n_r)")
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=sorted(PRECISIONS),
+                    help="fp64: the reference iteration; mixed: FP32 GMRES with Jacobi in FP64 refinement; mg: the same "
+                         "with the agglomeration-multigrid V-cycle as preconditioner")
+    ap.add_argument("--order", default="lat", choices=["lat", "hilbert"],
+                    help="cell numbering: the generator's lat-row order or the Hilbert order of the cell centres")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 step and the other preconditioner")
     ap.add_argument("--no-fp64-reference", action="store_true", help="skip the plain FP64 step measurement")
     ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     ap.add_argument("--cpu-net", type=int, nargs=3, default=(96, 192, 10))
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-stream", dest="stream", action="store_false", default=True,
+                    help="skip the streaming calibration (read, copy, triad kernels)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a device-to-device copy of GIB GiB five times (scale of the HBM byte counters)")
     ap.add_argument("--differentiation", action="store_true",
@@ -155,7 +206,6 @@ def main():
             ev[i + 1].record()
         torch.cuda.synchronize()
         us = sorted(ev[i].elapsed_time(ev[i + 1]) * 1e3 for i in range(5))
-        # the practical HBM ceiling of a streaming kernel on this device: read + write bytes over the median copy
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "repeats": 5, "median_us": round(us[2], 1),
                           "copy_GBps": round(8 * n / (us[2] * 1e-6) / 1e9, 1)}))
         return
@@ -167,22 +217,23 @@ def main():
     from mpcns_uns_amd import Device, workloads
 
     t0 = time.time()
-    w = workloads.build(args.workload, net=tuple(args.net) if args.net else None)
+    w = workloads.build(args.workload, net=tuple(args.net) if args.net else None, order=None if args.order == "lat" else "hilbert")
     mesh, cfg = w["mesh"], w["cfg"]
     setup_s = time.time() - t0
-    print(f"# workload {args.workload} cells={mesh.n_cells} faces={mesh.n_faces} edges={mesh.n_edges} setup={setup_s:.1f}s",
-          file=sys.stderr, flush=True)
+    print(f"# workload {args.workload} ({args.order} order) cells={mesh.n_cells} faces={mesh.n_faces} edges={mesh.n_edges} "
+          f"setup={setup_s:.1f}s", file=sys.stderr, flush=True)
     dev = Device(mesh, cfg)
     ns = max(int(cfg.n_species), 2)
-    precision = 1 if args.precision == "mixed" else 0
+    precision = PRECISIONS[args.precision]
     dt_max = 1e9
 
     def measure(prec, profile, restore=True):
         """Warmup steps with every kernel timed (the per-kernel breakdown), then the timed steps with HIP events
-        around the dominant kernel only (its launch durations for the roofline)."""
+        around the dominant kernel only (its launch durations for the roofline).  Returns the state after the last
+        step as well (with restore: the first step's result)."""
         workloads.load(dev, w)
         dev.state_save()
-        # the first warmup step carries one-time work (solver set-up, first touch): not in the breakdown
+        # the first warmup step carries one-time work (solver set-up, hierarchy build, first touch): not in the breakdown
         first = 1 if args.warmup > 1 else 0
         run_steps(dev, first, prec, dt_max, restore)
         if profile:
@@ -212,24 +263,51 @@ def main():
             if not breakdown:
                 breakdown = timed
                 dom = max(timed, key=lambda k: timed[k][1] if kernel_bytes(mesh, k, ns=ns) else -1)
-        return el, its, res, (breakdown, dom, timed), dts
+        return dict(el=el, its=its, res=res, prof=(breakdown, dom, timed), dts=dts, state=dev.get_state())
+
+    nbd = args.warmup - 1 if args.warmup > 1 else args.warmup  # steps in a breakdown
+
+    def summary(m):
+        return dict(value=round(mesh.n_cells * args.steps / m["el"], 1), ms_per_step=round(m["el"] / args.steps * 1e3, 3),
+                    gmres_iterations_per_step=float(np.mean(m["its"])), max_rel_residual=float(max(m["res"])),
+                    ms_per_step_by_class=split_by_class(m["prof"][0], nbd))
 
     # main measurement; the timed region carries events around the dominant kernel's launches only
-    el, its, res, prof, _ = measure(precision, True)
+    main_m = measure(precision, True)
+    el, its, res, prof = main_m["el"], main_m["its"], main_m["res"], main_m["prof"]
     value = mesh.n_cells * args.steps / el
     print(f"# {args.precision}: {el / args.steps * 1e3:.2f} ms/step, GMRES it/step {np.mean(its):.1f}, max res {max(res):.2e}",
           file=sys.stderr, flush=True)
-    fp64 = None
-    if precision == 1 and not args.no_fp64_reference:
-        el64, its64, res64, _, _ = measure(0, False)
-        fp64 = dict(value=mesh.n_cells * args.steps / el64, ms_per_step=el64 / args.steps * 1e3,
-                    gmres_iterations_per_step=float(np.mean(its64)), max_rel_residual=float(max(res64)))
-        print(f"# fp64: {fp64['ms_per_step']:.2f} ms/step, GMRES it/step {np.mean(its64):.1f}", file=sys.stderr, flush=True)
+    fp64, field_err, precond = None, None, {}
+    dirichlet = mesh.array("edge_mode") != 0 if cfg.wall_dirichlet else mesh.array("edge_mode") == 2
+    if precision != 0 and not (args.no_compare or args.no_fp64_reference):
+        m64 = measure(0, True)
+        fp64 = summary(m64)
+        print(f"# fp64: {fp64['ms_per_step']:.2f} ms/step, GMRES it/step {fp64['gmres_iterations_per_step']:.1f}", file=sys.stderr, flush=True)
+        # per-field difference of the first step's result against the all-FP64 step (both solves to 1e-8)
+        field_err = {}
+        for name, a, b in zip(("U_H", "U_Na", "Phi"), main_m["state"], m64["state"]):
+            field_err[name] = float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+    if precision != 0:
+        precond[PRECONDITIONER[precision]] = summary(main_m)
+        if not args.no_compare:
+            other = 2 if precision == 1 else 1
+            mo = measure(other, True)
+            precond[PRECONDITIONER[other]] = summary(mo)
+            print(f"# {PRECONDITIONER[other]}: {precond[PRECONDITIONER[other]]['ms_per_step']:.2f} ms/step, GMRES it/step "
+                  f"{np.mean(mo['its']):.1f}", file=sys.stderr, flush=True)
+        if "multigrid" in precond:
+            precond["multigrid"]["unknowns_per_level"] = dev.mg_info(dirichlet)
+            precond["multigrid"]["cycle"] = dict(levels=int(cfg.mg_levels), agglomeration_passes=int(cfg.mg_agglom_passes),
+                                                 pre=int(cfg.mg_pre), post=int(cfg.mg_post), coarse=int(cfg.mg_coarse),
+                                                 relax=float(cfg.mg_relax), fine_smoother="full operator" if cfg.mg_fine_hall
+                                                 else "resistive part")
 
     # roofline of the dominant kernel (largest total time in the warmup breakdown), timed in the timed region
     roof = None
     kernels = {}
     breakdown, dom, timed = prof
+    stream = stream_calibration() if args.stream else None
     if breakdown and dom in (timed or {}):
         total_ms = sum(v[1] for v in breakdown.values())
         for k, (n, ms) in sorted(breakdown.items(), key=lambda kv: -kv[1][1]):
@@ -238,17 +316,21 @@ def main():
         byt = kernel_bytes(mesh, dom, ns=ns)
         avg_s = ms / n / 1e3
         achieved = byt / avg_s
-        traffic = PMC_TRAFFIC.get((args.workload, dom)) if not args.net else None
+        traffic = PMC_TRAFFIC.get((args.workload, args.order, dom)) if not args.net else None
         roof = dict(kernel=dom, bound="hbm", achieved=round(achieved / 1e9, 1), peak=PEAK_HBM / 1e9, unit="GB/s",
                     frac=round(achieved / PEAK_HBM, 4), traffic=traffic, bytes_per_launch=int(byt),
                     avg_launch_us=round(avg_s * 1e6, 2), launches=n)
+        if stream:
+            roof["stream_copy_GBps"] = stream["copy_GBps"]
+            roof["frac_of_stream_copy"] = round(achieved / 1e9 / stream["copy_GBps"], 4)
 
     # the run's own consecutive steps after the warmup (no restore): dt collapses as the polar field grows
-    elt, itst, rest, _, dtt = measure(precision, False, restore=False)
-    trajectory = dict(ms_per_step=round(elt / args.steps * 1e3, 3), gmres_iterations_per_step=float(np.mean(itst)),
-                      max_rel_residual=float(max(rest)), dt_first=dtt[0], dt_last=dtt[-1])
-    print(f"# trajectory: {trajectory['ms_per_step']:.2f} ms/step, GMRES it/step {np.mean(itst):.1f}, dt {dtt[0]:.2e} -> {dtt[-1]:.2e}",
-          file=sys.stderr, flush=True)
+    mt = measure(precision, False, restore=False)
+    dtt = mt["dts"]
+    trajectory = dict(ms_per_step=round(mt["el"] / args.steps * 1e3, 3), gmres_iterations_per_step=float(np.mean(mt["its"])),
+                      max_rel_residual=float(max(mt["res"])), dt_first=dtt[0], dt_last=dtt[-1])
+    print(f"# trajectory: {trajectory['ms_per_step']:.2f} ms/step, GMRES it/step {trajectory['gmres_iterations_per_step']:.1f}, "
+          f"dt {dtt[0]:.2e} -> {dtt[-1]:.2e}", file=sys.stderr, flush=True)
     cpu = cpu_baseline(args) if args.cpu else None
     out = {
         "metric": "cell-steps/s, mixed-prec vs FP64 GMRES+MG Hall step at 1e-8 rel. residual",
@@ -264,17 +346,22 @@ def main():
         "dtype": "f64" if precision == 0 else "f64+f32",
         "data": "synthetic (seeded tetrahedral shell, seeded noisy state)",
         "config": {"workload": args.workload, "cells": mesh.n_cells, "faces": mesh.n_faces, "edges": mesh.n_edges,
-                   "net": list(w["spec"]["net"]), "precision": args.precision, "hall": bool(w["spec"]["hall"]),
-                   "ion_species": ns, "gmres_restart": 30,
-                   "rtol": 1e-8, "gmres_iterations_per_step": float(np.mean(its)), "max_rel_residual": float(max(res)),
+                   "net": list(w["spec"]["net"]), "cell_order": args.order, "precision": args.precision,
+                   "preconditioner": PRECONDITIONER[precision], "hall": bool(w["spec"]["hall"]), "ion_species": ns,
+                   "wall_dirichlet": int(cfg.wall_dirichlet), "gmres_restart": int(cfg.ind_restart), "rtol": float(cfg.ind_rtol),
+                   "gmres_iterations_per_step": float(np.mean(its)), "max_rel_residual": float(max(res)),
                    "setup_s": round(setup_s, 1)},
         "protocol": "every timed step is the first step from the seeded state (device-side restore, cold solve)",
+        "ms_per_step_by_class": split_by_class(breakdown, nbd),
         "trajectory": trajectory,
         "fp64_step": fp64,
         "vs_fp64": round(value / fp64["value"], 3) if fp64 else None,
+        "field_error_vs_fp64": field_err,
+        "preconditioners": precond or None,
         "roofline": roof,
+        "stream": stream,
         "kernels": kernels,  # per-kernel totals over the warmup steps
-        "kernels_over_steps": args.warmup - 1 if args.warmup > 1 else args.warmup,
+        "kernels_over_steps": nbd,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

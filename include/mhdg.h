@@ -251,6 +251,10 @@ int mhdg_step_hall(mhdg_ctx *ctx, double dt_max, int precision, double *dt_used,
 /* Derived cell fields of the last prepared state (Solver.h:97-101): which = 0 W_H, 1 W_Na (5C),
  * 2 B_ind, 3 B_tot, 4 J, 5 u_plus (3C), 6 E of the last step (E). */
 int mhdg_get_derived(mhdg_ctx *ctx, int which, double *out);
+/* Streaming calibration (no reference counterpart): kind 0 reads, 1 copies, 2 forms a = b + s c over arrays of `bytes`
+ * bytes each with one float4 per thread over the full grid; the median of `repeats` timed launches (HIP events, after
+ * one warm-up) in microseconds and the bytes one launch moves.  Allocates and frees its own buffers. */
+int mhdg_stream_calibration(int kind, int64_t bytes, int repeats, double *median_us, double *bytes_moved);
 /* Waits for the context's stream. */
 int mhdg_sync(mhdg_ctx *ctx);
 /* Per-kernel timing with HIP events recorded on the context's stream around every kernel the step and

@@ -76,6 +76,7 @@ def lib():
         L.mhdg_mesh_shell.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D, D, C.c_uint64, C.POINTER(P)]
         L.mhdg_mesh_from_cells.argtypes = [I32, P, I64, P, P, P, P, I64, P, P, P, C.POINTER(P)]
         L.mhdg_mesh_free.argtypes = [P]
+        L.mhdg_stream_calibration.argtypes = [C.c_int, I64, C.c_int, C.POINTER(D), C.POINTER(D)]
         L.mhdg_mesh_hilbert_order.argtypes = [P, P]
         L.mhdg_mesh_renumber_cells.argtypes = [P, P, C.POINTER(P)]
         L.mhdg_mesh_counts.argtypes = [P, P]

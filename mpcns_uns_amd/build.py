@@ -4,6 +4,7 @@ Host setup code is compiled with g++ (OpenMP), the HIP kernels and the device co
 gfx950; both without floating-point contraction, which the bit-exact parity with the reference's CPU
 arithmetic relies on.
 """
+import glob
 import os
 import subprocess
 import sys
@@ -16,8 +17,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("MHDG_ARCH", "gfx950")
 
-HOST_SRC = ["host/complex.cpp", "host/geometry.cpp", "host/shell.cpp", "host/mesh_capi.cpp", "host/tables.cpp", "host/unsbin.cpp", "host/vtu.cpp", "host/multigrid.cpp", "host/sfc.cpp"]
-HIP_SRC = ["kernels/explicit.hip", "kernels/induction.hip", "kernels/mixed.hip", "kernels/tangent.hip", "kernels/adjoint.hip", "kernels/multigrid.hip", "kernels/ctx.hip"]
+HOST_SRC = sorted(os.path.relpath(p, CSRC) for p in glob.glob(os.path.join(CSRC, "host", "*.cpp")))
+HIP_SRC = sorted(os.path.relpath(p, CSRC) for p in glob.glob(os.path.join(CSRC, "kernels", "*.hip")))
 FP = ["-ffp-contract=off", "-fno-fast-math"]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", *FP,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]

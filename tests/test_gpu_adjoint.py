@@ -3,7 +3,9 @@
 Objective as in the tangent tests: J = sum |B|^2 V over the shell 1.2 <= r <= 1.5 after 5 explicit steps.
 The adjoint is pinned by the tangent (forward mode, the Dual instantiation of the step's own kernels) through
 the dot-product identity <grad J, v> = dJ[v] for random seeds of every input, and by central finite
-differences of the double step for the 3 upstream IMF components and the upstream density.
+differences of the double step for the 3 upstream IMF components and the upstream density.  The objective is also
+wrapped in a torch.autograd.Function whose forward runs the double step and whose backward runs the adjoint
+(mhdg_step_vjp); and the dot-product identity is checked on the 1.95M-cell shell of the differentiation workload.
 """
 import numpy as np
 import pytest
@@ -105,3 +107,63 @@ def test_vjp_rejects_ten_ion_contexts():
     workloads.load(dev, w)
     with pytest.raises(RuntimeError):
         dev.step_vjp(1, R0, R1)
+
+
+def _shell_energy_function(w):
+    """J(p) as a torch.autograd.Function: forward = the double explicit steps on the device, backward = one reverse
+    sweep (mhdg_step_vjp) mapped to the 4 parameters by the chain rule through the inputs they set."""
+    torch = pytest.importorskip("torch")
+
+    class ShellEnergy(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, p):
+            ctx.save_for_backward(p)
+            return torch.tensor(energy(w, p.detach().cpu().double().numpy()), dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, g):
+            (p,) = ctx.saved_tensors
+            pn = p.detach().cpu().double().numpy()
+            _, grads = _device(w, pn).step_vjp(STEPS, R0, R1)
+            return g * torch.from_numpy(_param_gradient(w, pn, grads))
+
+    return ShellEnergy
+
+
+def test_torch_autograd_function_backward_is_the_adjoint(problem):
+    torch = pytest.importorskip("torch")
+
+    w = problem
+    p = torch.tensor(p0(w), dtype=torch.float64, requires_grad=True)
+    J = _shell_energy_function(w).apply(p)
+    (2.0 * J).backward()
+    grad = p.grad.numpy() / 2.0
+    h = 1e-6
+    pn = p0(w)
+    for k in range(4):
+        e = np.zeros(4)
+        e[k] = h
+        fd = (energy(w, pn + e) - energy(w, pn - e)) / (2 * h)
+        _, dJ = tangent(w, pn, k)
+        assert abs(grad[k] - dJ) <= 1e-9 * abs(dJ), (k, grad[k], dJ)
+        assert abs(grad[k] - fd) <= 1e-5 * abs(grad[k]) + 1e-9 * float(J), (k, grad[k], fd)
+
+
+def test_dot_product_at_differentiation_size():
+    """Config-size check: <grad J, v> = dJ[v] for a random seed of the state and the background on the 1.95M-cell
+    shell (5 explicit steps), one reverse sweep against one tangent sweep."""
+    from mpcns_uns_amd import Device, workloads
+
+    w = workloads.build("shell_2m_hall", hall=False)
+    m = w["mesh"]
+    print(f"\n[adjoint] shell_2m: {m.n_cells} cells", flush=True)
+    dev = Device(m, w["cfg"])
+    workloads.load(dev, w)
+    J, g = dev.step_vjp(STEPS, R0, R1)
+    rng = np.random.default_rng(91)
+    vU, vPhi, vPA = rng.normal(size=5 * m.n_cells) * 1e-3, rng.normal(size=m.n_faces) * 1e-3, rng.normal(size=m.n_faces) * 1e-3
+    Jt, dJ = dev.explicit_tangent(STEPS, R0, R1, dU_H=vU, dPhi=vPhi, dPhiAdd=vPA)
+    parts = [float(g["U_H"] @ vU), float(g["Phi"] @ vPhi), float(g["PhiAdd"] @ vPA)]
+    print(f"[adjoint] shell_2m: J {J:.6e}, <grad J, v> {sum(parts):.12e}, dJ[v] {dJ:.12e}", flush=True)
+    assert J == Jt and J > 0
+    assert abs(sum(parts) - dJ) <= 1e-9 * sum(abs(x) for x in parts)

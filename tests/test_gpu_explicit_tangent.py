@@ -3,8 +3,8 @@
 Objective: the magnetic energy J = sum |B|^2 V over the shell 1.2 <= r <= 1.5 after 5 explicit steps, as a function of
 the upstream IMF (3 components: background flux and field, imposed inflow EMF) and the upstream H+ density (inflow
 states and the initial density at fixed velocity and pressure).  Checked against central finite differences with step
-1e-6 (each evaluation a fresh device run of the double kernels), for linearity in the seed, for the value, and through
-a torch.autograd.Function whose backward multiplies the output cotangent by the tangent-mode gradient.
+1e-6 (each evaluation a fresh device run of the double kernels), for linearity in the seed and for the value.  (The
+torch.autograd.Function over the step, whose backward is the adjoint, is in tests/test_gpu_adjoint.py.)
 """
 import numpy as np
 import pytest
@@ -113,27 +113,3 @@ def test_tangent_is_linear_in_the_seed(problem):
     _, d2 = dev.explicit_tangent(STEPS, R0, R1, dPhi=b)
     _, d12 = dev.explicit_tangent(STEPS, R0, R1, dU_H=2 * a, dPhi=-3 * b)
     assert abs(d12 - (2 * d1 - 3 * d2)) <= 1e-10 * (abs(2 * d1) + abs(3 * d2))
-
-
-def test_torch_autograd_function(problem):
-    torch = pytest.importorskip("torch")
-
-    w = problem
-
-    class ShellEnergy(torch.autograd.Function):
-        @staticmethod
-        def forward(ctx, p):
-            pn = p.detach().cpu().double().numpy()
-            ctx.grad = np.array([tangent(w, pn, k)[1] for k in range(4)])
-            return torch.tensor(energy(w, pn), dtype=torch.float64)
-
-        @staticmethod
-        def backward(ctx, g):
-            return g * torch.from_numpy(ctx.grad)
-
-    p = torch.tensor(p0(w), dtype=torch.float64, requires_grad=True)
-    J = ShellEnergy.apply(p)
-    (2.0 * J).backward()
-    h = 1e-6
-    fd = np.array([(energy(w, p0(w) + h * np.eye(4)[k]) - energy(w, p0(w) - h * np.eye(4)[k])) / (2 * h) for k in range(4)])
-    assert np.allclose(p.grad.numpy(), 2.0 * fd, rtol=1e-5, atol=1e-9 * float(J.detach()))

@@ -120,14 +120,44 @@ def run_steps(dev, steps, precision, dt_max, restore=True):
 
 
 def cpu_baseline(args):
-    cmd = [sys.executable, "-u", os.path.join(HERE, "oracle", "cpu_baseline.py"), "--net", *map(str, args.cpu_net),
+    """The CPU leg, in a subprocess: by default the compiled reference under MPI on the host cores, on the benchmark's
+    own workload (oracle/cpu_baseline.py)."""
+    cmd = [sys.executable, "-u", os.path.join(HERE, "oracle", "cpu_baseline.py"), "--workload", args.workload,
            "--steps", str(args.cpu_steps)]
+    if args.cpu_net:
+        cmd += ["--net", *map(str, args.cpu_net)]
+    elif args.net:
+        cmd += ["--net", *map(str, args.net)]
     try:
-        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
         line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
         return json.loads(line[-1]) if line else {"error": (r.stderr or r.stdout)[-400:]}
     except Exception as e:  # the baseline is reported, never required
         return {"error": str(e)}
+
+
+def same_step_as_reference(w, precision, steps=2):
+    """The device step the CPU reference times: the first step from the seeded state under the reference Mercury
+    solver's Dirichlet rule (inflow edges only, Solver.cpp:262), so the two solves run the same iteration."""
+    from mpcns_uns_amd import Device, workloads
+
+    cfg = type(w["cfg"]).from_buffer_copy(w["cfg"])
+    cfg.wall_dirichlet = 0
+    dev = Device(w["mesh"], cfg)
+    try:
+        workloads.load(dev, w)
+        dev.state_save()
+        run_steps(dev, 1, precision, 1e9)
+        dev.sync()
+        t = time.perf_counter()
+        its, res, _ = run_steps(dev, steps, precision, 1e9)
+        dev.sync()
+        el = (time.perf_counter() - t) / steps
+    finally:
+        dev.close()
+    return dict(ms_per_step=round(el * 1e3, 3), value=round(w["mesh"].n_cells / el, 1), gmres_iterations_per_step=float(np.mean(its)),
+                max_rel_residual=float(max(res)), precision={0: "fp64", 1: "mixed", 2: "mg"}[precision],
+                dirichlet="inflow edges (Mercury)")
 
 
 def differentiation(args):
@@ -180,8 +210,9 @@ def main():
     ap.add_argument("--no-fp64-reference", action="store_true", help="skip the plain FP64 step measurement")
     ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
-    ap.add_argument("--cpu-net", type=int, nargs=3, default=(96, 192, 10))
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-net", type=int, nargs=3, default=None,
+                    help="a smaller sample net for the CPU leg (default: the workload's own net)")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="first steps the CPU leg times")
     ap.add_argument("--no-stream", dest="stream", action="store_false", default=True,
                     help="skip the streaming calibration (read, copy, triad kernels)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -331,7 +362,14 @@ def main():
                       max_rel_residual=float(max(mt["res"])), dt_first=dtt[0], dt_last=dtt[-1])
     print(f"# trajectory: {trajectory['ms_per_step']:.2f} ms/step, GMRES it/step {trajectory['gmres_iterations_per_step']:.1f}, "
           f"dt {dtt[0]:.2e} -> {dtt[-1]:.2e}", file=sys.stderr, flush=True)
-    cpu = cpu_baseline(args) if args.cpu else None
+    cpu = None
+    if args.cpu:
+        # the device on the step the CPU reference runs, then the CPU reference itself
+        same = same_step_as_reference(w, precision) if w["spec"]["hall"] else None
+        cpu = cpu_baseline(args)
+        if same and "value" in cpu and not args.cpu_net:
+            cpu["gpu_same_step"] = same
+            cpu["gpu_over_cpu_same_step"] = round(same["value"] / cpu["value"], 1)
     out = {
         "metric": "cell-steps/s, mixed-prec vs FP64 GMRES+MG Hall step at 1e-8 rel. residual",
         "value": round(value, 1),

@@ -40,9 +40,12 @@
 #include <sstream>
 
 #include "../include/mhdg.h"
+#include "ref_config.h"
 
 using namespace uns;
 using namespace mhd;
+using refh::build_mesh;
+using refh::make_config;
 
 struct ref_ctx
 {
@@ -62,42 +65,6 @@ struct ref_ctx
 
 namespace
 {
-Config make_config(const mhdg_config &c, int dipole, const double *m, const double *cc, double soft)
-{
-    Config k;
-    k.sc = Scales::simple(c.gamma, c.inver_MA2, c.M_Na / c.M_H);
-    k.sc.M_ref = c.M_ref; k.sc.M_H = c.M_H; k.sc.M_Na = c.M_Na;
-    k.sc.ne_floor = c.ne_floor;
-    k.sc.hall_coef = c.hall_coef;
-    k.sc.induce_coeff = c.induce_coeff;
-    k.sc.inver_Rem = c.inver_Rem;
-    k.cfl = c.cfl;
-    k.second_order = c.second_order != 0;
-    k.recon_rel_limit = c.recon_rel_limit;
-    k.rho_floor = c.rho_floor; k.p_floor = c.p_floor; k.rho_floor_dt = c.rho_floor_dt;
-    k.lorentz_force = c.lorentz_force != 0;
-    k.time_order = 1;
-    k.hall = c.hall != 0;
-    k.resistive = c.resistive != 0;
-    k.eta_r_inner = c.eta_r_inner; k.eta_r_outer = c.eta_r_outer; k.eta_width = c.eta_width;
-    k.hall_r_min = c.hall_r_min; k.hall_r_max = c.hall_r_max;
-    k.ind_rtol = c.ind_rtol; k.ind_max_it = c.ind_max_it; k.ind_restart = c.ind_restart;
-    k.ind_ilu = false;
-    k.ideal.cuct = c.cuct;
-    k.implicit_ideal = c.implicit_ideal != 0;
-    k.fsH = {c.fs[0][0], c.fs[0][4]};
-    k.fsNa = {c.fs[1][0], c.fs[1][4]};
-    k.u_inf = {c.u_inf[0], c.u_inf[1], c.u_inf[2]};
-    k.na_seed_at_rest = !(c.fs[1][1] == c.u_inf[0] && c.fs[1][2] == c.u_inf[1] && c.fs[1][3] == c.u_inf[2]);
-    k.B_imf = {c.B_imf[0], c.B_imf[1], c.B_imf[2]};
-    k.dipole = dipole != 0;
-    if (m) k.dipole_m = {m[0], m[1], m[2]};
-    if (cc) k.dipole_c = {cc[0], cc[1], cc[2]};
-    k.dipole_soft = soft;
-    k.write_vtk = false; k.write_fields = false; k.write_profile = false; k.log_every = 0;
-    k.bc_default = "outflow";
-    return k;
-}
 Prim prim5(const double *p) { return Prim{p[0], p[1], p[2], p[3], p[4]}; }
 template <class T> void put(const std::vector<T> &v, T *out) { std::memcpy(out, v.data(), v.size() * sizeof(T)); }
 void put3(const std::vector<Vec3> &v, double *out)
@@ -120,23 +87,8 @@ int ref_create(int32_t n_nodes, const double *xyz, int64_t n_cells, const int32_
         MPI_Initialized(&init);
         if (!init) MPI_Init(nullptr, nullptr);
         auto *r = new ref_ctx();
-        ComplexBuilder b(n_nodes);
-        std::vector<std::vector<Id>> loops;
-        for (int64_t c = 0; c < n_cells; ++c)
-        {
-            loops.clear();
-            for (int32_t l = cell_ptr[c]; l < cell_ptr[c + 1]; ++l) loops.emplace_back(loop_nodes + loop_ptr[l], loop_nodes + loop_ptr[l + 1]);
-            b.add_cell(loops, material ? material[c] : kFluid);
-        }
-        for (int64_t t = 0; t < n_tags; ++t) b.tag_face(std::vector<Id>(tag_nodes + tag_ptr[t], tag_nodes + tag_ptr[t + 1]), tag_id[t]);
-        r->lm.mesh.cx = b.finish();
-        r->lm.mesh.x.resize(n_nodes);
-        for (int32_t i = 0; i < n_nodes; ++i) r->lm.mesh.x[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
-        if (tagset == 0) r->lm.mesh.tag_names = {"", "inner", "outer"};
-        else r->lm.mesh.tag_names = {"", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax"};
+        r->lm.mesh = build_mesh(n_nodes, xyz, n_cells, cell_ptr, loop_ptr, loop_nodes, material, n_tags, tag_ptr, tag_nodes, tag_id, tagset);
         r->cfg = make_config(*cfg, dipole, dm, dc, soft);
-        r->cfg.bc_by_tag["inner"] = "wall";
-        r->cfg.bc_by_tag["outer"] = "farfield";
         r->geo = compute_geometry(r->lm.mesh);
         r->ex = std::make_unique<comm::Exchanger>(MPI_COMM_SELF, r->lm);
         r->bc = std::make_unique<BoundaryInfo>(BoundaryInfo::build(r->lm, r->geo, r->cfg));

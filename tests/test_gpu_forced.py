@@ -26,13 +26,19 @@ def test_floors_forced_bitwise(small_explicit):
     w = small_explicit
     state, rho_cells, p_cells = floored_state(w)
     dev, orc = _pair(w, state)
-    for _ in range(2):
+    cfg = w["cfg"]
+    for k in range(2):
         assert dev.step() == orc.step()
         for a, b in zip(dev.get_state(), orc.get_state()):
             assert np.array_equal(a, b)
-    cfg = w["cfg"]
-    Na = orc.get_state()[1].reshape(-1, 5)
-    assert np.any(np.all(Na[rho_cells] == [cfg.rho_floor, 0, 0, 0, cfg.p_floor / (cfg.gamma - 1.0)], axis=1))
+        if k == 0:
+            # the first step took the density branch: stagnant floor state in the seeded cells
+            Na = orc.get_state()[1].reshape(-1, 5)
+            assert np.all(Na[rho_cells] == [cfg.rho_floor, 0, 0, 0, cfg.p_floor / (cfg.gamma - 1.0)])
+            # and the pressure branch: H+ re-clamped to the pressure floor
+            H = orc.get_state()[0].reshape(-1, 5)
+            ke = 0.5 * (H[p_cells, 1] ** 2 + H[p_cells, 2] ** 2 + H[p_cells, 3] ** 2) / H[p_cells, 0]
+            assert np.allclose((cfg.gamma - 1.0) * (H[p_cells, 4] - ke), cfg.p_floor, rtol=1e-6, atol=1e-15)
 
 
 @pytest.mark.parametrize("species", [0, 1])

@@ -84,10 +84,12 @@ def test_mg_solve_matches_fp64_and_oracle(small_hall):
     ref = Device(m, w["cfg"])
     ref.set_background(*w["background"])
     ref.set_state(*w["state"])
-    for _ in range(3):
+    for k in range(3):
         dt2, it2, r2 = dev.step(precision=2)
         dt0, it0, r0 = ref.step(precision=0)
-        assert r2 <= 1e-8 and r0 <= 1e-8 and dt2 == dt0
+        assert r2 <= 1e-8 and r0 <= 1e-8
+        # the first step's dt comes from the same state; later ones from states that differ within the solver tolerance
+        assert dt2 == dt0 if k == 0 else abs(dt2 - dt0) <= 1e-8 * dt0
     for a, b in zip(dev.get_state(), ref.get_state()):
         assert maxrel(a, b) < 1e-8
     _, rel, _ = dev.div_b(dev.get_state()[2], w["background"][0])

@@ -228,6 +228,24 @@ int mhdg_get_species_state(mhdg_ctx *ctx, int species, double *U);
 int mhdg_get_species_prim(mhdg_ctx *ctx, int species, double *W);
 int mhdg_get_state(mhdg_ctx *ctx, double *U_H, double *U_Na, double *Phi);
 int mhdg_set_background(mhdg_ctx *ctx, const double *PhiAdd, const double *BaddCell);
+/* Reference scales and particle masses of MarsSolver's point-implicit sources (MarsParams,
+ * MarsUns/driver/MarsSolver.h:42-59; MarsSolver.cpp:40-62): SI scales, rho_ref = Mref n / NA, p_ref = rho_ref U^2,
+ * time_ref = L / U; ion_mass[s] the particle mass of ion species s [kg] (MARS::ParticleMass); n_neutral reactive
+ * neutrals (<= 10, MARS::NumReactiveNeutral) with particle masses neutral_mass[n] [kg], read per cell from a
+ * density array [cm^-3] of neutral_stride values per cell (MARS::NumNeutral = 12 in the reference). */
+typedef struct mhdg_mars_params {
+    double kB, q, L, U, B, n;
+    double rho_ref, p_ref, time_ref;
+    double rho_floor, p_floor;
+    double ion_mass[10];
+    int32_t n_neutral, neutral_stride;
+    double neutral_mass[10];
+} mhdg_mars_params;
+/* MARS::MarsSolver::implicit_sources_ (MarsUns/driver/MarsSolver.cpp:422-527) on the resident state of every ion
+ * species, with B_tot of the last prepare (mhdg_prepare or the step's own): Crank-Nicolson gyration of the coupled
+ * ions about B, backward-Euler ion-neutral drag, and backward-Euler thermal relaxation with frictional heating, per
+ * fluid cell.  neutral_cm3: neutral densities, neutral_stride per cell (host array). */
+int mhdg_mars_implicit_sources(mhdg_ctx *ctx, const mhdg_mars_params *prm, const double *neutral_cm3, double dt);
 /* Solver::prepare (MercuryUns/driver/Solver.cpp:94-125) on the resident state: primitives, B_ind, B_tot, u+ and
  * J of the current state for mhdg_get_derived (what Solver::write_vtk outputs, Solver.cpp:677-696). */
 int mhdg_prepare(mhdg_ctx *ctx);

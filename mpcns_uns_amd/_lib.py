@@ -56,6 +56,18 @@ class Config(C.Structure):
     ]
 
 
+class MarsParams(C.Structure):
+    """mhdg_mars_params (include/mhdg.h): scales and particle masses of the Mars point-implicit sources."""
+    _fields_ = [
+        ("kB", C.c_double), ("q", C.c_double), ("L", C.c_double), ("U", C.c_double), ("B", C.c_double), ("n", C.c_double),
+        ("rho_ref", C.c_double), ("p_ref", C.c_double), ("time_ref", C.c_double),
+        ("rho_floor", C.c_double), ("p_floor", C.c_double),
+        ("ion_mass", C.c_double * 10),
+        ("n_neutral", C.c_int32), ("neutral_stride", C.c_int32),
+        ("neutral_mass", C.c_double * 10),
+    ]
+
+
 _lib = None
 
 
@@ -112,7 +124,7 @@ def lib():
             "mhdg_flux_rhs": [P, C.c_int, P, P, P, P, P],
             "mhdg_ideal_emf": [P, P, P, P, P, P, D, P],
             "mhdg_faraday": [P, P, P, D], "mhdg_faraday_edge_dt": [P, P, P, P],
-            "mhdg_div_b": [P, P, P, P, P, P], "mhdg_local_dt": [P, P, P, P, P],
+            "mhdg_div_b": [P, P, P, P, P, P], "mhdg_mars_implicit_sources": [P, P, P, D], "mhdg_local_dt": [P, P, P, P, P],
             "mhdg_add_sources": [P, C.c_int, P, P, P, P, D, P],
             "mhdg_ind_set_frozen": [P, P, P], "mhdg_ind_apply_F": [P, P, P],
             "mhdg_ind_solve": [P, P, P, P, D, P, C.c_int, P, P, P],

@@ -276,6 +276,12 @@ class Device:
         """Solver::prepare on the resident state: derived() then gives the current state's fields."""
         self._ok(lib().mhdg_prepare(self._h))
 
+    def mars_implicit_sources(self, prm, neutral, dt):
+        """MarsSolver::implicit_sources_ on the resident state with B_tot of the last prepare (prm: MarsParams;
+        neutral: densities [cm^-3], prm.neutral_stride per cell)."""
+        nd = None if prm.n_neutral == 0 else self._sized(np.ascontiguousarray(neutral, dtype=np.float64), self.nc * prm.neutral_stride)
+        self._ok(lib().mhdg_mars_implicit_sources(self._h, C.byref(prm), ptr(nd), dt))
+
     def derived(self, name):
         n = {0: 5, 1: 5, 6: None}.get(DERIVED[name], 3)
         out = np.zeros(self.ne if n is None else n * self.nc)

@@ -91,6 +91,41 @@ def mars_config(hall=True, n_dipoles=100, seed=SEED0 + 5, **kw):
     return physics_config(hall, **base)
 
 
+# MARS::ParticleMass (MarsUns/physics/MarsSpecies.h:34-36) [kg]: Ar, CO2, CO, C, H, N2, N, NO, O, O2; the ions and the
+# reactive neutrals of MarsSolver share the table
+MARS_PARTICLE_MASS = (6.645e-26, 7.309e-26, 4.651e-26, 1.993e-26, 1.673e-27, 4.651e-26, 2.326e-26, 4.983e-26, 2.658e-26, 5.316e-26)
+
+
+def mars_params(n_neutral=10, neutral_stride=12, **kw):
+    """MarsParams of the point-implicit sources with the reference's defaults (MarsSolver.cpp:40-62: NA, R_uni, q_e,
+    U_ref, n_ref, B_ref, L_ref, Molecular_mass 0.001, floors 1e-10); keyword overrides."""
+    from ._lib import MarsParams
+
+    NA, R = 6.02214076e23, 8.314462618
+    p = MarsParams()
+    p.kB, p.q = R / NA, 1.602176634e-19
+    p.U, p.n, p.B, p.L = 3.27e4, 4.0e6, 3.0e-9, 3.396e6
+    p.rho_ref = 0.001 / NA * p.n
+    p.p_ref = p.rho_ref * p.U * p.U
+    p.time_ref = p.L / p.U
+    p.rho_floor = p.p_floor = 1e-10
+    for s in range(10):
+        p.ion_mass[s] = p.neutral_mass[s] = MARS_PARTICLE_MASS[s]
+    p.n_neutral, p.neutral_stride = n_neutral, neutral_stride
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def neutral_atmosphere(mesh, seed=SEED0 + 5, stride=12, scale_height=0.02, r0=1.0):
+    """Seeded exponential neutral atmosphere [cm^-3] per cell (stride values per cell): surface densities
+    10^U(6, 11) per neutral, falling off as exp(-(|x| - r0) / scale_height)."""
+    rng = np.random.default_rng(seed + 2000)
+    n0 = 10.0 ** rng.uniform(6.0, 11.0, size=stride)
+    r = np.linalg.norm(mesh.array("cell_center").reshape(-1, 3), axis=1)
+    return np.ascontiguousarray((np.exp(-(r - r0) / scale_height)[:, None] * n0[None, :]).ravel())
+
+
 def crustal_dipoles(n=100, seed=SEED0 + 5, r=0.98, depth=0.02):
     """Seeded point dipoles at radius r: positions and orientations uniform on the sphere, moments
     lognormal(-3, 1) * depth^3."""

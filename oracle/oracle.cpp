@@ -8,6 +8,7 @@
 // independent entities run under OpenMP; every sum and every reduction with an order stays serial.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -1207,6 +1208,104 @@ int orc_step(orc_ctx *o, double dt_max, double *dt_used, int32_t *iters, double 
         return 0;
     }
     catch (const std::exception &e) { o->err = e.what(); return 1; }
+}
+// MARS::MarsSolver::implicit_sources_ (MarsUns/driver/MarsSolver.cpp:422-527) on the current state of every ion
+// species with B_tot of the last prepare: Crank-Nicolson gyration of the coupled ions (closed form, std::complex as
+// the reference), backward-Euler drag and thermal relaxation, per fluid cell in ascending order.
+int orc_mars_implicit_sources(orc_ctx *o, const mhdg_mars_params *prm, const double *neutral, double dt)
+{
+    using cd = std::complex<double>;
+    const double g = o->cfg.gamma;
+    const double a5 = 3.0e6 * prm->L * prm->kB / (prm->rho_ref * prm->U * prm->U * prm->U);
+    const int NS = o->ns, NN = prm->n_neutral;
+    for (int32_t c = 0; c < o->nc; ++c)
+    {
+        if (o->mat[size_t(c)] != 0) continue;
+        const size_t i = size_t(c);
+        double nrel[10], ntot = 0;
+        for (int s = 0; s < NS; ++s)
+        {
+            nrel[s] = std::max(o->U[s][5 * i], 0.0) * prm->rho_ref / (prm->ion_mass[s] * prm->n);
+            ntot += nrel[s];
+        }
+        const V3 B(o->Btot[3 * i], o->Btot[3 * i + 1], o->Btot[3 * i + 2]);
+        const double Bm = norm(B);
+        if (ntot > 1e-300 && Bm > 1e-30)
+        {
+            const V3 k = (1.0 / Bm) * B;
+            const V3 t = std::abs(k.x) < 0.9 ? V3(1, 0, 0) : V3(0, 1, 0);
+            V3 e1 = cross(k, t);
+            e1 = (1.0 / norm(e1)) * e1;
+            const V3 e2 = cross(k, e1);
+            cd z[10], al[10], Z0 = 0;
+            for (int s = 0; s < NS; ++s)
+            {
+                const double *u = &o->U[s][5 * i];
+                const double rho = std::max(u[0], prm->rho_floor);
+                const V3 v(u[1] / rho, u[2] / rho, u[3] / rho);
+                z[s] = cd(dot(v, e1), dot(v, e2));
+                al[s] = cd(0.0, prm->q * prm->B * prm->L / (prm->ion_mass[s] * prm->U) * Bm * dt * 0.5);
+                Z0 += (nrel[s] / ntot) * z[s];
+            }
+            cd Pn = 0, Qn = 0;
+            for (int s = 0; s < NS; ++s)
+            {
+                const double chi = nrel[s] / ntot;
+                const cd a = -al[s];
+                Pn += chi * (z[s] * (1.0 - a) + a * Z0) / (1.0 + a);
+                Qn += chi * a / (1.0 + a);
+            }
+            const cd Zn = Pn / (1.0 - Qn);
+            for (int s = 0; s < NS; ++s)
+            {
+                const cd a = -al[s];
+                const cd zn = (z[s] * (1.0 - a) + a * (Z0 + Zn)) / (1.0 + a);
+                double *u = &o->U[s][5 * i];
+                const double rho = std::max(u[0], prm->rho_floor);
+                const V3 v(u[1] / rho, u[2] / rho, u[3] / rho);
+                const double vpar = dot(v, k);
+                const V3 vn = vpar * k + zn.real() * e1 + zn.imag() * e2;
+                u[4] += 0.5 * rho * (dot(vn, vn) - dot(v, v));
+                u[1] = rho * vn.x; u[2] = rho * vn.y; u[3] = rho * vn.z;
+            }
+        }
+        double coll_n[10], collision = 0;
+        for (int n = 0; n < NN; ++n)
+        {
+            coll_n[n] = std::max(neutral[i * size_t(prm->neutral_stride) + size_t(n)], 0.0) * 2.0e-10;
+            collision += coll_n[n];
+        }
+        const double nu = prm->time_ref * collision;
+        for (int s = 0; s < NS; ++s)
+        {
+            double *u = &o->U[s][5 * i];
+            const double m = prm->ion_mass[s];
+            const double rho = std::max(u[0], prm->rho_floor);
+            const V3 v(u[1] / rho, u[2] / rho, u[3] / rho);
+            const double ke_old = 0.5 * rho * dot(v, v);
+            const double den = 1.0 + nu * dt;
+            u[1] /= den; u[2] /= den; u[3] /= den;
+            const double ke = ke_old / (den * den);
+            const double p = std::max((g - 1.0) * (u[4] - ke), prm->p_floor);
+            double b1c = 0, b2 = 0;
+            for (int n = 0; n < NN; ++n)
+            {
+                const double mm = m + prm->neutral_mass[n];
+                b1c += m * coll_n[n] / mm;
+                b2 += prm->neutral_mass[n] * coll_n[n] / mm;
+            }
+            const double n_m3 = rho * prm->rho_ref / m;
+            const double ns_cm3 = n_m3 * 1e-6;
+            const double kk = a5 * ns_cm3 * b1c;
+            const double cT = prm->p_ref / (std::max(n_m3, 1.0) * prm->kB);
+            const double Sf = prm->time_ref * rho * dot(v, v) * b2;
+            double pn;
+            if (p * cT >= 50.0) pn = (p + dt * (g - 1.0) * (kk * 185.0 + Sf)) / (1.0 + dt * (g - 1.0) * kk * cT);
+            else pn = p + dt * (g - 1.0) * (kk * (185.0 - 50.0) + Sf);
+            u[4] = pn / (g - 1.0) + ke;
+        }
+    }
+    return 0;
 }
 // Solver::prepare (Solver.cpp:94-125) on the current state
 int orc_prepare(orc_ctx *o)

@@ -71,7 +71,7 @@ class Oracle:
                 "orc_set_background": [_P, _P, _P], "orc_step": [_P, _D, _P, _P, _P],
                 "orc_get_derived": [_P, C.c_int, _P], "orc_prepare": [_P],
                 "orc_set_species_state": [_P, C.c_int, _P], "orc_get_species_state": [_P, C.c_int, _P],
-                "orc_get_species_prim": [_P, C.c_int, _P],
+                "orc_get_species_prim": [_P, C.c_int, _P], "orc_mars_implicit_sources": [_P, _P, _P, _D],
             }.items():
                 getattr(L, name).argtypes = args
             cls._lib = L
@@ -89,6 +89,13 @@ class Oracle:
         if getattr(self, "_h", None):
             self.lib().orc_destroy(self._h)
             self._h = None
+
+    def mars_implicit_sources(self, prm, neutral, dt):
+        """MarsSolver::implicit_sources_ (MarsSolver.cpp:422-527) on the current state with B_tot of the last prepare."""
+        nd = None if prm.n_neutral == 0 else _f(neutral)
+        if nd is not None and nd.size != self.nc * prm.neutral_stride:
+            raise ValueError("neutral densities: neutral_stride values per cell")
+        self.lib().orc_mars_implicit_sources(self._h, C.addressof(prm), _ptr(nd), dt)
 
     def cell_vector(self, flux):
         out = np.zeros(3 * self.nc)

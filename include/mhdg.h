@@ -142,6 +142,14 @@ int mhdg_faraday_edge_dt(mhdg_ctx *ctx, const double *E, const double *dt_edge, 
 /* mhd::div_b (Common/diagnostics/DivB.cpp:8-24): max|d2 B|, max relative, worst cell. */
 int mhdg_div_b(mhdg_ctx *ctx, const double *B, const double *extra, double *max_abs, double *max_rel,
                int64_t *worst_cell);
+/* No counterpart in the reference: the allocation-free form of mhdg_div_b, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  B (n_faces doubles),
+ * extra (n_faces doubles or NULL) and out (3 doubles: max|d2 B|, max relative, worst cell id or -1) are DEVICE
+ * pointers; the work is queued on the context's stream and out is valid after it (mhdg_sync).  The reduction's
+ * partials go to a fixed device array of the library (1.25 MiB per GPU, kept until the process ends); no device
+ * memory is allocated.  At most 16,777,216 cells (larger meshes are refused with an error).  Calls that overlap on
+ * one GPU share that array and race.  Same numbers as mhdg_div_b. */
+int mhdg_div_b_noalloc(mhdg_ctx *ctx, const double *B, const double *extra, double *out);
 /* mercury::cell_time_steps / local_time_step (MercuryUns/fluid/MercurySources.cpp:65-116). */
 int mhdg_local_dt(mhdg_ctx *ctx, const double *W_H, const double *W_Na, const double *B_tot, double *dt);
 /* mercury::add_sources, Lorentz part (MercurySources.cpp:20-63). */

@@ -124,7 +124,7 @@ def lib():
             "mhdg_flux_rhs": [P, C.c_int, P, P, P, P, P],
             "mhdg_ideal_emf": [P, P, P, P, P, P, D, P],
             "mhdg_faraday": [P, P, P, D], "mhdg_faraday_edge_dt": [P, P, P, P],
-            "mhdg_div_b": [P, P, P, P, P, P], "mhdg_mars_implicit_sources": [P, P, P, D], "mhdg_local_dt": [P, P, P, P, P],
+            "mhdg_div_b": [P, P, P, P, P, P], "mhdg_div_b_noalloc": [P, P, P, P], "mhdg_mars_implicit_sources": [P, P, P, D], "mhdg_local_dt": [P, P, P, P, P],
             "mhdg_add_sources": [P, C.c_int, P, P, P, P, D, P],
             "mhdg_ind_set_frozen": [P, P, P], "mhdg_ind_apply_F": [P, P, P],
             "mhdg_ind_solve": [P, P, P, P, D, P, C.c_int, P, P, P],

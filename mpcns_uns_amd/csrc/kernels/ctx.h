@@ -9,9 +9,15 @@
 #include "induction.h"
 #include "mixed.h"
 
+namespace mhdg
+{
+void noalloc_setup(int device); // noalloc.hip: the device address of the fixed scratch array, once per device
+}
+
 struct mhdg_ctx
 {
     hipStream_t stream = nullptr;
+    int device = 0; // the device current at creation (its stream's)
     mhdg_config cfg{};
     mhdg::Params P{};
     mhdg::DevMesh d{};

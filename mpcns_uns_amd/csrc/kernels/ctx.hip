@@ -188,6 +188,8 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
     op_hall = P.hall != 0;
     op_res = P.resistive != 0;
     ideal_cfg = c.implicit_ideal != 0;
+    hip_check(hipGetDevice(&device), "hipGetDevice");
+    noalloc_setup(device);
     hip_check(hipStreamCreate(&stream), "hipStreamCreate");
     Tables t = build_tables(m, c, kLightMax);
     if (t.max_valence > kHeavyMax) throw std::runtime_error("edge valence exceeds the supported maximum");

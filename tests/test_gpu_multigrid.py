@@ -90,8 +90,9 @@ def test_mg_solve_matches_fp64_and_oracle(small_hall):
         assert r2 <= 1e-8 and r0 <= 1e-8
         # the first step's dt comes from the same state; later ones from states that differ within the solver tolerance
         assert dt2 == dt0 if k == 0 else abs(dt2 - dt0) <= 1e-8 * dt0
+    # three steps, each solve stopped at a 1e-8 relative residual: the differences add up over the steps
     for a, b in zip(dev.get_state(), ref.get_state()):
-        assert maxrel(a, b) < 1e-8
+        assert maxrel(a, b) < 3e-8
     _, rel, _ = dev.div_b(dev.get_state()[2], w["background"][0])
     assert rel < 1e-12
 

@@ -172,8 +172,10 @@ int mhdg_ind_ideal_linear(mhdg_ctx *ctx, const double *phi, double *out);
  * Gram-Schmidt (the reference iteration); 1 = mixed precision (FP32 operator and Krylov basis, classical
  * Gram-Schmidt with FP64 accumulation, inside FP64 iterative refinement to the same relative residual;
  * not available with the implicit ideal part); 2 = the mixed solve with flexible GMRES preconditioned by one FP32
- * agglomeration-multigrid V-cycle per step (mhdg_mg_apply; needs e2n in the mesh descriptor, scalar dt only).  The warm-start correction is kept in the context between
- * calls, as the reference keeps corr_. */
+ * agglomeration-multigrid V-cycle per step (mhdg_mg_apply; needs e2n in the mesh descriptor, scalar dt only); 3 = the
+ * FP64 iteration of precision 0 with classical Gram-Schmidt (one multi-dot and one multi-axpy pass per step, as the
+ * mixed solve orthogonalises) in place of modified Gram-Schmidt.  The warm-start correction is kept in the context
+ * between calls, as the reference keeps corr_. */
 int mhdg_ind_solve(mhdg_ctx *ctx, const double *Phi_n, const double *E_expl, const int8_t *dirichlet,
                    double dt, const double *dt_edge, int precision, double *E, int32_t *iterations,
                    double *rel_res);

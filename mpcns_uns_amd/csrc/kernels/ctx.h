@@ -111,6 +111,9 @@ struct mhdg_ctx
     void apply_op(const double *phi, const double *v, const double *inv, const int8_t *dir, double dt, int mode, int mask, const double *b,
                   double *out);
     int solve64(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
+    // the same system and iteration with classical Gram-Schmidt (cgs64.hip, precision 3)
+    int solve64_cgs(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
+    double *cgs_part = nullptr, *cgs_h = nullptr;
     void mixed_build();
     // use_mg: the inner solve is flexible GMRES preconditioned by one V-cycle of the multigrid hierarchy built for dir_host
     // (the host copy of dir); otherwise GMRES with the Jacobi preconditioner

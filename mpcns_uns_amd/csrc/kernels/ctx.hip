@@ -428,9 +428,10 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
 double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
 {
     // refusals before any launch, so that a throw leaves the resident state as it was
-    if (precision < 0 || precision > 2) throw std::runtime_error("step: precision must be 0 (FP64), 1 (mixed) or 2 (mixed with multigrid)");
+    if (precision < 0 || precision > 3)
+        throw std::runtime_error("step: precision must be 0 (FP64), 1 (mixed), 2 (mixed with multigrid) or 3 (FP64 with classical Gram-Schmidt)");
     if (precision == 2 && mgh.n_nodes <= 0) throw std::runtime_error("multigrid preconditioner: the mesh descriptor carried no e2n incidence");
-    if (precision != 0 && ideal_cfg && u_ideal)
+    if ((precision == 1 || precision == 2) && ideal_cfg && u_ideal)
         throw std::runtime_error("mixed-precision solve: the implicit ideal part has no FP32 operator; use precision 0");
     const int nb = nblk(nc);
     SpC Uc{}, Wc{};
@@ -511,6 +512,7 @@ double mhdg_ctx::step(double dt_max, int precision, int *iters, double *res)
     {
         hip_check(hipMemcpyAsync(B0, Btot, 3 * size_t(nc) * 8, hipMemcpyDeviceToDevice, stream), "copy");
         if (precision == 0) *iters = solve64(Phi, E_expl, dir_mercury, gdt, E_last, res);
+        else if (precision == 3) *iters = solve64_cgs(Phi, E_expl, dir_mercury, gdt, E_last, res);
         else *iters = solve_mixed(Phi, E_expl, dir_mercury, gdt, E_last, res, precision == 2, dir_mercury_h.data());
         Euse = E_last;
     }
@@ -849,8 +851,14 @@ int mhdg_ind_solve(mhdg_ctx *c, const double *Phi_n, const double *E_expl, const
         c->dte_cur = dt_edge ? c->stage_in(4, dt_edge, c->ne) : nullptr;
         double rr = 0;
         int it = 0;
-        if (precision < 0 || precision > 2) throw std::runtime_error("solve: precision must be 0 (FP64), 1 (mixed) or 2 (mixed with multigrid)");
-        try { it = precision == 0 ? c->solve64(p, ee, dd, dt, eo, &rr) : c->solve_mixed(p, ee, dd, dt, eo, &rr, precision == 2, dir); }
+        if (precision < 0 || precision > 3)
+            throw std::runtime_error("solve: precision must be 0 (FP64), 1 (mixed), 2 (mixed with multigrid) or 3 (FP64 with classical Gram-Schmidt)");
+        try
+        {
+            it = precision == 0   ? c->solve64(p, ee, dd, dt, eo, &rr)
+                 : precision == 3 ? c->solve64_cgs(p, ee, dd, dt, eo, &rr)
+                                  : c->solve_mixed(p, ee, dd, dt, eo, &rr, precision == 2, dir);
+        }
         catch (...) { c->dte_cur = nullptr; throw; }
         c->dte_cur = nullptr;
         c->d2h(E, eo, size_t(c->ne) * 8);

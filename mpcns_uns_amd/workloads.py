@@ -2,7 +2,8 @@
 
 shell_60k      FP64 ideal two-ion MHD, Hall off, one explicit stage; 24x48 net, 9 layers (59,616 tets)
 shell_2m_hall  Hall + uniform resistivity, GMRES(30) to 1e-8, all FP64; 128x256 net, 10 layers (1.95M)
-shell_8m_mixed Hall + resistivity, mixed-precision GMRES + multigrid; 256x512 net, 10 layers (7.83M)
+shell_8m_mixed Hall + resistivity, mixed-precision GMRES (Jacobi; the agglomeration-multigrid V-cycle with
+               precision 2); 256x512 net, 10 layers (7.83M)
 shell_4m_mars  ten-ion Mars-like fluid, crustal dipoles, exponential-atmosphere ion production, Hall on;
                shell r in [1, 3], 192x384 net, 9 layers (3.98M)
 

@@ -108,4 +108,4 @@ def test_mg_refuses_what_it_cannot_serve(small_hall):
     with pytest.raises(RuntimeError, match="per-edge"):
         dev.ind_solve(w["state"][2], np.zeros(m.n_edges), d, 2e-3, precision=2, dt_edge=np.full(m.n_edges, 2e-3))
     with pytest.raises(RuntimeError, match="precision"):
-        dev.ind_solve(w["state"][2], np.zeros(m.n_edges), d, 2e-3, precision=3)
+        dev.ind_solve(w["state"][2], np.zeros(m.n_edges), d, 2e-3, precision=4)

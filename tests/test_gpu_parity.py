@@ -184,6 +184,27 @@ def test_step_mixed_vs_fp64(small_hall):
     assert rel < 1e-12
 
 
+def test_step_fp64_classical_gram_schmidt(small_hall):
+    """precision 3 (the FP64 iteration with classical Gram-Schmidt) against the oracle's reference iteration (modified
+    Gram-Schmidt): the same Krylov spaces, so the same iteration counts and states to rounding."""
+    from mpcns_uns_amd import Device
+    from oracle.oracle import Oracle
+
+    w = small_hall
+    dev, orc = Device(w["mesh"], w["cfg"]), Oracle(w["mesh"], w["cfg"])
+    for x in (dev, orc):
+        x.set_background(*w["background"])
+        x.set_state(*w["state"])
+    for _ in range(3):
+        dt_d, it_d, r_d = dev.step(precision=3)
+        dt_o, it_o, r_o = orc.step()
+        assert abs(dt_d - dt_o) <= 1e-12 * dt_o and abs(it_d - it_o) <= 1 and r_d <= 1e-8
+        for a, b in zip(dev.get_state(), orc.get_state()):
+            assert maxrel(a, b) < 1e-10
+    _, rel, _ = dev.div_b(dev.get_state()[2], w["background"][0])
+    assert rel < 1e-12
+
+
 def test_implicit_ideal_and_edge_dt():
     """set_ideal_state / ideal_linear (bitwise: the same face and edge EMF kernels as the explicit stage) and the
     per-edge pseudo-time solve, FP64 and mixed."""

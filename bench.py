@@ -216,6 +216,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="first steps the CPU leg times")
     ap.add_argument("--no-stream", dest="stream", action="store_false", default=True,
                     help="skip the streaming calibration (read, copy, triad kernels)")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="only run the library's streaming calibration kernels (read, copy, triad over 2 GiB arrays)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a device-to-device copy of GIB GiB five times (scale of the HBM byte counters)")
     ap.add_argument("--differentiation", action="store_true",
@@ -224,6 +226,9 @@ def main():
     args = ap.parse_args()
     if args.differentiation:
         differentiation(args)
+        return
+    if args.stream_only:
+        print(json.dumps(stream_calibration()))
         return
     if args.copy_calibration > 0:
         import torch

@@ -132,17 +132,18 @@ __device__ void face_state_adjoint(const double *W, const double *G, int32_t c, 
 
 // one column of the limited gradient of species sp at cell c (Reconstruction.cpp:80-142 as in k_grad_current):
 // the input seeded is W[seed_cell][seed_m], or fs[seed_fs] of the inflow ghost
-__device__ void grad_column(const DevMesh &d, const Params &P, int32_t c, int32_t sp, const double *W, int32_t seed_cell, int32_t seed_m,
-                            int32_t seed_fs, Dual out[15])
+__device__ void grad_column(const DevMesh &d, const Params &P, int32_t c, int32_t sp, const double *W, int32_t seed_cell, uint32_t seed_m,
+                            uint32_t seed_fs, Dual out[15])
 {
+    // seed_m, seed_fs: bit masks of the primitives seeded (of seed_cell's W, of the free-stream state)
     const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1];
     Dual fs[5];
-    for (int m = 0; m < 5; ++m) fs[m] = Dual(P.fs[sp][m], m == seed_fs ? 1.0 : 0.0);
+    for (int m = 0; m < 5; ++m) fs[m] = Dual(P.fs[sp][m], (seed_fs >> m) & 1u ? 1.0 : 0.0);
     const V3 xc = L3(d.ccen, c);
     Dual wc[5], r[5][3], lo[5], hi[5];
     for (int m = 0; m < 5; ++m)
     {
-        wc[m] = Dual(W[5 * size_t(c) + m], (c == seed_cell && m == seed_m) ? 1.0 : 0.0);
+        wc[m] = Dual(W[5 * size_t(c) + m], (c == seed_cell && ((seed_m >> m) & 1u)) ? 1.0 : 0.0);
         lo[m] = hi[m] = wc[m];
         r[m][0] = r[m][1] = r[m][2] = 0.0;
     }
@@ -152,7 +153,7 @@ __device__ void grad_column(const DevMesh &d, const Params &P, int32_t c, int32_
         if (nb == kNoStencil) continue;
         Dual wn[5];
         if (nb >= 0)
-            for (int m = 0; m < 5; ++m) wn[m] = Dual(W[5 * size_t(nb) + m], (nb == seed_cell && m == seed_m) ? 1.0 : 0.0);
+            for (int m = 0; m < 5; ++m) wn[m] = Dual(W[5 * size_t(nb) + m], (nb == seed_cell && ((seed_m >> m) & 1u)) ? 1.0 : 0.0);
         else
         {
             const int32_t f = -(nb + 1);
@@ -306,7 +307,8 @@ __global__ void __launch_bounds__(256) k_adj_edge_emf(DevMesh d, Params P, const
         }
         bool any_fluid = false;
         for (int32_t k = c0; k < c1 && !any_fluid; ++k) any_fluid = d.mat[d.ec_cell[k]] == 0;
-        if (Ebar != 0.0 && nf > 0 && d.elen[e] > 0 && any_fluid)
+        // high-valence edges (the pole axes) take their UCT part in k_adj_edge_emf_heavy
+        if (Ebar != 0.0 && nf > 0 && nf <= kLightMax && d.elen[e] > 0 && any_fluid)
         {
             const V3 t = L3(d.evec, e);
             const double L = d.elen[e];
@@ -397,6 +399,149 @@ __global__ void __launch_bounds__(256) k_adj_edge_emf(DevMesh d, Params P, const
         }
     }
     block_sums<3>(einf, part);
+}
+
+// The UCT part of k_adj_edge_emf for the high-valence edges (valence > kLightMax: the pole axes, n_lon faces and cells),
+// one block per edge: the edge's cell velocities staged in LDS, the per-face terms in parallel (each face's max over
+// the cells read from LDS), the sums, the 2x2 solve and its adjoint by one thread in face order, and the eC
+// accumulation by one thread in face order, as the light kernel does them (which zeroed the edge's eF and eC slots).
+__global__ void __launch_bounds__(256) k_adj_edge_emf_heavy(DevMesh d, const double *u_cell, const double *phi_ind, double cuct,
+                                                            const double *Eb, const int32_t *heavy, double *eF, double *eC)
+{
+    constexpr int kH = kHeavyMax;
+    constexpr int kPer = kH / 256; // faces per thread
+    __shared__ double s_a[kH], s_b[kH], s_c[kH];
+    __shared__ int32_t s_k[kH];
+    __shared__ double s_r[8];
+    const int32_t e = heavy[blockIdx.x];
+    const int32_t f0 = d.e2f_ptr[e], nf = d.e2f_ptr[e + 1] - f0;
+    const int32_t c0 = d.ec_ptr[e], nc = d.ec_ptr[e + 1] - c0;
+    const double Ebar = Eb[e];
+    bool any_fluid = false;
+    for (int32_t k = 0; k < nc && !any_fluid; ++k) any_fluid = d.mat[d.ec_cell[c0 + k]] == 0;
+    // the same condition for every thread of the block: mode 1 and 2 edges carry no UCT part
+    if (d.emode[e] != 0 || !(Ebar != 0.0 && nf > kLightMax && nf <= kH && nc <= kH && d.elen[e] > 0 && any_fluid)) return;
+    const V3 t = L3(d.evec, e);
+    const double L = d.elen[e];
+    const V3 th = (1.0 / L) * t;
+    V3 e1, e2;
+    plane_basis(th, e1, e2);
+    for (int32_t k = threadIdx.x; k < nc; k += blockDim.x)
+    {
+        const V3 u = L3(u_cell, d.ec_cell[c0 + k]);
+        s_a[k] = u.x; s_b[k] = u.y; s_c[k] = u.z;
+    }
+    __syncthreads();
+    V3 m_r[kPer], n_r[kPer];
+    double aef_r[kPer], sig_r[kPer], qa_r[kPer], qb_r[kPer], sg_r[kPer];
+    int32_t km_r[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+    {
+        const int32_t i = int32_t(threadIdx.x) + j * int32_t(blockDim.x);
+        km_r[j] = -1;
+        if (i >= nf) continue;
+        const int32_t f = pid(d.e2f[f0 + i]);
+        face_dirs(d, f, e, th, m_r[j], n_r[j]);
+        double aef = 0.0, sg = 1.0;
+        for (int32_t k = 0; k < nc; ++k)
+        {
+            const double as = dot(V3(s_a[k], s_b[k], s_c[k]), m_r[j]), a = fabs(as);
+            if (aef < a) { aef = a; km_r[j] = k; sg = as < 0 ? -1.0 : 1.0; }
+        }
+        aef_r[j] = aef;
+        sg_r[j] = sg;
+        sig_r[j] = dot(th, cross(m_r[j], n_r[j]));
+        const V3 qv = (aef * sig_r[j]) * n_r[j];
+        qa_r[j] = dot(qv, e1);
+        qb_r[j] = dot(qv, e2);
+    }
+    __syncthreads(); // every read of the staged velocities is done
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+    {
+        const int32_t i = int32_t(threadIdx.x) + j * int32_t(blockDim.x);
+        if (i < nf) { s_a[i] = qa_r[j]; s_b[i] = qb_r[j]; s_c[i] = d.efw[f0 + i]; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double M11 = 0, M12 = 0, M22 = 0, g1 = 0, g2 = 0;
+        for (int32_t i = 0; i < nf; ++i)
+        {
+            const double w = s_c[i], qa = s_a[i], qb = s_b[i];
+            M11 += w * qa * qa; M12 += w * qa * qb; M22 += w * qb * qb; g1 += w * qa; g2 += w * qb;
+        }
+        double l1, l2;
+        uct_solve(M11, M12, M22, g1, g2, l1, l2);
+        s_r[0] = l1; s_r[1] = l2;
+        s_r[2] = M11; s_r[3] = M12; s_r[4] = M22; s_r[5] = g1; s_r[6] = g2;
+    }
+    __syncthreads();
+    const double Eu_b = 2.0 * cuct * L * Ebar, l1 = s_r[0], l2 = s_r[1];
+    // the omega weights' parts of the lambda adjoint, per face
+    double pf_r[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+    {
+        const int32_t i = int32_t(threadIdx.x) + j * int32_t(blockDim.x);
+        if (i >= nf) continue;
+        const int32_t f = pid(d.e2f[f0 + i]);
+        pf_r[j] = phi_ind[f] / d.famag[f];
+        const double w = d.efw[f0 + i], omb = Eu_b * aef_r[j] * sig_r[j] * pf_r[j];
+        s_a[i] = omb * w * qa_r[j];
+        s_b[i] = omb * w * qb_r[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double l1b = 0, l2b = 0;
+        for (int32_t i = 0; i < nf; ++i) { l1b -= s_a[i]; l2b -= s_b[i]; }
+        // (l1, l2) = solve(M11, M12, M22, g1, g2): transposed Jacobian by five forward columns
+        double sb[5];
+        for (int q = 0; q < 5; ++q)
+        {
+            Dual in[5] = {Dual(s_r[2]), Dual(s_r[3]), Dual(s_r[4]), Dual(s_r[5]), Dual(s_r[6])};
+            in[q].d = 1.0;
+            Dual a, b;
+            uct_solve(in[0], in[1], in[2], in[3], in[4], a, b);
+            sb[q] = l1b * a.d + l2b * b.d;
+        }
+        for (int q = 0; q < 5; ++q) s_r[2 + q] = sb[q];
+    }
+    __syncthreads();
+    const double sb0 = s_r[2], sb1 = s_r[3], sb2 = s_r[4], sb3 = s_r[5], sb4 = s_r[6];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+    {
+        const int32_t i = int32_t(threadIdx.x) + j * int32_t(blockDim.x);
+        if (i >= nf) continue;
+        const int32_t f = pid(d.e2f[f0 + i]);
+        const double w = d.efw[f0 + i], A = d.famag[f], pf = pf_r[j], aef = aef_r[j], sig = sig_r[j], qa = qa_r[j], qb = qb_r[j];
+        const double omega = w * (1.0 - (qa * l1 + qb * l2));
+        const double omb = Eu_b * aef * sig * pf;
+        double aefb = Eu_b * omega * sig * pf;
+        const double pfb = Eu_b * omega * aef * sig;
+        const double qab = -omb * w * l1 + 2.0 * w * qa * sb0 + w * qb * sb1 + w * sb3;
+        const double qbb = -omb * w * l2 + w * qa * sb1 + 2.0 * w * qb * sb2 + w * sb4;
+        const V3 sn = sig * n_r[j];
+        aefb += qab * dot(sn, e1) + qbb * dot(sn, e2);
+        eF[4 * size_t(f0 + i)] = Ebar * w * t.x;
+        eF[4 * size_t(f0 + i) + 1] = Ebar * w * t.y;
+        eF[4 * size_t(f0 + i) + 2] = Ebar * w * t.z;
+        eF[4 * size_t(f0 + i) + 3] = pfb / A;
+        s_k[i] = km_r[j];
+        const double gsc = aefb * sg_r[j];
+        s_a[i] = gsc * m_r[j].x; s_b[i] = gsc * m_r[j].y; s_c[i] = gsc * m_r[j].z;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int32_t i = 0; i < nf; ++i)
+            if (s_k[i] >= 0)
+            {
+                const size_t k = size_t(c0 + s_k[i]);
+                eC[3 * k] += s_a[i]; eC[3 * k + 1] += s_b[i]; eC[3 * k + 2] += s_c[i];
+            }
 }
 
 // face gather of the edge parts: Efb_f = sum over its edges, Phib_f += the induced-flux parts
@@ -724,31 +869,53 @@ __global__ void __launch_bounds__(256) k_adj_grad(DevMesh d, Params P, SpC Ws, S
             for (int q = 0; q < 15; ++q) nz = nz || gb[q] != 0.0;
             if (!d.svalid[c] || !nz) continue;
             const double *W = Ws.p[s];
-            auto column = [&](int32_t cell, int32_t m, int32_t fsm) {
+            // Columns of the local Jacobian, seeded several primitives at a time: primitive m's gradient depends on
+            // primitive m of the stencil only (LSQ sums, Barth-Jespersen and relative limiters per primitive), except
+            // through the wall ghost of the first species, which mirrors the cell's own velocity (components 1-3
+            // couple); those three are seeded one at a time, with every output summed, as a single column is.
+            auto columns = [&](int32_t cell, uint32_t mm, uint32_t fm, double g[5]) {
                 Dual out[15];
-                grad_column(d, P, c, s, W, cell, m, fsm, out);
+                grad_column(d, P, c, s, W, cell, mm, fm, out);
+                for (int m = 0; m < 5; ++m) g[m] = gb[3 * m] * out[3 * m].d + gb[3 * m + 1] * out[3 * m + 1].d + gb[3 * m + 2] * out[3 * m + 2].d;
+            };
+            auto column = [&](int32_t cell, int32_t m) {
+                Dual out[15];
+                grad_column(d, P, c, s, W, cell, 1u << m, 0u, out);
                 double g = 0;
                 for (int q = 0; q < 15; ++q) g += gb[q] * out[q].d;
                 return g;
             };
-            for (int m = 0; m < 5; ++m) Wb.p[s][5 * size_t(c) + m] += column(c, m, -1);
-            bool inflow = false;
+            bool wall = false, inflow = false;
             for (int32_t k = k0; k < k1; ++k)
             {
                 const int32_t nb = d.snb[k];
-                if (nb == kNoStencil) continue;
-                if (nb < 0)
-                {
-                    inflow = inflow || d.kind[-(nb + 1)] == kInflow;
-                    continue;
-                }
+                if (nb == kNoStencil || nb >= 0) continue;
+                wall = wall || d.kind[-(nb + 1)] == kWall;
+                inflow = inflow || d.kind[-(nb + 1)] == kInflow;
+            }
+            double g[5];
+            if (s == 0 && wall)
+            {
+                columns(c, (1u << 0) | (1u << 4), 0u, g);
+                g[1] = column(c, 1); g[2] = column(c, 2); g[3] = column(c, 3);
+            }
+            else columns(c, 0x1fu, 0u, g);
+            for (int m = 0; m < 5; ++m) Wb.p[s][5 * size_t(c) + m] += g[m];
+            for (int32_t k = k0; k < k1; ++k)
+            {
+                const int32_t nb = d.snb[k];
+                if (nb == kNoStencil || nb < 0) continue;
                 bool first = nb != c;
                 for (int32_t j = k0; j < k && first; ++j) first = d.snb[j] != nb;
                 if (!first) continue;
-                for (int m = 0; m < 5; ++m) sG.p[s][5 * size_t(k) + m] = column(nb, m, -1);
+                columns(nb, 0x1fu, 0u, g);
+                for (int m = 0; m < 5; ++m) sG.p[s][5 * size_t(k) + m] = g[m];
             }
             if (inflow)
-                for (int m = 0; m < 5; ++m) fsb[5 * s + m] += column(-1, 0, m);
+            {
+                columns(-1, 0u, 0x1fu, g);
+                for (int m = 0; m < 5; ++m) fsb[5 * s + m] += g[m];
+            }
         }
     }
     block_sums<5 * kAdjNs>(fsb, part);
@@ -1111,6 +1278,10 @@ double mhdg_ctx::step_vjp(int n_steps, double dt_max, double r0, double r1, doub
         // edge EMF
         LAUNCH("adj_edge_emf", k_adj_edge_emf, dim3(nbe), dim3(256), 0, stream, d, P, static_cast<const double *>(aw.up),
                static_cast<const double *>(aw.Phi), P.cuct, static_cast<const double *>(aw.Eb), aw.eF, aw.eC, aw.part);
+        if (n_heavy > 0)
+            LAUNCH("adj_edge_emf_heavy", k_adj_edge_emf_heavy, dim3(n_heavy), dim3(256), 0, stream, d, static_cast<const double *>(aw.up),
+                   static_cast<const double *>(aw.Phi), P.cuct, static_cast<const double *>(aw.Eb), static_cast<const int32_t *>(heavy), aw.eF,
+                   aw.eC);
         const std::vector<double> ei = sum_parts(*this, aw.part, nbe, 3);
         for (int k = 0; k < 3; ++k) eib[k] += ei[size_t(k)];
         LAUNCH("adj_gather_edge_faces", k_adj_gather_edge_faces, dim3(nbf), dim3(256), 0, stream, d, static_cast<const int32_t *>(aw.f2e_pos),

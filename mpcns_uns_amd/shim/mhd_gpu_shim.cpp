@@ -22,7 +22,7 @@
 // of the device call, with the first-order cell state as `inside` (exact for ghost functions that ignore the
 // inside state, which is the only use MarsSolver makes of it: MarsSolver.cpp:124-128, 148-167).
 // The implicit solve runs in FP64 (the reference iteration) unless MHDG_SOLVE_PRECISION=1 selects the
-// mixed-precision solve (2: mixed precision with the multigrid preconditioner).
+// mixed-precision solve (2: mixed precision with the multigrid preconditioner; 3: FP64 with classical Gram-Schmidt).
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -213,7 +213,7 @@ int solve_precision()
 {
     const char *p = std::getenv("MHDG_SOLVE_PRECISION");
     const int v = p ? std::atoi(p) : 0;
-    return (v == 1 || v == 2) ? v : 0;
+    return (v >= 1 && v <= 3) ? v : 0;
 }
 
 // ImplicitInduction instances -> a context of their own (each instance keeps its own warm-start correction,

@@ -30,14 +30,6 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 } // namespace
 
-// Block index that gives each XCD one contiguous range of blocks (the hardware runs block b on XCD b % 8), so that
-// neighbouring edges share an L2 for their gathers: 4 % faster for the edge kernel, slower for the cell kernels
-__device__ __forceinline__ int32_t xcd_block(uint32_t nb)
-{
-    const uint32_t b = blockIdx.x, per = nb / 8;
-    if (b >= 8 * per) return int32_t(b);
-    return int32_t((b % 8) * per + b / 8);
-}
 // Row range of a CSR table: fixed degree (deg > 0, every row the same length) or the ptr array
 __device__ __forceinline__ void rows(int32_t deg, const int32_t *ptr, int32_t r, int32_t &k0, int32_t &k1)
 {

@@ -17,6 +17,7 @@ namespace mhdg
 {
 namespace
 {
+constexpr int kPairUnroll = 4; // 16-byte loads (two row entries each) issued together in the FP32 edge kernel
 struct F3 { float x, y, z; };
 __device__ __forceinline__ F3 f3(const float *p, long i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 __device__ __forceinline__ F3 fcross(const F3 &a, const F3 &b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
@@ -42,6 +43,18 @@ __global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, DevMesh32 s, const fl
 {
     const int32_t f = gid();
     if (f >= d.nf) return;
+    if (s.fdeg == 3 && !dte)
+    {
+        // triangles: the face's three edge entries in one 12-byte load, then the three gathers, summed in entry order
+        const int3 q = reinterpret_cast<const int3 *>(d.f2e)[f];
+        const float x0 = pv[pid(q.x)], x1 = pv[pid(q.y)], x2 = pv[pid(q.z)];
+        float sum = 0.0f;
+        sum += fsg(q.x) * x0;
+        sum += fsg(q.y) * x1;
+        sum += fsg(q.z) * x2;
+        phi[f] = sum;
+        return;
+    }
     int32_t k0, k1;
     rows(s.fdeg, d.f2e_ptr, f, k0, k1);
     float sum = 0.0f;
@@ -134,38 +147,53 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, 
     const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
     if (f1 - f0 > kLightMax) return; // one of the heavy blocks
     float p0 = 0.0f, p1 = 0.0f;
+    // Row entries are read two per 16-byte load from the even entry at or before the row start (the pair arrays carry
+    // one padding entry at the end), half the load instructions of one 8-byte load per entry; entries outside the row
+    // are skipped, and the sums still run in entry order.
     if (resistive)
-        for (int32_t k = f0; k < f1; k += kEdgeUnroll)
+        for (int32_t k = f0 & ~1; k < f1; k += 2 * kPairUnroll)
         {
-            int2 q[kEdgeUnroll];
-            float x[kEdgeUnroll];
+            int4 q[kPairUnroll];
+            float x[2 * kPairUnroll];
 #pragma unroll
-            for (int u = 0; u < kEdgeUnroll; ++u)
-                if (k + u < f1) q[u] = s.e2fp[k + u];
+            for (int u = 0; u < kPairUnroll; ++u)
+                if (k + 2 * u < f1) q[u] = reinterpret_cast<const int4 *>(s.e2fp)[(k >> 1) + u];
 #pragma unroll
-            for (int u = 0; u < kEdgeUnroll; ++u)
-                if (k + u < f1) x[u] = phi[q[u].x];
+            for (int u = 0; u < kPairUnroll; ++u)
+            {
+                if (k + 2 * u >= f0 && k + 2 * u < f1) x[2 * u] = phi[q[u].x];
+                if (k + 2 * u + 1 >= f0 && k + 2 * u + 1 < f1) x[2 * u + 1] = phi[q[u].z];
+            }
 #pragma unroll
-            for (int u = 0; u < kEdgeUnroll; ++u)
-                if (k + u < f1) p0 += __int_as_float(q[u].y) * x[u];
+            for (int u = 0; u < kPairUnroll; ++u)
+            {
+                if (k + 2 * u >= f0 && k + 2 * u < f1) p0 += __int_as_float(q[u].y) * x[2 * u];
+                if (k + 2 * u + 1 >= f0 && k + 2 * u + 1 < f1) p0 += __int_as_float(q[u].w) * x[2 * u + 1];
+            }
         }
     if (hall)
     {
         const F3 t = f3(s.evec, e);
         const int32_t c0 = d.ec_ptr[e], c1 = d.ec_ptr[e + 1];
-        for (int32_t k = c0; k < c1; k += kEdgeUnroll)
+        for (int32_t k = c0 & ~1; k < c1; k += 2 * kPairUnroll)
         {
-            int2 q[kEdgeUnroll];
-            F3 x[kEdgeUnroll];
+            int4 q[kPairUnroll];
+            F3 x[2 * kPairUnroll];
 #pragma unroll
-            for (int u = 0; u < kEdgeUnroll; ++u)
-                if (k + u < c1) q[u] = s.ecp[k + u];
+            for (int u = 0; u < kPairUnroll; ++u)
+                if (k + 2 * u < c1) q[u] = reinterpret_cast<const int4 *>(s.ecp)[(k >> 1) + u];
 #pragma unroll
-            for (int u = 0; u < kEdgeUnroll; ++u)
-                if (k + u < c1) x[u] = f3(g, q[u].x);
+            for (int u = 0; u < kPairUnroll; ++u)
+            {
+                if (k + 2 * u >= c0 && k + 2 * u < c1) x[2 * u] = f3(g, q[u].x);
+                if (k + 2 * u + 1 >= c0 && k + 2 * u + 1 < c1) x[2 * u + 1] = f3(g, q[u].z);
+            }
 #pragma unroll
-            for (int u = 0; u < kEdgeUnroll; ++u)
-                if (k + u < c1) p1 += __int_as_float(q[u].y) * fdot(x[u], t);
+            for (int u = 0; u < kPairUnroll; ++u)
+            {
+                if (k + 2 * u >= c0 && k + 2 * u < c1) p1 += __int_as_float(q[u].y) * fdot(x[2 * u], t);
+                if (k + 2 * u + 1 >= c0 && k + 2 * u + 1 < c1) p1 += __int_as_float(q[u].w) * fdot(x[2 * u + 1], t);
+            }
         }
     }
     out[e] = edge_out<MODE>(s, e, dt * (s.m_e[e] * p0 + p1), v, pv, b, omega);
@@ -651,14 +679,15 @@ void mhdg_ctx::mixed_build()
     M.s.recon = f32(d.recon, 3 * nc2f);
     M.s.ecw = f32(d.ecw, nec);
     {
-        int2 *ecp = static_cast<int2 *>(alloc(nec * sizeof(int2)));
+        int2 *ecp = static_cast<int2 *>(alloc((nec + 1) * sizeof(int2))); // + 1: the last pair load of the last row
         hipLaunchKernelGGL(k_pack_ec, dim3(nb(nec)), dim3(256), 0, stream, d.ec_cell, M.s.ecw, int64_t(nec), ecp);
         M.s.ecp = ecp;
-        int2 *e2fp = static_cast<int2 *>(alloc(e2f_nnz * sizeof(int2)));
+        int2 *e2fp = static_cast<int2 *>(alloc((e2f_nnz + 1) * sizeof(int2)));
         hipLaunchKernelGGL(k_pack_e2f, dim3(nb(e2f_nnz)), dim3(256), 0, stream, d.e2f, M.s.star2, int64_t(e2f_nnz), e2fp);
         M.s.e2fp = e2fp;
     }
     M.s.evec = f32(d.evec, 3 * E);
+
     {
         float *aV = static_cast<float *>(alloc(3 * nc2f * sizeof(float)));
         int32_t *nbr = static_cast<int32_t *>(alloc(nc2f * sizeof(int32_t)));

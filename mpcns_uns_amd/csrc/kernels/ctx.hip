@@ -209,6 +209,7 @@ void mhdg_ctx::build(const mhdg_mesh_desc &m, const mhdg_config &c)
     if (c.ind_restart > 60) throw std::runtime_error("ind_restart above 60 is not supported");
     const size_t E = ne, F = nf, C = nc;
     d.nc = nc; d.nf = nf; d.ne = ne;
+    d.cdeg = cdeg; d.fdeg = fdeg;
     d.c2f_ptr = upload(t.c2f_ptr); d.c2f = upload(t.c2f); d.cfs = upload(t.cfs);
     d.f2e_ptr = upload(t.f2e_ptr); d.f2e = upload(t.f2e);
     d.e2f_ptr = upload(t.e2f_ptr); d.e2f = upload(t.e2f);

@@ -28,6 +28,7 @@ using Prim = PrimT<double>;
 struct DevMesh
 {
     int32_t nc, nf, ne;
+    int32_t cdeg, fdeg;             // faces per cell / edges per face when every row has that many (tets: 4, 3), else 0
     const int32_t *c2f_ptr, *c2f;   // cell -> faces, stored order
     const int32_t *cfs;             // cell -> faces sorted by face id (same ptr as c2f), signed
     const int32_t *f2e_ptr, *f2e;   // face -> edges, loop order

@@ -132,6 +132,29 @@ __global__ void __launch_bounds__(256) k_d1(DevMesh d, const double *v, const do
     const int32_t f = gid();
     if (f >= d.nf) return;
     double s = 0.0;
+    if (d.fdeg == 3 && !dte)
+    {
+        // triangles: the face's three edge entries in one 12-byte load, then the gathers, summed in entry order
+        const int3 q = reinterpret_cast<const int3 *>(d.f2e)[f];
+        const int32_t e[3] = {pid(q.x), pid(q.y), pid(q.z)};
+        double x[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+        {
+            const double ve = v[e[u]];
+            if (invdiag)
+            {
+                const double iv = invdiag[e[u]];
+                x[u] = dir[e[u]] ? 0.0 : ve * iv;
+            }
+            else x[u] = ve;
+        }
+        s += psg(q.x) * x[0];
+        s += psg(q.y) * x[1];
+        s += psg(q.z) * x[2];
+        phi[f] = s;
+        return;
+    }
     const int32_t k0 = d.f2e_ptr[f], k1 = d.f2e_ptr[f + 1];
     for (int32_t k = k0; k < k1; k += kRowUnroll)
     {
@@ -240,7 +263,9 @@ template <int G> __global__ void __launch_bounds__(256) k_bcell_g(DevMesh d, con
     const int32_t t = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
     const int32_t c = t / G, lane = t % G;
     const bool ok = c < d.nc;
-    const int32_t k0 = ok ? d.c2f_ptr[c] : 0, k1 = ok ? d.c2f_ptr[c + 1] : 0;
+    // a fixed row length (tetrahedra) spares the row-bound loads ahead of the gathers
+    const int32_t k0 = !ok ? 0 : d.cdeg > 0 ? c * d.cdeg : d.c2f_ptr[c];
+    const int32_t k1 = !ok ? 0 : d.cdeg > 0 ? k0 + d.cdeg : d.c2f_ptr[c + 1];
     double bx = 0.0, by = 0.0, bz = 0.0;
     for (int32_t base = k0; base < k1; base += G)
     {
@@ -272,7 +297,7 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_g(DevMesh d, co
     {
         V3 sum;
         const V3 bc = L3(b, c);
-        const int32_t k0 = d.c2f_ptr[c], k1 = d.c2f_ptr[c + 1];
+        const int32_t k0 = d.cdeg > 0 ? c * d.cdeg : d.c2f_ptr[c], k1 = d.cdeg > 0 ? k0 + d.cdeg : d.c2f_ptr[c + 1];
         for (int32_t base = k0; base < k1; base += G)
         {
             const int32_t k = base + lane;

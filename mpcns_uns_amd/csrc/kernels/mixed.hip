@@ -145,6 +145,10 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, 
     if (e >= d.ne) return;
     if (dir[e]) { out[e] = 0.0f; return; }
     const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
+    // the cell row's bounds and the edge vector load with the face row's bounds, not after the face row's sum
+    int32_t c0 = 0, c1 = 0;
+    F3 t{0.0f, 0.0f, 0.0f};
+    if (hall) { c0 = d.ec_ptr[e]; c1 = d.ec_ptr[e + 1]; t = f3(s.evec, e); }
     if (f1 - f0 > kLightMax) return; // one of the heavy blocks
     float p0 = 0.0f, p1 = 0.0f;
     // Row entries are read two per 16-byte load from the even entry at or before the row start (the pair arrays carry
@@ -173,8 +177,6 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, 
         }
     if (hall)
     {
-        const F3 t = f3(s.evec, e);
-        const int32_t c0 = d.ec_ptr[e], c1 = d.ec_ptr[e + 1];
         for (int32_t k = c0 & ~1; k < c1; k += 2 * kPairUnroll)
         {
             int4 q[kPairUnroll];
@@ -246,12 +248,25 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_f32_g(DevMesh d
     const int32_t t = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
     const int32_t c = t / G, lane = t % G;
     const bool ok = c < d.nc;
+    // alpha B0 gates only the result: a test of it ahead of the neighbour gathers made every load of the cell wait
+    // for it (174 against 151 us per launch on the 8M shell)
     const F3 ab = ok ? f3(s.aB0, c) : F3{0.0f, 0.0f, 0.0f};
-    const bool hall = ab.x != 0.0f || ab.y != 0.0f || ab.z != 0.0f;
     F3 J{0.0f, 0.0f, 0.0f};
-    if (ok)
+    if (ok && s.cdeg == G)
     {
-        // the table loads do not wait for alpha B0; the neighbour gathers do
+        // one entry per lane (tetrahedra with four lanes): no row bounds to wait for
+        const int32_t k = c * G + lane, nb = s.nbr[k];
+        const F3 av = f3(s.aV, k), bc = f3(b, c);
+        F3 bf = bc;
+        if (nb >= 0)
+        {
+            const F3 bo = f3(b, nb);
+            bf = {0.5f * (bc.x + bo.x), 0.5f * (bc.y + bo.y), 0.5f * (bc.z + bo.z)};
+        }
+        J = fcross(av, bf);
+    }
+    else if (ok)
+    {
         int32_t k0, k1;
         rows(s.cdeg, d.c2f_ptr, c, k0, k1);
         const F3 bc = f3(b, c);
@@ -259,7 +274,6 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_f32_g(DevMesh d
         {
             const int32_t nb = s.nbr[k];
             const F3 av = f3(s.aV, k);
-            if (!hall) continue;
             F3 bf = bc;
             if (nb >= 0)
             {
@@ -273,6 +287,7 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_f32_g(DevMesh d
     J.x = group_sum<G>(J.x); J.y = group_sum<G>(J.y); J.z = group_sum<G>(J.z);
     if (ok && lane == 0)
     {
+        const bool hall = ab.x != 0.0f || ab.y != 0.0f || ab.z != 0.0f;
         const F3 out = hall ? fcross(J, ab) : F3{0.0f, 0.0f, 0.0f};
         g[3 * size_t(c)] = out.x; g[3 * size_t(c) + 1] = out.y; g[3 * size_t(c) + 2] = out.z;
     }

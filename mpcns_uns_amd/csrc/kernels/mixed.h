@@ -33,8 +33,6 @@ __global__ void k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int3
                                  const float *pv, const int8_t *dir, float dt, float *out, const int32_t *heavy, int32_t n_light, const float *b,
                                  float omega);
 __global__ void k_tables_f32(DevMesh d, float *aV, int32_t *nbr);
-__global__ void k_multidot_f32(const float *const *V, int32_t i0, int32_t nv, const float *w, int32_t n, double *part, int32_t with_ww,
-                               RedScratch rs, double *out);
 __global__ void k_multiaxpy_f32(const float *const *V, int32_t nv, const double *h, float *w, int32_t n, RedScratch rs, double *nrm);
 __global__ void k_to_f32_scaled(const double *x, const double *s, const float *invdiag, const int8_t *dir, int32_t n, float *out, float *pv);
 __global__ void k_scale_f32(const float *w, const double *h, const float *invdiag, const int8_t *dir, int32_t n, float *out, float *pv);

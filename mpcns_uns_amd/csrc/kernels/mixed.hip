@@ -6,6 +6,8 @@
 //                                     one multi-axpy pass per step, re-orthogonalised when the DGKS test
 //                                     asks for it), Hessenberg and Givens in FP64 on the host.
 #include <algorithm>
+#include <array>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -23,6 +25,15 @@ __device__ __forceinline__ F3 f3(const float *p, long i) { return {p[3 * i], p[3
 __device__ __forceinline__ F3 fcross(const F3 &a, const F3 &b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 __device__ __forceinline__ float fdot(const F3 &a, const F3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ float fsg(int32_t v) { return v >= 0 ? 1.0f : -1.0f; }
+// a float4 load through a pointer known to be global memory (a pointer read from LDS or an array is generic, and its
+// loads become flat loads that also count against the LDS wait counter)
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) F4v *GlobalF4;
+__device__ __forceinline__ float4 ldg4(const float *p, int32_t q)
+{
+    const F4v v = ((GlobalF4)p)[q];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -316,31 +327,31 @@ __global__ void __launch_bounds__(256) k_tables_f32(DevMesh d, float *aV, int32_
     }
 }
 
-// h[i0+k] = <w, V_{i0+k}> for k < nv (FP64 accumulation, 4 elements per thread and iteration); with_ww
-// also <w, w> into out[63]
-__global__ void __launch_bounds__(256) k_multidot_f32(const float *const *V, int32_t i0, int32_t nv, const float *w, int32_t n,
-                                                      double *part, int32_t with_ww, RedScratch rs, double *out)
+// h[i0+k] = <w, V_{i0+k}> for k < NV (FP64 accumulation, 4 elements per thread and iteration); with_ww
+// also <w, w> into out[63].  NV is a template argument so that the NV loads of an iteration are issued together (a
+// run-time count guarded each load by a branch and kept one load in flight per thread).
+template <int NV>
+__global__ void __launch_bounds__(256) k_multidot_f32(const float *const *V, int32_t i0, const float *w, int32_t n, double *part,
+                                                      int32_t with_ww, RedScratch rs, double *out)
 {
     __shared__ double sh[4][kDotChunk + 1];
     __shared__ int last;
     double acc[kDotChunk + 1];
 #pragma unroll
     for (int k = 0; k <= kDotChunk; ++k) acc[k] = 0.0;
-    const float *Vk[kDotChunk];
+    const float *Vk[NV];
 #pragma unroll
-    for (int k = 0; k < kDotChunk; ++k) Vk[k] = k < nv ? V[i0 + k] : V[i0];
+    for (int k = 0; k < NV; ++k) Vk[k] = V[i0 + k];
     const int32_t n4 = n >> 2;
-    const float4 *w4 = reinterpret_cast<const float4 *>(w);
     for (int32_t q = blockIdx.x * 256 + threadIdx.x; q < n4; q += gridDim.x * 256)
     {
-        const float4 a = w4[q];
+        const float4 a = ldg4(w, q);
+        float4 b[NV];
 #pragma unroll
-        for (int k = 0; k < kDotChunk; ++k)
-            if (k < nv)
-            {
-                const float4 b = reinterpret_cast<const float4 *>(Vk[k])[q];
-                acc[k] += (double(a.x) * b.x + double(a.y) * b.y) + (double(a.z) * b.z + double(a.w) * b.w);
-            }
+        for (int k = 0; k < NV; ++k) b[k] = ldg4(Vk[k], q);
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            acc[k] += (double(a.x) * b[k].x + double(a.y) * b[k].y) + (double(a.z) * b[k].z + double(a.w) * b[k].w);
         acc[kDotChunk] += (double(a.x) * a.x + double(a.y) * a.y) + (double(a.z) * a.z + double(a.w) * a.w);
     }
     if (blockIdx.x == 0 && threadIdx.x < (n & 3))
@@ -348,19 +359,19 @@ __global__ void __launch_bounds__(256) k_multidot_f32(const float *const *V, int
         const int32_t e = 4 * n4 + threadIdx.x;
         const double a = w[e];
 #pragma unroll
-        for (int k = 0; k < kDotChunk; ++k)
-            if (k < nv) acc[k] += a * Vk[k][e];
+        for (int k = 0; k < NV; ++k) acc[k] += a * Vk[k][e];
         acc[kDotChunk] += a * a;
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k <= kDotChunk; ++k)
-    {
-        const double sm = wave_sum(acc[k]);
-        if (lane == 0) sh[wv][k] = sm;
-    }
+        if (k < NV || k == kDotChunk)
+        {
+            const double sm = wave_sum(acc[k]);
+            if (lane == 0) sh[wv][k] = sm;
+        }
     __syncthreads();
-    if (threadIdx.x <= kDotChunk)
+    if (threadIdx.x < NV || threadIdx.x == kDotChunk)
     {
         const int k = threadIdx.x;
         part[size_t(blockIdx.x) * (kDotChunk + 1) + k] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
@@ -378,22 +389,66 @@ __global__ void __launch_bounds__(256) k_multidot_f32(const float *const *V, int
     for (uint32_t bl = threadIdx.x; bl < gridDim.x; bl += 256)
 #pragma unroll
         for (int k = 0; k <= kDotChunk; ++k)
-            t[k] += __hip_atomic_load(&part[size_t(bl) * (kDotChunk + 1) + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (k < NV || k == kDotChunk)
+                t[k] += __hip_atomic_load(&part[size_t(bl) * (kDotChunk + 1) + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int k = 0; k <= kDotChunk; ++k)
-    {
-        const double s2 = wave_sum(t[k]);
-        if (lane == 0) sh[wv][k] = s2;
-    }
+        if (k < NV || k == kDotChunk)
+        {
+            const double s2 = wave_sum(t[k]);
+            if (lane == 0) sh[wv][k] = s2;
+        }
     __syncthreads();
-    if (threadIdx.x <= kDotChunk)
+    if (threadIdx.x < NV || threadIdx.x == kDotChunk)
     {
         const int k = threadIdx.x;
         const double s2 = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
-        if (k < nv) out[i0 + k] = s2;
+        if (k < NV) out[i0 + k] = s2;
         if (k == kDotChunk && with_ww) out[63] = s2;
     }
     if (threadIdx.x == 0) *rs.ticket = 0u;
+}
+using MultidotFn = void (*)(const float *const *, int32_t, const float *, int32_t, double *, int32_t, RedScratch, double *);
+template <int... I> constexpr std::array<MultidotFn, sizeof...(I) + 1> multidot_table(std::integer_sequence<int, I...>)
+{
+    return {nullptr, &k_multidot_f32<I + 1>...};
+}
+// kMultidot[nv]: the multi-dot pass over nv basis vectors, 1 <= nv <= kDotChunk
+const std::array<MultidotFn, kDotChunk + 1> kMultidot = multidot_table(std::make_integer_sequence<int, kDotChunk>{});
+
+// x -= h_i V_i for the G vectors from i0 on, in order (their G loads issued together)
+template <int G>
+__device__ __forceinline__ void axpy_group(const float *const *vp, const double *hs, int32_t i0, int32_t q, double &x, double &y, double &z,
+                                           double &u)
+{
+    float4 b[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) b[k] = ldg4(vp[i0 + k], q);
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+    {
+        const double hi = hs[i0 + k];
+        x -= hi * b[k].x; y -= hi * b[k].y; z -= hi * b[k].z; u -= hi * b[k].w;
+    }
+}
+constexpr int kAxpyGroup = 8;
+// x -= sum_{i < nv} h_i V_i in order of i: groups of kAxpyGroup loads, then the remainder's loads together
+__device__ __forceinline__ void axpy_all(const float *const *vp, const double *hs, int32_t nv, int32_t q, double &x, double &y, double &z,
+                                         double &u)
+{
+    int32_t i = 0;
+    for (; i + kAxpyGroup <= nv; i += kAxpyGroup) axpy_group<kAxpyGroup>(vp, hs, i, q, x, y, z, u);
+    switch (nv - i)
+    {
+    case 7: axpy_group<7>(vp, hs, i, q, x, y, z, u); break;
+    case 6: axpy_group<6>(vp, hs, i, q, x, y, z, u); break;
+    case 5: axpy_group<5>(vp, hs, i, q, x, y, z, u); break;
+    case 4: axpy_group<4>(vp, hs, i, q, x, y, z, u); break;
+    case 3: axpy_group<3>(vp, hs, i, q, x, y, z, u); break;
+    case 2: axpy_group<2>(vp, hs, i, q, x, y, z, u); break;
+    case 1: axpy_group<1>(vp, hs, i, q, x, y, z, u); break;
+    default: break;
+    }
 }
 
 // w -= sum_{i < nv} h_i V_i (FP64 arithmetic, FP32 storage, 4 elements per thread); nrm[0] = |w| after
@@ -413,12 +468,7 @@ __global__ void __launch_bounds__(256) k_multiaxpy_f32(const float *const *V, in
     {
         const float4 a = w4[q];
         double x = a.x, y = a.y, z = a.z, u = a.w;
-        for (int32_t i = 0; i < nv; ++i)
-        {
-            const float4 b = reinterpret_cast<const float4 *>(vp[i])[q];
-            const double hi = hs[i];
-            x -= hi * b.x; y -= hi * b.y; z -= hi * b.z; u -= hi * b.w;
-        }
+        axpy_all(vp, hs, nv, q, x, y, z, u);
         const float4 r = make_float4(float(x), float(y), float(z), float(u));
         w4[q] = r;
         acc += (double(r.x) * r.x + double(r.y) * r.y) + (double(r.z) * r.z + double(r.w) * r.w);
@@ -514,12 +564,7 @@ __global__ void __launch_bounds__(256) k_multiaxpy_scale_f32(const float *const 
     {
         const float4 a = w4[q];
         double x = a.x, y = a.y, z = a.z, u = a.w;
-        for (int32_t i = 0; i < nv; ++i)
-        {
-            const float4 b = reinterpret_cast<const float4 *>(vp[i])[q];
-            const double hi = hs[i];
-            x -= hi * b.x; y -= hi * b.y; z -= hi * b.z; u -= hi * b.w;
-        }
+        axpy_all(vp, hs, nv, q, x, y, z, u);
         if (direct)
         {
             const float4 v = make_float4(float(x * inv), float(y * inv), float(z * inv), float(u * inv));
@@ -607,14 +652,35 @@ __global__ void __launch_bounds__(256) k_scale_f32(const float *w, const double 
     out[e] = v;
     pv[e] = dir[e] ? 0.0f : v * invdiag[e];
 }
+// delta += P sum_{i < j} y_i V_i on the non-Dirichlet edges, four edges per thread (the grid covers n / 4 + 1
+// threads), the sum in order of i with the loads of a group of vectors issued together
 __global__ void __launch_bounds__(256) k_update_mixed(const float *const *V, const double *y, int32_t j, const float *invdiag,
                                                       const int8_t *dir, int32_t n, double *delta)
 {
-    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n || dir[e]) return;
-    double s = 0.0;
-    for (int32_t i = 0; i < j; ++i) s += y[i] * static_cast<double>(V[i][e]);
-    delta[e] += s * static_cast<double>(invdiag[e]);
+    __shared__ double ny[64];
+    __shared__ const float *vp[64];
+    if (threadIdx.x < j) { ny[threadIdx.x] = -y[threadIdx.x]; vp[threadIdx.x] = V[threadIdx.x]; }
+    __syncthreads();
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x, n4 = n >> 2;
+    if (q < n4)
+    {
+        // s - (-y_i) v = s + y_i v exactly
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        axpy_all(vp, ny, j, q, s[0], s[1], s[2], s[3]);
+        const float4 iv = ldg4(invdiag, q);
+        const float ivs[4] = {iv.x, iv.y, iv.z, iv.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (!dir[4 * q + k]) delta[4 * q + k] += s[k] * static_cast<double>(ivs[k]);
+    }
+    else if (q == n4)
+        for (int32_t e = 4 * n4; e < n; ++e)
+        {
+            if (dir[e]) continue;
+            double s = 0.0;
+            for (int32_t i = 0; i < j; ++i) s += y[i] * static_cast<double>(vp[i][e]);
+            delta[e] += s * static_cast<double>(invdiag[e]);
+        }
 }
 __global__ void __launch_bounds__(256) k_pack_ec(const int32_t *cell, const float *w, int64_t n, int2 *out)
 {
@@ -820,7 +886,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
             for (int i0 = 0; i0 <= j; i0 += kDotChunk)
             {
                 const int nv = std::min(kDotChunk, j + 1 - i0);
-                LAUNCH("multidot", k_multidot_f32, dim3(kOrthoBlocks), dim3(256), 0, stream, Vp, i0, nv, M.w, ne, M.part, i0 == 0 ? 1 : 0,
+                LAUNCH("multidot", kMultidot[nv], dim3(kOrthoBlocks), dim3(256), 0, stream, Vp, i0, M.w, ne, M.part, i0 == 0 ? 1 : 0,
                        RedScratch{nullptr, nullptr}, M.h);
                 LAUNCH("multidot_final", k_multidot_final, dim3(1), dim3(256), 0, stream, static_cast<const double *>(M.part), kOrthoBlocks,
                        i0, nv, i0 == 0 ? 1 : 0, M.h);
@@ -833,7 +899,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
             {
                 // DGKS: one more projection
                 for (int i0 = 0; i0 <= j; i0 += kDotChunk)
-                    LAUNCH("multidot", k_multidot_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, i0, std::min(kDotChunk, j + 1 - i0), M.w,
+                    LAUNCH("multidot", kMultidot[std::min(kDotChunk, j + 1 - i0)], dim3(kRedBlocks), dim3(256), 0, stream, Vp, i0, M.w,
                            ne, M.part, 0, rs, M.h);
                 LAUNCH("multiaxpy", k_multiaxpy_f32, dim3(kRedBlocks), dim3(256), 0, stream, Vp, j + 1, M.h, M.w, ne, rs, M.h + j + 1);
                 d2h(c2.data(), M.h, sizeof(double) * 64);
@@ -868,7 +934,7 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
         if (use_mg)
             LAUNCH("update_flex", k_update_flex, dim3(nb(ne)), dim3(256), 0, stream, static_cast<const float *const *>(M.Zptrs), ydev, j, dir, ne, delta);
         else
-            LAUNCH("update_mixed", k_update_mixed, dim3(nb(ne)), dim3(256), 0, stream, Vp, ydev, j, M.s.invdiag, dir, ne, delta);
+            LAUNCH("update_mixed", k_update_mixed, dim3(nb(ne / 4 + 1)), dim3(256), 0, stream, Vp, ydev, j, M.s.invdiag, dir, ne, delta);
     }
     LAUNCH("solve_final", k_solve_final, dim3(nb(ne)), dim3(256), 0, stream, x0, delta, Eex, dir, ne, Eout, corr);
     check_launch("mixed solve");

@@ -27,7 +27,7 @@ PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 # HBM bytes per launch moved by a kernel on a workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by
 # the same counters on a 1 GiB device copy (x2.00 and x1.00); the command and the per-kernel table are in
 # profiles/hall_step_shell_8m_hbm_counters.md.  Counter runs cannot share a process with this measurement.
-PMC_TRAFFIC = {("shell_8m_mixed", "lat", "edge_apply_f32"): 1376126198}
+PMC_TRAFFIC = {("shell_8m_mixed", "lat", "edge_apply_f32"): 1369223077}
 
 
 def kernel_bytes(mesh, name, precision_bytes=8, ns=2):

@@ -183,9 +183,7 @@ int mhdg_ctx::solve64_cgs(const double *Phi_n, const double *Eex, const int8_t *
     }
     LAUNCH("solve_init", k_solve_init, dim3(nbk(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt,
            dte_cur, ne, x0, invdiag);
-    apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp); // F(Phi^n)
-    LAUNCH("axpy", k_add, dim3(nbk(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
-    apply_op(nullptr, x0, nullptr, dir, dt, 1, 0, bvec, rvec); // r = b - A x0
+    rhs_residual(Phi_n, Eex, x0, dir, dt); // b = E_expl + F(Phi^n), r = b - A x0
     chk(hipMemsetAsync(delta, 0, size_t(ne) * 8, stream), "memset");
     const double bb = dot(bvec, bvec, 0, 0);
     const double bnorm = std::sqrt(std::max(bb, 1e-300));

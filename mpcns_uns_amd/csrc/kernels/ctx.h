@@ -110,6 +110,10 @@ struct mhdg_ctx
     double dot(const double *a, const double *b, int op, int slot);
     void apply_op(const double *phi, const double *v, const double *inv, const int8_t *dir, double dt, int mode, int mask, const double *b,
                   double *out);
+    // bvec = E_expl + F(Phi_n) and rvec = bvec - A x (no Jacobi scaling), the two products in one pass over the
+    // tables where they allow it (no implicit ideal part, no tangent operator); the same bits as two apply_op calls
+    void rhs_residual(const double *Phi_n, const double *E_expl, const double *x, const int8_t *dir, double dt);
+    double *bcell2 = nullptr, *gcell2 = nullptr; // the second vector's cell fields of rhs_residual
     int solve64(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);
     // the same system and iteration with classical Gram-Schmidt (cgs64.hip, precision 3)
     int solve64_cgs(const double *Phi_n, const double *E_expl, const int8_t *dir, double dt, double *E, double *rel_res);

@@ -352,6 +352,37 @@ void mhdg_ctx::apply_op(const double *phi_in, const double *v, const double *inv
     check_launch("apply_op");
 }
 
+void mhdg_ctx::rhs_residual(const double *Phi_n, const double *Eex, const double *x, const int8_t *dir, double dt)
+{
+    if ((ideal_cfg && u_ideal) || lin_b)
+    {
+        apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp); // F(Phi^n)
+        LAUNCH("axpy", k_add, dim3(nblk(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
+        apply_op(nullptr, x, nullptr, dir, dt, 1, 0, bvec, rvec); // r = b - A x
+        return;
+    }
+    const size_t C = nc;
+    if (!bcell2) { bcell2 = static_cast<double *>(alloc(3 * C * 8)); gcell2 = static_cast<double *>(alloc(3 * C * 8)); }
+    LAUNCH("d1", k_d1, dim3(nblk(nf)), dim3(256), 0, stream, d, x, static_cast<const double *>(nullptr), dir, dte_cur, phi_w);
+    if (dte_cur) dt = 1.0; // d1 already carries dt_e (ImplicitInduction.cpp:196)
+    const int64_t lanes = kCellLanes64 * int64_t(nc);
+    if (op_hall)
+    {
+        LAUNCH("bcell", k_bcell2_g<kCellLanes64>, dim3(nblk(lanes)), dim3(256), 0, stream, d, Phi_n, static_cast<const double *>(phi_w), bcell,
+               bcell2);
+        LAUNCH("hall_g", k_hall_g2_g<kCellLanes64>, dim3(nblk(lanes)), dim3(256), 0, stream, d, static_cast<const double *>(bcell),
+               static_cast<const double *>(bcell2), alpha, B0, gcell, gcell2);
+    }
+    LAUNCH("edge_apply", k_edge_rhs_residual, dim3(nblk(ne)), dim3(256), 0, stream, d, int32_t(op_res), int32_t(op_hall), Phi_n,
+           static_cast<const double *>(gcell), static_cast<const double *>(phi_w), static_cast<const double *>(gcell2), Eex, x, dir, dt, bvec,
+           rvec);
+    if (n_heavy)
+        LAUNCH("edge_apply_heavy", k_edge_rhs_residual_heavy, dim3(n_heavy), dim3(256), 0, stream, d, int32_t(op_res), int32_t(op_hall),
+               static_cast<const int32_t *>(heavy), Phi_n, static_cast<const double *>(gcell), static_cast<const double *>(phi_w),
+               static_cast<const double *>(gcell2), Eex, x, dir, dt, bvec, rvec);
+    check_launch("rhs_residual");
+}
+
 // ImplicitInduction::solve (Common/induction/ImplicitInduction.cpp:299-390), FP64, Jacobi, MGS Arnoldi:
 // the reference's control flow with the Hessenberg / Givens arithmetic on the host.
 int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir, double dt, double *Eout, double *rel_res)
@@ -361,9 +392,7 @@ int mhdg_ctx::solve64(const double *Phi_n, const double *Eex, const int8_t *dir,
     const double rtol = rtol_override > 0 ? rtol_override : cfg.ind_rtol;
     LAUNCH("solve_init", k_solve_init, dim3(nblk(ne)), dim3(256), 0, stream, Eex, have_corr ? corr : nullptr, dir, d.m_e, d.d2, dt,
            dte_cur, ne, x0, invdiag);
-    apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp); // F(Phi^n)
-    LAUNCH("axpy", k_add, dim3(nblk(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
-    apply_op(nullptr, x0, nullptr, dir, dt, 1, 0, bvec, rvec); // r = b - A x0
+    rhs_residual(Phi_n, Eex, x0, dir, dt); // b = E_expl + F(Phi^n), r = b - A x0
     hip_check(hipMemsetAsync(delta, 0, size_t(ne) * 8, stream), "memset");
     const double bb = dot(bvec, bvec, 0, 0);
     const double bnorm = std::sqrt(std::max(bb, 1e-300));

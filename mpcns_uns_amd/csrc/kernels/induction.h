@@ -29,6 +29,16 @@ __global__ void k_edge_apply(DevMesh d, int32_t resistive, int32_t hall, const d
 __global__ void k_edge_apply_heavy(DevMesh d, int32_t resistive, int32_t hall, const int32_t *heavy, const double *phi, const double *g,
                                    const double *ide, const int8_t *imask, const double *v, const double *invdiag, const int8_t *dir,
                                    double dt, int32_t mode, int32_t mask, const double *bvec, double *out);
+// b = E_expl + F(Phi^n) and r0 = b - A x0 in one pass over the tables (the same bits as the separate products)
+template <int G> __global__ void k_bcell2_g(DevMesh d, const double *phi1, const double *phi2, double *b1, double *b2);
+template <int G>
+__global__ void k_hall_g2_g(DevMesh d, const double *b1, const double *b2, const double *alpha, const double *B0, double *g1, double *g2);
+__global__ void k_edge_rhs_residual(DevMesh d, int32_t resistive, int32_t hall, const double *phi1, const double *g1, const double *phi2,
+                                    const double *g2, const double *E_expl, const double *x, const int8_t *dir, double dt, double *bvec,
+                                    double *r);
+__global__ void k_edge_rhs_residual_heavy(DevMesh d, int32_t resistive, int32_t hall, const int32_t *heavy, const double *phi1,
+                                          const double *g1, const double *phi2, const double *g2, const double *E_expl, const double *x,
+                                          const int8_t *dir, double dt, double *bvec, double *r);
 __global__ void k_gmres_update(const double *const *V, const double *y, int32_t j, const double *invdiag, const int8_t *dir, int32_t n,
                                double *delta);
 __global__ void k_solve_init(const double *E_expl, const double *corr, const int8_t *dir, const double *m_e, const double *d2, double dt,

@@ -446,6 +446,214 @@ __global__ void __launch_bounds__(256) k_edge_apply_heavy(DevMesh d, int32_t res
     out[e] = r;
 }
 
+// ---------------------------------------------------------------- right-hand side and first residual, fused
+// A solve starts with two products of F on different vectors, b = E_expl + F(Phi^n) and r0 = b - A x0 (A x0 formed
+// from Phi' = d1 x0). The kernels below evaluate both in one pass over the tables: every row index and table value
+// is loaded once for the two vectors, and each vector's terms are the same operations in the same order as in
+// k_bcell_g, k_hall_g_g and k_edge_apply, so b and r0 are the same bits as the two separate products give.
+template <int G> __global__ void __launch_bounds__(256) k_bcell2_g(DevMesh d, const double *phi1, const double *phi2, double *b1, double *b2)
+{
+    const int32_t t = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    const int32_t c = t / G, lane = t % G;
+    const bool ok = c < d.nc;
+    const int32_t k0 = !ok ? 0 : d.cdeg > 0 ? c * d.cdeg : d.c2f_ptr[c];
+    const int32_t k1 = !ok ? 0 : d.cdeg > 0 ? k0 + d.cdeg : d.c2f_ptr[c + 1];
+    double bx[2] = {0.0, 0.0}, by[2] = {0.0, 0.0}, bz[2] = {0.0, 0.0};
+    for (int32_t base = k0; base < k1; base += G)
+    {
+        const int32_t k = base + lane;
+        double px[2] = {0.0, 0.0}, py[2] = {0.0, 0.0}, pz[2] = {0.0, 0.0};
+        if (k < k1)
+        {
+            const int32_t f = pid(d.c2f[k]);
+            const double v[2] = {phi1[f], phi2[f]};
+            const double r0 = d.recon[3 * size_t(k)], r1 = d.recon[3 * size_t(k) + 1], r2 = d.recon[3 * size_t(k) + 2];
+#pragma unroll
+            for (int n = 0; n < 2; ++n) { px[n] = r0 * v[n]; py[n] = r1 * v[n]; pz[n] = r2 * v[n]; }
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
+            {
+                const double qx = __shfl(px[n], q, G), qy = __shfl(py[n], q, G), qz = __shfl(pz[n], q, G);
+                if (base + q < k1) { bx[n] += qx; by[n] += qy; bz[n] += qz; }
+            }
+    }
+    if (ok && lane == 0)
+    {
+        b1[3 * size_t(c)] = bx[0]; b1[3 * size_t(c) + 1] = by[0]; b1[3 * size_t(c) + 2] = bz[0];
+        b2[3 * size_t(c)] = bx[1]; b2[3 * size_t(c) + 1] = by[1]; b2[3 * size_t(c) + 2] = bz[1];
+    }
+}
+template <int G>
+__global__ void __launch_bounds__(256) k_hall_g2_g(DevMesh d, const double *b1, const double *b2, const double *alpha, const double *B0,
+                                                   double *g1, double *g2)
+{
+    const int32_t t = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    const int32_t c = t / G, lane = t % G;
+    const bool ok = c < d.nc;
+    const double a = ok ? alpha[c] : 0.0;
+    V3 out[2];
+    if (ok)
+    {
+        V3 sum[2];
+        const V3 bc[2] = {L3(b1, c), L3(b2, c)};
+        const int32_t k0 = d.cdeg > 0 ? c * d.cdeg : d.c2f_ptr[c], k1 = d.cdeg > 0 ? k0 + d.cdeg : d.c2f_ptr[c + 1];
+        for (int32_t base = k0; base < k1; base += G)
+        {
+            const int32_t k = base + lane;
+            V3 t3[2];
+            if (k < k1)
+            {
+                const int32_t v = d.c2f[k], f = pid(v), o = d.c2n[k];
+                const V3 A = psg(v) * L3(d.farea, f);
+                V3 bf[2] = {bc[0], bc[1]};
+                if (o >= 0)
+                {
+                    bf[0] = 0.5 * (bc[0] + L3(b1, o));
+                    bf[1] = 0.5 * (bc[1] + L3(b2, o));
+                }
+                t3[0] = cross(A, bf[0]);
+                t3[1] = cross(A, bf[1]);
+            }
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+#pragma unroll
+                for (int n = 0; n < 2; ++n)
+                {
+                    const V3 tq(__shfl(t3[n].x, q, G), __shfl(t3[n].y, q, G), __shfl(t3[n].z, q, G));
+                    if (base + q < k1) sum[n] += tq;
+                }
+        }
+        if (a != 0.0)
+        {
+            const V3 B = L3(B0, c);
+            const double iv = 1.0 / d.cvol[c];
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
+            {
+                const V3 J = iv * sum[n];
+                out[n] = a * cross(J, B);
+            }
+        }
+    }
+    if (ok && lane == 0) { st3(g1, c, out[0]); st3(g2, c, out[1]); }
+}
+template __global__ void k_bcell2_g<kCellLanes64>(DevMesh, const double *, const double *, double *, double *);
+template __global__ void k_hall_g2_g<kCellLanes64>(DevMesh, const double *, const double *, const double *, const double *, double *,
+                                                   double *);
+
+// The edge sums of F for (phi1, g1) and (phi2, g2), then b = E_expl + F1 and r0 = dir ? 0 : b - (x + dt F2)
+// (k_edge_apply's mode 0 and mode 1 with bvec, no implicit ideal part)
+__device__ __forceinline__ void rhs_residual_out(const DevMesh &d, int32_t e, double P0a, double P1a, double P0b, double P1b,
+                                                 const double *E_expl, const double *x, const int8_t *dir, double dt, double *bvec,
+                                                 double *r)
+{
+    const double p2 = 0.0;
+    const double F1 = d.m_e[e] * P0a + P1a + p2, F2 = d.m_e[e] * P0b + P1b + p2;
+    const double bv = E_expl[e] + F1;
+    double rr = x[e] + dt * F2;
+    rr = dir[e] ? 0.0 : bv - rr;
+    bvec[e] = bv;
+    r[e] = rr;
+}
+__global__ void __launch_bounds__(256) k_edge_rhs_residual(DevMesh d, int32_t resistive, int32_t hall, const double *phi1, const double *g1,
+                                                           const double *phi2, const double *g2, const double *E_expl, const double *x,
+                                                           const int8_t *dir, double dt, double *bvec, double *r)
+{
+    const int32_t e = gid();
+    if (e >= d.ne) return;
+    const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
+    if (f1 - f0 > kLightMax) return; // k_edge_rhs_residual_heavy
+    double p0[2] = {0.0, 0.0}, p1[2] = {0.0, 0.0};
+    if (resistive)
+        for (int32_t k = f0; k < f1; k += kEdgeUnroll)
+        {
+            int32_t q[kEdgeUnroll];
+            double st[kEdgeUnroll], xa[kEdgeUnroll], xb[kEdgeUnroll];
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < f1) q[u] = d.e2f[k + u];
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < f1) { const int32_t f = pid(q[u]); st[u] = d.star2[f]; xa[u] = phi1[f]; xb[u] = phi2[f]; }
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < f1)
+                {
+                    const double va = st[u] * xa[u], vb = st[u] * xb[u];
+                    p0[0] += psg(q[u]) * va;
+                    p0[1] += psg(q[u]) * vb;
+                }
+        }
+    if (hall)
+    {
+        const V3 t = L3(d.evec, e);
+        const int32_t c0 = d.ec_ptr[e], c1 = d.ec_ptr[e + 1];
+        for (int32_t k = c0; k < c1; k += kEdgeUnroll)
+        {
+            int32_t q[kEdgeUnroll];
+            double w[kEdgeUnroll];
+            V3 xa[kEdgeUnroll], xb[kEdgeUnroll];
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < c1) { q[u] = d.ec_cell[k + u]; w[u] = d.ecw[k + u]; }
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < c1) { xa[u] = L3(g1, q[u]); xb[u] = L3(g2, q[u]); }
+#pragma unroll
+            for (int u = 0; u < kEdgeUnroll; ++u)
+                if (k + u < c1)
+                {
+                    const V3 wt = w[u] * t;
+                    p1[0] += dot(xa[u], wt);
+                    p1[1] += dot(xb[u], wt);
+                }
+        }
+    }
+    rhs_residual_out(d, e, p0[0], p1[0], p0[1], p1[1], E_expl, x, dir, dt, bvec, r);
+}
+// the high-valence edges, one block per edge (terms in parallel into LDS, sums sequential, as k_edge_apply_heavy)
+__global__ void __launch_bounds__(256) k_edge_rhs_residual_heavy(DevMesh d, int32_t resistive, int32_t hall, const int32_t *heavy,
+                                                                 const double *phi1, const double *g1, const double *phi2, const double *g2,
+                                                                 const double *E_expl, const double *x, const int8_t *dir, double dt,
+                                                                 double *bvec, double *r)
+{
+    __shared__ double t0[2][kHeavyMax], t1[2][2 * kHeavyMax];
+    const int32_t e = heavy[blockIdx.x];
+    const int32_t f0 = d.e2f_ptr[e], nf = d.e2f_ptr[e + 1] - f0;
+    const int32_t c0 = d.ec_ptr[e], ncl = d.ec_ptr[e + 1] - c0;
+    if (resistive)
+        for (int32_t i = threadIdx.x; i < nf; i += blockDim.x)
+        {
+            const int32_t q = d.e2f[f0 + i], f = pid(q);
+            const double va = d.star2[f] * phi1[f], vb = d.star2[f] * phi2[f];
+            t0[0][i] = psg(q) * va;
+            t0[1][i] = psg(q) * vb;
+        }
+    if (hall)
+    {
+        const V3 t = L3(d.evec, e);
+        for (int32_t i = threadIdx.x; i < ncl; i += blockDim.x)
+        {
+            const V3 wt = d.ecw[c0 + i] * t;
+            const int32_t c = d.ec_cell[c0 + i];
+            t1[0][i] = dot(L3(g1, c), wt);
+            t1[1][i] = dot(L3(g2, c), wt);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double p0[2] = {0.0, 0.0}, p1[2] = {0.0, 0.0};
+    for (int n = 0; n < 2; ++n)
+    {
+        if (resistive) for (int32_t i = 0; i < nf; ++i) p0[n] += t0[n][i];
+        if (hall) for (int32_t i = 0; i < ncl; ++i) p1[n] += t1[n][i];
+    }
+    rhs_residual_out(d, e, p0[0], p1[0], p0[1], p1[1], E_expl, x, dir, dt, bvec, r);
+}
+
 // delta += dir ? 0 : (sum_i y_i V_i) * invdiag  (ImplicitInduction.cpp:366-376)
 __global__ void __launch_bounds__(256) k_gmres_update(const double *const *V, const double *y, int32_t j, const double *invdiag,
                                                       const int8_t *dir, int32_t n, double *delta)

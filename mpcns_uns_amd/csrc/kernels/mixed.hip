@@ -829,8 +829,8 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
         mg_set_solve(dir, static_cast<float>(dt));
         mg_prepare(dt);
     }
-    apply_op(Phi_n, nullptr, nullptr, nullptr, 0.0, 0, 0, nullptr, tmp);
-    LAUNCH("axpy", k_add, dim3(nb(ne)), dim3(256), 0, stream, Eex, tmp, ne, bvec);
+    // b = E_expl + F(Phi^n) and the first cycle's residual b - A x0, in one pass
+    rhs_residual(Phi_n, Eex, x0, dir, dt);
     chk(hipMemsetAsync(delta, 0, size_t(ne) * 8, stream), "memset");
     const double bb = dot(bvec, bvec, 0, 0);
     const double bnorm = std::sqrt(std::max(bb, 1e-300));
@@ -846,9 +846,12 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
     double est = 0;
     for (int outer = 0; outer < 200; ++outer)
     {
-        // FP64 residual of x0 + delta
-        LAUNCH("axpy", k_add, dim3(nb(ne)), dim3(256), 0, stream, x0, delta, ne, tmp);
-        apply_op(nullptr, tmp, nullptr, dir, dt, 1, 0, bvec, rvec);
+        // FP64 residual of x0 + delta (delta = 0 in the first cycle: rhs_residual formed it)
+        if (outer > 0)
+        {
+            LAUNCH("axpy", k_add, dim3(nb(ne)), dim3(256), 0, stream, x0, delta, ne, tmp);
+            apply_op(nullptr, tmp, nullptr, dir, dt, 1, 0, bvec, rvec);
+        }
         beta = dot(rvec, rvec, 1, 1);
         if (trace)
             std::fprintf(stderr, "[mixed] cycle %d: residual %.3e |b|, previous cycle %d iterations, inner estimate %.3e |b|\n", outer,

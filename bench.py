@@ -30,6 +30,20 @@ PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 PMC_TRAFFIC = {("shell_8m_mixed", "lat", "edge_apply_f32"): 1369223077}
 
 
+def step_compulsory_bytes(its, restart, ns, mixed, vcycle=False):
+    """Compulsory HBM bytes per cell-step of the Hall step (every input read once, every output written once; tet
+    ratios F = 2C, E = 7C/6): the explicit stage 80 ns + 437 B; a matrix-free apply_A 329 B in FP64, 190 B in FP32;
+    Arnoldi step j (2j + 5) 9.33 B in FP64, half in FP32; the Jacobi product 28 B per iteration. The solve takes `its`
+    Krylov iterations in cycles of at most `restart` and three kinds of FP64 product besides: b, the first residual
+    and one residual per cycle. With the multigrid preconditioner each iteration adds one V-cycle, about 1.3 kB (six
+    FP32 operator products and the transfers over a hierarchy of ratio 8)."""
+    its = int(round(its))
+    cycles = [restart] * (its // restart) + ([its % restart] if its % restart else [])
+    arnoldi = sum((2 * j + 5) * 9.33 for n in cycles for j in range(n)) * (0.5 if mixed else 1.0)
+    krylov = its * (190.0 if mixed else 329.0)
+    return 80.0 * ns + 437.0 + krylov + (2 + len(cycles)) * 329.0 + arnoldi + 28.0 * its + (1300.0 * its if vcycle else 0.0)
+
+
 def kernel_bytes(mesh, name, precision_bytes=8, ns=2):
     """Compulsory HBM bytes of one launch of a kernel: every array it reads or writes counted once (ns ion species)."""
     C, F, E = mesh.n_cells, mesh.n_faces, mesh.n_edges
@@ -366,6 +380,13 @@ def main():
             roof["stream_copy_GBps"] = stream["copy_GBps"]
             roof["frac_of_stream_copy"] = round(achieved / 1e9 / stream["copy_GBps"], 4)
 
+    # the whole step against the compulsory-byte model of the path (no intermediates, no index overhead)
+    step_comp = None
+    if w["spec"]["hall"]:
+        bpc = step_compulsory_bytes(np.mean(its), int(cfg.ind_restart), ns, precision in (1, 2), precision == 2)
+        step_comp = dict(bytes_per_cell_step=round(bpc, 1), achieved_GBps=round(bpc * value / 1e9, 1),
+                         frac=round(bpc * value / PEAK_HBM, 4))
+
     # the run's own consecutive steps after the warmup (no restore): dt collapses as the polar field grows
     mt = measure(precision, False, restore=False)
     dtt = mt["dts"]
@@ -410,6 +431,7 @@ def main():
         "field_error_vs_fp64": field_err,
         "preconditioners": precond or None,
         "roofline": roof,
+        "step_compulsory": step_comp,
         "stream": stream,
         "kernels": kernels,  # per-kernel totals over the warmup steps
         "kernels_over_steps": nbd,

@@ -21,7 +21,7 @@ struct DevMesh32
 };
 
 constexpr int kDotChunk = 16; // basis vectors per multi-dot pass
-constexpr int kOrthoBlocks = 1024; // grid of the FP32 orthogonalisation kernels (measured best of 512..4096)
+constexpr int kOrthoBlocks = 1024; // grid of the FP32 orthogonalisation kernels (512 and 2048 measured no better)
 
 __global__ void k_d1_f32(DevMesh d, DevMesh32 s, const float *pv, const float *dte, float *phi);
 constexpr int kCellLanes = 4; // lanes per cell of the FP32 cell kernels

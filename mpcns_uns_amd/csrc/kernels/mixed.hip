@@ -789,7 +789,8 @@ void mhdg_ctx::mixed_build()
     M.Vptrs = static_cast<float **>(alloc(sizeof(float *) * M.V.size()));
     chk(hipMemcpy(M.Vptrs, M.V.data(), sizeof(float *) * M.V.size(), hipMemcpyHostToDevice), "upload");
     M.h = static_cast<double *>(alloc(64 * sizeof(double)));
-    M.part = static_cast<double *>(alloc(size_t(kOrthoBlocks) * (kDotChunk + 1) * sizeof(double)));
+    // partial rows of the multi-dot: kOrthoBlocks in the main pass, kRedBlocks in the DGKS re-projection
+    M.part = static_cast<double *>(alloc(size_t(std::max(kOrthoBlocks, kRedBlocks)) * (kDotChunk + 1) * sizeof(double)));
     M.part_axpy = static_cast<double *>(alloc(size_t(kOrthoBlocks) * sizeof(double)));
     chk(hipStreamSynchronize(stream), "sync");
     M.built = true;

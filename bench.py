@@ -420,7 +420,7 @@ def main():
                    "preconditioner": PRECONDITIONER[precision], "hall": bool(w["spec"]["hall"]), "ion_species": ns,
                    "wall_dirichlet": int(cfg.wall_dirichlet), "gmres_restart": int(cfg.ind_restart), "rtol": float(cfg.ind_rtol),
                    "gmres_iterations_per_step": float(np.mean(its)), "max_rel_residual": float(max(res)),
-                   "setup_s": round(setup_s, 1)},
+                   "setup_s": round(setup_s, 1), "field_error_vs_fp64": field_err},
         "protocol": "every timed step is the first step from the seeded state (device-side restore, cold solve)",
         "ms_per_step_by_class": split_by_class(breakdown, nbd),
         "trajectory": trajectory,

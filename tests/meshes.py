@@ -82,3 +82,23 @@ def d2_of(mesh, Phi):
     """(d2 Phi)_c = sum over the cell's faces of the signed flux, by integer incidence (c2f)."""
     ptr, idx, sgn = mesh.array("c2f.ptr"), mesh.array("c2f.idx"), mesh.array("c2f.sgn")
     return np.add.reduceat(sgn * Phi[idx], ptr[:-1])
+
+
+def hex_workload(hall, nx=6, ny=5, nz=4, jitter=0.1, seed=11, noise=0.01, **cfg_kw):
+    """A Hall (or ideal) workload on a jittered hexahedral box away from the dipole (every boundary face outflow):
+    cells of six quadrilateral faces, so the device runs its row code for rows other than the tetrahedron's four faces
+    and three edges. Returns the workload dict of mpcns_uns_amd.workloads.build plus the cells and tags."""
+    from mpcns_uns_amd import Mesh, workloads
+
+    lo, hi = (1.5, -1.0, -1.0), (3.5, 1.0, 1.0)
+    xyz, cells, tags = box_cells(nx, ny, nz, lo, hi)
+    rng = np.random.default_rng(seed)
+    h = np.array([(hi[0] - lo[0]) / nx, (hi[1] - lo[1]) / ny, (hi[2] - lo[2]) / nz])
+    inner = np.all((xyz > np.array(lo) + 1e-9) & (xyz < np.array(hi) - 1e-9), axis=1)
+    xyz = xyz + inner[:, None] * rng.uniform(-jitter, jitter, xyz.shape) * h
+    mesh = Mesh.from_cells(xyz, cells, tags=tags)
+    mesh.boundary({}, default="outflow", u_inf=workloads.U_INF)
+    cfg = workloads.physics_config(hall, **cfg_kw)
+    PhiAdd, Badd = mesh.background(workloads.B_IMF, workloads.DIPOLE_M, (0.0, 0.0, 0.0), 1e-3)
+    UH, UNa, Phi = mesh.seeded_state(cfg, noise, seed, Badd)
+    return dict(mesh=mesh, cfg=cfg, state=(UH, UNa, Phi), background=(PhiAdd, Badd), cells=cells, tags=tags, xyz=xyz)

@@ -205,6 +205,45 @@ def test_step_fp64_classical_gram_schmidt(small_hall):
     assert rel < 1e-12
 
 
+def test_hex_box_steps():
+    """The device on a jittered hexahedral box (six faces per cell, four edges per face: the row code other than the
+    tetrahedron's) against the oracle, which matches the compiled reference bit for bit there: explicit steps bitwise,
+    the FP64 Hall step to 1e-12, the mixed and FP64-CGS steps within 1e-6 (every solve stops at a 1e-8 relative residual;
+    with hall_coef 50 the system's conditioning puts the state difference of two such solves at a few 1e-8)."""
+    from mpcns_uns_amd import Device
+    from meshes import hex_workload
+    from oracle.oracle import Oracle
+
+    we = hex_workload(False, nx=8, ny=6, nz=6)
+    dev, orc = Device(we["mesh"], we["cfg"]), Oracle(we["mesh"], we["cfg"])
+    for x in (dev, orc):
+        x.set_background(*we["background"])
+        x.set_state(*we["state"])
+    for _ in range(3):
+        assert dev.step() == orc.step()
+        for a, b in zip(dev.get_state(), orc.get_state()):
+            assert np.array_equal(a, b)
+    w = hex_workload(True, nx=8, ny=6, nz=6, hall_coef=50.0)
+    orc = Oracle(w["mesh"], w["cfg"])
+    devs = {p: Device(w["mesh"], w["cfg"]) for p in (0, 1, 3)}
+    for x in (orc, *devs.values()):
+        x.set_background(*w["background"])
+        x.set_state(*w["state"])
+    for k in range(3):
+        dt_o, it_o, r_o = orc.step()
+        so = orc.get_state()
+        for p, dev in devs.items():
+            dt_d, it_d, r_d = dev.step(precision=p)
+            print(f"[hex box] step {k + 1} precision {p}: GMRES {it_d}/{it_o}, residual {r_d:.2e}", flush=True)
+            # the states of the mixed and CGS paths differ at the solver tolerance, and the next CFL step with them
+            tol = 1e-12 if p == 0 else 1e-6
+            assert abs(dt_d - dt_o) <= tol * dt_o and r_d <= 1e-8 and abs(it_d - it_o) <= (2 if p == 1 else 1)
+            for a, b in zip(dev.get_state(), so):
+                assert maxrel(a, b) < tol
+            _, rel, _ = dev.div_b(dev.get_state()[2], w["background"][0])
+            assert rel < 1e-12
+
+
 def test_implicit_ideal_and_edge_dt():
     """set_ideal_state / ideal_linear (bitwise: the same face and edge EMF kernels as the explicit stage) and the
     per-edge pseudo-time solve, FP64 and mixed."""

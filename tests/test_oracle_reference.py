@@ -7,7 +7,7 @@ tests/test_oracle_known_answers.py, which needs no reference).
 import numpy as np
 import pytest
 
-from meshes import box_mesh
+from meshes import box_mesh, hex_workload
 
 pytestmark = pytest.mark.skipif(not __import__("oracle.oracle", fromlist=["have_ref"]).have_ref(),
                                 reason="reference library oracle/_ref not built")
@@ -60,6 +60,37 @@ def test_hex_box_complex_and_geometry_bitwise():
         assert np.array_equal(s, m.array(tb + ".sgn"))
     for g in GEOMETRY:
         assert np.array_equal(ref.geometry(g), m.array(g)), g
+
+
+@pytest.mark.parametrize("hall", [False, True])
+def test_hex_box_full_steps_bitwise(hall):
+    """The reference's step_first_order and the oracle on a jittered hexahedral box (six quadrilateral faces per
+    cell, non-planar faces inside), Hall solve with 11 GMRES iterations per step: background, boundary kinds and
+    three steps bit for bit."""
+    from oracle.oracle import Oracle, Reference
+    from mpcns_uns_amd import workloads
+
+    w = hex_workload(hall, nx=8, ny=6, nz=6, wall_dirichlet=0, hall_coef=50.0)
+    m = w["mesh"]
+    ref = Reference(w["xyz"], w["cells"], w["cfg"], tags=w["tags"], tagset=1, dipole_m=workloads.DIPOLE_M, dipole_c=(0.0, 0.0, 0.0),
+                    soft=1e-3)
+    PA, BA = ref.background()
+    assert np.array_equal(PA, w["background"][0]) and np.array_equal(BA, w["background"][1])
+    fk, em = ref.boundary()
+    assert np.array_equal(fk, m.array("face_kind")) and np.array_equal(em, m.array("edge_mode"))
+    orc = Oracle(m, w["cfg"])
+    orc.set_background(*w["background"])
+    orc.set_state(*w["state"])
+    ref.solver_init()
+    ref.set_state(*w["state"])
+    for _ in range(3):
+        dt, it, res = orc.step()
+        assert ref.step() == dt
+        assert not hall or it >= 10
+        for a, b in zip(ref.get_state(), orc.get_state()):
+            assert np.array_equal(a, b)
+        for n in ("W_H", "W_Na", "B_ind", "B_tot", "J"):
+            assert np.array_equal(ref.derived(n), orc.derived(n))
 
 
 def test_operations_bitwise(shell):

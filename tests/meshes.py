@@ -102,3 +102,41 @@ def hex_workload(hall, nx=6, ny=5, nz=4, jitter=0.1, seed=11, noise=0.01, **cfg_
     PhiAdd, Badd = mesh.background(workloads.B_IMF, workloads.DIPOLE_M, (0.0, 0.0, 0.0), 1e-3)
     UH, UNa, Phi = mesh.seeded_state(cfg, noise, seed, Badd)
     return dict(mesh=mesh, cfg=cfg, state=(UH, UNa, Phi), background=(PhiAdd, Badd), cells=cells, tags=tags, xyz=xyz)
+
+
+def pyramid_split(xyz, cells, every=2):
+    """Split every `every`-th hexahedron into six pyramids about its centroid (a new node), each on one of the hex's
+    outward face loops with side triangles (b, a, apex) for base edge a -> b, so the mesh stays conforming: cells of
+    six and of five faces, faces of four and of three edges."""
+    xyz = [np.asarray(p, float) for p in xyz]
+    out = []
+    for i, loops in enumerate(cells):
+        if i % every:
+            out.append(loops)
+            continue
+        corners = sorted({v for lp in loops for v in lp})
+        apex = len(xyz)
+        xyz.append(np.mean([xyz[v] for v in corners], axis=0))
+        for lp in loops:
+            sides = [[lp[(k + 1) % 4], lp[k], apex] for k in range(4)]
+            out.append([list(lp)] + sides)
+    return np.array(xyz), out
+
+
+def mixed_workload(hall, nx=6, ny=5, nz=4, jitter=0.1, seed=13, noise=0.01, **cfg_kw):
+    """hex_workload's box with every other hexahedron split into six pyramids: rows of varying length everywhere."""
+    from mpcns_uns_amd import Mesh, workloads
+
+    lo, hi = (1.5, -1.0, -1.0), (3.5, 1.0, 1.0)
+    xyz, cells, tags = box_cells(nx, ny, nz, lo, hi)
+    rng = np.random.default_rng(seed)
+    h = np.array([(hi[0] - lo[0]) / nx, (hi[1] - lo[1]) / ny, (hi[2] - lo[2]) / nz])
+    inner = np.all((xyz > np.array(lo) + 1e-9) & (xyz < np.array(hi) - 1e-9), axis=1)
+    xyz = xyz + inner[:, None] * rng.uniform(-jitter, jitter, xyz.shape) * h
+    xyz, cells = pyramid_split(xyz, cells)
+    mesh = Mesh.from_cells(xyz, cells, tags=tags)
+    mesh.boundary({}, default="outflow", u_inf=workloads.U_INF)
+    cfg = workloads.physics_config(hall, **cfg_kw)
+    PhiAdd, Badd = mesh.background(workloads.B_IMF, workloads.DIPOLE_M, (0.0, 0.0, 0.0), 1e-3)
+    UH, UNa, Phi = mesh.seeded_state(cfg, noise, seed, Badd)
+    return dict(mesh=mesh, cfg=cfg, state=(UH, UNa, Phi), background=(PhiAdd, Badd), cells=cells, tags=tags, xyz=xyz)

@@ -205,16 +205,20 @@ def test_step_fp64_classical_gram_schmidt(small_hall):
     assert rel < 1e-12
 
 
-def test_hex_box_steps():
-    """The device on a jittered hexahedral box (six faces per cell, four edges per face: the row code other than the
-    tetrahedron's) against the oracle, which matches the compiled reference bit for bit there: explicit steps bitwise,
-    the FP64 Hall step to 1e-12, the mixed and FP64-CGS steps within 1e-6 (every solve stops at a 1e-8 relative residual;
-    with hall_coef 50 the system's conditioning puts the state difference of two such solves at a few 1e-8)."""
+@pytest.mark.parametrize("cells", ["hex", "hex_pyramid"])
+def test_box_steps(cells):
+    """The device on a jittered hexahedral box (six faces per cell, four edges per face: fixed row lengths other than
+    the tetrahedron's) and on the same box with every other hexahedron split into six pyramids (rows of varying length,
+    the kernels' row-pointer code; 31-33 GMRES iterations per step, across a restart) against the oracle, which matches
+    the compiled reference bit for bit on both: explicit steps bitwise, the FP64 Hall step to 1e-12, the mixed and
+    FP64-CGS steps within 1e-6 (every solve stops at a 1e-8 relative residual; with hall_coef 50 the system's
+    conditioning puts the state difference of two such solves at a few 1e-8)."""
     from mpcns_uns_amd import Device
-    from meshes import hex_workload
+    from meshes import hex_workload, mixed_workload
     from oracle.oracle import Oracle
 
-    we = hex_workload(False, nx=8, ny=6, nz=6)
+    make = hex_workload if cells == "hex" else mixed_workload
+    we = make(False, nx=8, ny=6, nz=6)
     dev, orc = Device(we["mesh"], we["cfg"]), Oracle(we["mesh"], we["cfg"])
     for x in (dev, orc):
         x.set_background(*we["background"])
@@ -223,7 +227,7 @@ def test_hex_box_steps():
         assert dev.step() == orc.step()
         for a, b in zip(dev.get_state(), orc.get_state()):
             assert np.array_equal(a, b)
-    w = hex_workload(True, nx=8, ny=6, nz=6, hall_coef=50.0)
+    w = make(True, nx=8, ny=6, nz=6, hall_coef=50.0)
     orc = Oracle(w["mesh"], w["cfg"])
     devs = {p: Device(w["mesh"], w["cfg"]) for p in (0, 1, 3)}
     for x in (orc, *devs.values()):
@@ -234,7 +238,7 @@ def test_hex_box_steps():
         so = orc.get_state()
         for p, dev in devs.items():
             dt_d, it_d, r_d = dev.step(precision=p)
-            print(f"[hex box] step {k + 1} precision {p}: GMRES {it_d}/{it_o}, residual {r_d:.2e}", flush=True)
+            print(f"[{cells} box] step {k + 1} precision {p}: GMRES {it_d}/{it_o}, residual {r_d:.2e}", flush=True)
             # the states of the mixed and CGS paths differ at the solver tolerance, and the next CFL step with them
             tol = 1e-12 if p == 0 else 1e-6
             assert abs(dt_d - dt_o) <= tol * dt_o and r_d <= 1e-8 and abs(it_d - it_o) <= (2 if p == 1 else 1)

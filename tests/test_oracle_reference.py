@@ -7,7 +7,7 @@ tests/test_oracle_known_answers.py, which needs no reference).
 import numpy as np
 import pytest
 
-from meshes import box_mesh, hex_workload
+from meshes import box_mesh, hex_workload, mixed_workload
 
 pytestmark = pytest.mark.skipif(not __import__("oracle.oracle", fromlist=["have_ref"]).have_ref(),
                                 reason="reference library oracle/_ref not built")
@@ -63,15 +63,19 @@ def test_hex_box_complex_and_geometry_bitwise():
 
 
 @pytest.mark.parametrize("hall", [False, True])
-def test_hex_box_full_steps_bitwise(hall):
+@pytest.mark.parametrize("cells", ["hex", "hex_pyramid"])
+def test_box_full_steps_bitwise(hall, cells):
     """The reference's step_first_order and the oracle on a jittered hexahedral box (six quadrilateral faces per
-    cell, non-planar faces inside), Hall solve with 11 GMRES iterations per step: background, boundary kinds and
-    three steps bit for bit."""
+    cell, non-planar faces inside), and on the same box with every other hexahedron split into six pyramids (rows of
+    five and six faces, of three and four edges); Hall solve with 11 and 31-33 GMRES iterations per step (the latter
+    across a GMRES(30) restart): background, boundary kinds and three steps bit for bit."""
     from oracle.oracle import Oracle, Reference
     from mpcns_uns_amd import workloads
 
-    w = hex_workload(hall, nx=8, ny=6, nz=6, wall_dirichlet=0, hall_coef=50.0)
+    make = hex_workload if cells == "hex" else mixed_workload
+    w = make(hall, nx=8, ny=6, nz=6, wall_dirichlet=0, hall_coef=50.0)
     m = w["mesh"]
+    assert Reference(w["xyz"], w["cells"], w["cfg"], tags=w["tags"], tagset=1).validate()["ok"]
     ref = Reference(w["xyz"], w["cells"], w["cfg"], tags=w["tags"], tagset=1, dipole_m=workloads.DIPOLE_M, dipole_c=(0.0, 0.0, 0.0),
                     soft=1e-3)
     PA, BA = ref.background()

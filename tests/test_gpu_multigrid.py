@@ -17,13 +17,19 @@ def _frozen(m, seed=31):
     return B0, ne_eff
 
 
+@pytest.mark.parametrize("cells", ["shell", "hex_pyramid"])
 @pytest.mark.parametrize("fine_hall", [1, 0])
-def test_vcycle_matches_stage_matrices(fine_hall):
+def test_vcycle_matches_stage_matrices(fine_hall, cells):
+    """On a small tetrahedral shell and on a box of hexahedra and pyramids (cells of five and six faces)."""
+    from meshes import mixed_workload
     from mpcns_uns_amd import Device, workloads
     from oracle.multigrid import operator_F, vcycle
     from oracle.oracle import Oracle
 
-    w = workloads.build(net=(6, 12, 3), hall=True, mg_fine_hall=fine_hall)
+    if cells == "shell":
+        w = workloads.build(net=(6, 12, 3), hall=True, mg_fine_hall=fine_hall)
+    else:
+        w = mixed_workload(True, nx=8, ny=6, nz=6, mg_fine_hall=fine_hall)
     m, cfg = w["mesh"], w["cfg"]
     B0, ne_eff = _frozen(m)
     dirichlet = m.array("edge_mode") != 0

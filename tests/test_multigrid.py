@@ -6,13 +6,18 @@ import numpy as np
 import pytest
 
 
-@pytest.fixture(scope="module")
-def shell():
+@pytest.fixture(scope="module", params=["shell", "hex_pyramid"])
+def shell(request):
+    """A small tetrahedral shell, or a box of hexahedra and pyramids (cells of five and six faces)."""
+    from meshes import mixed_workload
     from mpcns_uns_amd import workloads
     from oracle.multigrid import operator_F
     from oracle.oracle import Oracle
 
-    w = workloads.build(net=(6, 12, 3), hall=True)
+    if request.param == "shell":
+        w = workloads.build(net=(6, 12, 3), hall=True)
+    else:
+        w = mixed_workload(True, nx=8, ny=6, nz=6)
     m = w["mesh"]
     res = type(w["cfg"]).from_buffer_copy(w["cfg"])
     res.hall = 0  # the resistive part only: what the coarse levels carry

@@ -58,18 +58,30 @@ def _param_gradient(w, p, g):
 
 @pytest.mark.parametrize("second_order", [1, 0])
 def test_dot_product_with_the_tangent(problem, second_order):
-    w = problem
+    _check_dot_product(problem, second_order, R0, R1)
+
+
+@pytest.mark.parametrize("second_order", [1, 0])
+def test_dot_product_on_a_hexahedron_pyramid_box(second_order):
+    """The same identity on a box of hexahedra and pyramids (cells of five and six faces, faces of three and four
+    edges; every boundary face outflow), J over 1.5 <= r <= 3 of the box, which lies in 1.5 <= r <= 3.9."""
+    from meshes import mixed_workload
+
+    _check_dot_product(mixed_workload(False, nx=8, ny=6, nz=6), second_order, 1.5, 3.0)
+
+
+def _check_dot_product(w, second_order, r0, r1):
     m = w["mesh"]
     dev = _device(w, cfg_kw=dict(second_order=second_order))
     before = dev.get_state()
-    J, g = dev.step_vjp(STEPS, R0, R1)
+    J, g = dev.step_vjp(STEPS, r0, r1)
     for a, b in zip(before, dev.get_state()):
         assert np.array_equal(a, b)  # the resident state is left unchanged
     rng = np.random.default_rng(57)
     seeds = dict(U_H=rng.normal(size=5 * m.n_cells) * 1e-3, U_Na=rng.normal(size=5 * m.n_cells) * 1e-6,
                  Phi=rng.normal(size=m.n_faces) * 1e-3, PhiAdd=rng.normal(size=m.n_faces) * 1e-3,
                  Badd=rng.normal(size=3 * m.n_cells) * 1e-3, fs=rng.normal(size=10) * 1e-3, E_inf=rng.normal(size=3) * 1e-3)
-    Jt, dJ = dev.explicit_tangent(STEPS, R0, R1, dU_H=seeds["U_H"], dU_Na=seeds["U_Na"], dPhi=seeds["Phi"], dPhiAdd=seeds["PhiAdd"],
+    Jt, dJ = dev.explicit_tangent(STEPS, r0, r1, dU_H=seeds["U_H"], dU_Na=seeds["U_Na"], dPhi=seeds["Phi"], dPhiAdd=seeds["PhiAdd"],
                                   dBadd=seeds["Badd"], dfs=seeds["fs"], dE_inf=seeds["E_inf"])
     assert J == Jt
     parts = {k: float(np.ravel(g[k]) @ np.ravel(seeds[k])) for k in seeds}
@@ -79,7 +91,7 @@ def test_dot_product_with_the_tangent(problem, second_order):
     # each input separately (a wrong transposed kernel shows up in its own term)
     for k in ("U_H", "Phi", "PhiAdd", "Badd", "E_inf"):
         kw = {"d" + k: seeds[k]}
-        _, dk = dev.explicit_tangent(STEPS, R0, R1, **kw)
+        _, dk = dev.explicit_tangent(STEPS, r0, r1, **kw)
         assert abs(parts[k] - dk) <= 1e-9 * (abs(dk) + 1e-12 * scale), (k, parts[k], dk)
 
 

@@ -40,3 +40,32 @@ def test_reference_solver_on_device(hall):
             assert dt_h == dt_o
             for a, b in zip(sh, so):
                 assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("hall", [False, True])
+def test_reference_solver_on_device_hexahedron_pyramid_box(hall):
+    """The same drop-in on a box of hexahedra and pyramids (rows of varying length; every boundary face outflow)."""
+    from meshes import mixed_workload
+    from mpcns_uns_amd import workloads
+    from oracle.oracle import Oracle, ReferenceOnDevice
+
+    w = mixed_workload(hall, nx=8, ny=6, nz=6, wall_dirichlet=0, hall_coef=50.0)
+    hyb = ReferenceOnDevice(w["xyz"], w["cells"], w["cfg"], tags=w["tags"], tagset=1, dipole_m=workloads.DIPOLE_M,
+                            dipole_c=(0.0, 0.0, 0.0), soft=1e-3)
+    hyb.solver_init()
+    hyb.set_state(*w["state"])
+    orc = Oracle(w["mesh"], w["cfg"])
+    orc.set_background(*w["background"])
+    orc.set_state(*w["state"])
+    for _ in range(3):
+        dt_h = hyb.step()
+        dt_o = orc.step()[0]
+        sh, so = hyb.get_state(), orc.get_state()
+        if hall:
+            assert abs(dt_h - dt_o) <= 1e-12 * dt_o
+            for a, b in zip(sh, so):
+                assert maxrel(a, b) < 1e-12
+        else:
+            assert dt_h == dt_o
+            for a, b in zip(sh, so):
+                assert np.array_equal(a, b)

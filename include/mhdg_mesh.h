@@ -90,6 +90,14 @@ int mhdg_mesh_seeded_state(const mhdg_mesh *m, const mhdg_config *cfg, double no
  * mesh: the stored incidences, numbering and orientation are used as they are, geometry is recomputed, and
  * the stored topology hash is verified.  A part file of a several-rank partition is refused. */
 int mhdg_mesh_read_unsbin(const char *path, mhdg_mesh **out);
+/* The n part files part_0 .. part_{n-1}.unsb of an n-rank partition (uns::io::write_local_mesh of each
+ * uns::Distributor::build(r), src/uns/parallel/Distributor.cpp:107-160, written by the reference's preprocessing
+ * pipeline, Preprocess/Pipeline.cpp:133-156) merged into the one global complex they were cut from, in its global
+ * numbering: every entity's row comes from the file of its owner (the first n_owned entities of each dimension), its
+ * column ids mapped through that file's global ids, signs as stored; each file's topology hash and every ghost copy are
+ * checked against the merge. Geometry is recomputed. No counterpart in the reference, whose ranks each read one part:
+ * it lets one GPU run the reference's partitioned preprocessing output. */
+int mhdg_mesh_read_unsbin_parts(const char *const *paths, int32_t n_parts, mhdg_mesh **out);
 int mhdg_mesh_write_unsbin(const mhdg_mesh *m, const char *path);
 /* uns::topology_hash (src/uns/complex/CellComplex.cpp:56-70). */
 int mhdg_mesh_topology_hash(const mhdg_mesh *m, uint64_t *hash);

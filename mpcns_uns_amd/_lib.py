@@ -99,6 +99,7 @@ def lib():
         L.mhdg_mesh_cell_vector.argtypes = [P, P, P]
         L.mhdg_mesh_seeded_state.argtypes = [P, C.POINTER(Config), D, C.c_uint64, P, P, P, P]
         L.mhdg_mesh_read_unsbin.argtypes = [C.c_char_p, C.POINTER(P)]
+        L.mhdg_mesh_read_unsbin_parts.argtypes = [C.POINTER(C.c_char_p), C.c_int32, C.POINTER(P)]
         L.mhdg_mesh_background_dipoles.argtypes = [P, P, I32, P, P, P, D, P, P]
         L.mhdg_dipole_edge_integrals.argtypes = [I32, P, P, P, D, I64, P, P, P]
         L.mhdg_mesh_write_unsbin.argtypes = [P, C.c_char_p]

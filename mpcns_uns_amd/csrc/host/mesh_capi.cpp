@@ -126,6 +126,24 @@ int mhdg_mesh_read_unsbin(const char *path, mhdg_mesh **out)
     }
     catch (const std::exception &e) { g_err = e.what(); return 1; }
 }
+int mhdg_mesh_read_unsbin_parts(const char *const *paths, int32_t n_parts, mhdg_mesh **out)
+{
+    try
+    {
+        if (!paths || n_parts < 1) throw std::runtime_error("mhdg_mesh_read_unsbin_parts: no part files");
+        std::vector<std::string> v;
+        for (int32_t k = 0; k < n_parts; ++k)
+        {
+            if (!paths[k]) throw std::runtime_error("mhdg_mesh_read_unsbin_parts: null path");
+            v.emplace_back(paths[k]);
+        }
+        auto m = std::make_unique<mhdg_mesh>();
+        mhdg::unsbin::read_partition(v, m->cx, m->xyz, m->tag_names);
+        *out = finish_mesh(m.release());
+        return 0;
+    }
+    catch (const std::exception &e) { g_err = e.what(); return 1; }
+}
 int mhdg_mesh_write_unsbin(const mhdg_mesh *m, const char *path)
 {
     try

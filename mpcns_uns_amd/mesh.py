@@ -99,6 +99,17 @@ class Mesh:
             raise RuntimeError(lib().mhdg_mesh_error().decode())
         return cls(h)
 
+    @classmethod
+    def read_unsbin_parts(cls, paths):
+        """The part files of an n-rank partition (the reference's preprocessing output) merged into the global
+        complex, in its global numbering; hashes and ghost copies checked, geometry recomputed."""
+        enc = [str(p).encode() for p in paths]
+        arr = (C.c_char_p * len(enc))(*enc)
+        h = C.c_void_p()
+        if lib().mhdg_mesh_read_unsbin_parts(arr, len(enc), C.byref(h)):
+            raise RuntimeError(lib().mhdg_mesh_error().decode())
+        return cls(h)
+
     def write_unsbin(self, path):
         """Complex file readable by the reference (uns::io::write_local_mesh layout, one rank)."""
         if lib().mhdg_mesh_write_unsbin(self._h, str(path).encode()):

@@ -143,3 +143,99 @@ def test_field_files_match_the_reference(tmp_path):
         assert L.ref_read_field(str(po).encode(), name.encode(), out.ctypes.data, C.byref(n), C.byref(d), C.byref(c), C.byref(hh)) == 0
         assert np.array_equal(out, v) and (d.value, c.value, hh.value) == (dim, nc, m.topology_hash)
         assert np.array_equal(got[name][2], v)
+
+
+def _records(raw):
+    """{name: (dtype, cols, rows, payload offset)} of an UNSBIN file's records."""
+    n, off, out = int(np.frombuffer(raw[8:16], np.uint64)[0]), 16, {}
+    size = {0: 1, 1: 4, 2: 8, 3: 4, 4: 8}
+    for _ in range(n):
+        ln = int(np.frombuffer(raw[off:off + 4], np.uint32)[0])
+        name = bytes(raw[off + 4:off + 4 + ln]).decode()
+        off += 4 + ln
+        dt, cols, rows = raw[off], int(np.frombuffer(raw[off + 4:off + 8], np.uint32)[0]), int(np.frombuffer(raw[off + 8:off + 16], np.uint64)[0])
+        off += 16
+        out[name] = (dt, cols, rows, off)
+        nbytes = rows * cols * size[dt]
+        off += nbytes + (8 - nbytes % 8) % 8
+    return out
+
+
+def _reference_parts(tmp_path, nranks):
+    """Part files of the shell workload written by the reference's own preprocessing steps (build_mesh,
+    partition_cells, Distributor::build, write_local_mesh) through oracle/_ref/ref_mpi_bench."""
+    import subprocess
+
+    from mpcns_uns_amd import workloads
+    from oracle.cpu_baseline import MPI_BENCH, write_mpi_input
+
+    if not os.path.exists(MPI_BENCH):
+        pytest.skip("oracle/_ref/ref_mpi_bench not built")
+    w = workloads.build(net=(8, 16, 3), hall=True)
+    d = tmp_path / f"np{nranks}"
+    d.mkdir()
+    write_mpi_input(str(d / "input.bin"), w, workloads.DIPOLE_M)
+    subprocess.run([MPI_BENCH, "parts", str(d), str(nranks)], check=True, capture_output=True, timeout=300)
+    return w["mesh"], [d / f"part_{k:05d}.unsb" for k in range(nranks)]
+
+
+@pytest.mark.parametrize("nranks", [1, 3, 4])
+def test_partition_merges_to_the_global_complex(tmp_path, nranks):
+    """The part files of an n-rank partition, merged through their global ids, give the complex the reference
+    partitioned: every table, sign, tag, coordinate and the recomputed geometry bit for bit."""
+    from mpcns_uns_amd import Mesh
+
+    m, paths = _reference_parts(tmp_path, nranks)
+    for order in (paths, paths[::-1]):
+        g = Mesh.read_unsbin_parts(order)
+        assert (g.n_nodes, g.n_edges, g.n_faces, g.n_cells) == (m.n_nodes, m.n_edges, m.n_faces, m.n_cells)
+        for a in ARRAYS + ("xyz",):
+            assert np.array_equal(g.array(a), m.array(a)), a
+        assert g.topology_hash == m.topology_hash and g.tag_names == ["", "inner", "outer"]
+    if nranks == 1:
+        assert Mesh.read_unsbin(paths[0]).topology_hash == m.topology_hash
+    else:
+        with pytest.raises(RuntimeError, match="partition"):
+            Mesh.read_unsbin(paths[0])
+
+
+def test_partition_refusals(tmp_path):
+    from mpcns_uns_amd import Mesh
+
+    _, paths = _reference_parts(tmp_path, 3)
+    with pytest.raises(RuntimeError, match="3-rank partition, 2 files given"):
+        Mesh.read_unsbin_parts(paths[:2])
+    with pytest.raises(RuntimeError, match="once each"):
+        Mesh.read_unsbin_parts([paths[0], paths[0], paths[2]])
+    with pytest.raises(RuntimeError, match="no part files"):
+        Mesh.read_unsbin_parts([])
+    raw = bytearray(paths[1].read_bytes())
+    rec = _records(raw)
+    meta = np.frombuffer(bytes(raw[rec["meta"][3]:rec["meta"][3] + 96]), np.int64)
+    n_nodes, owned_nodes = int(meta[3]), int(meta[7])
+    assert n_nodes > owned_nodes  # the part has ghost nodes
+    bad = tmp_path / "bad"
+    bad.mkdir()
+    # a ghost copy of a node moved: the owner's coordinates no longer match
+    x = bytearray(raw)
+    o = rec["x"][3] + 8 * 3 * (n_nodes - 1)
+    x[o:o + 8] = (np.frombuffer(bytes(raw[o:o + 8]), np.float64) + 1e-9).tobytes()
+    (bad / "x.unsb").write_bytes(x)
+    with pytest.raises(RuntimeError, match="node coordinates"):
+        Mesh.read_unsbin_parts([paths[0], bad / "x.unsb", paths[2]])
+    # an owned node given the global id of another part's owned node: two owners
+    g0 = _records(bytearray(paths[0].read_bytes()))["ent0.gid"][3]
+    other = int(np.frombuffer(paths[0].read_bytes()[g0:g0 + 8], np.int64)[0])
+    dup = bytearray(raw)
+    o = rec["ent0.gid"][3]
+    dup[o:o + 8] = np.int64(other).tobytes()
+    (bad / "dup.unsb").write_bytes(dup)
+    with pytest.raises(RuntimeError, match="owned once each"):
+        Mesh.read_unsbin_parts([paths[0], bad / "dup.unsb", paths[2]])
+    # a row of a part edited: the stored topology hash catches it
+    e = bytearray(raw)
+    o = rec["e2n.sgn"][3]
+    e[o] = (e[o] ^ 0xFE) & 0xFF
+    (bad / "sgn.unsb").write_bytes(e)
+    with pytest.raises(RuntimeError, match="hash"):
+        Mesh.read_unsbin_parts([paths[0], bad / "sgn.unsb", paths[2]])

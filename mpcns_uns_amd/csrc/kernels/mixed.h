@@ -15,11 +15,20 @@ struct DevMesh32
     float *aB0;               // per cell alpha_c B0_c of the frozen state (3)
     const float *aV;          // per (cell, face) entry: s A_f / V_c (3)
     const int32_t *nbr;       // per (cell, face) entry: the cell across the face, -1 on the boundary
-    const int2 *ecp;          // per edge-cell entry: (cell, bits of the FP32 weight), one 8-byte load
-    const int2 *e2fp;         // per edge-face entry: (face, bits of the FP32 signed star2 of the face)
+    // The edge kernel's rows in slices of 64 edges (one wavefront): entries 2p and 2p+1 of the row of edge 64 s + i as
+    // one 16-byte pair (index, bits of the FP32 weight, index, bits of the FP32 weight) at pair slot
+    // slice[s] + 64 p + i, so each pair load of a wavefront reads one contiguous kilobyte.  A slice holds as many
+    // pairs per edge as its longest row needs; slots past a row, and the rows of high-valence edges (the heavy
+    // blocks' edges), hold (0, +0), so the kernel gathers and adds them without a branch (a zero term leaves the
+    // value of every sum as it was).
+    const int4 *fpairs;       // edge-face rows: (face, signed star2 of the face)
+    const int4 *cpairs;       // edge-cell rows: (cell, edge-cell weight)
+    const int32_t *fslice, *cslice; // first pair slot of every slice (n_slices + 1 entries)
+    const int8_t *eheavy;     // 1 on the edges of valence above kLightMax (the edge kernel's heavy blocks)
     int32_t cdeg, fdeg;       // faces per cell / edges per face when uniform (tets: 4, 3), else 0 (ptr arrays)
 };
 
+constexpr int kEdgeBlock = 256; // threads per block of the FP32 edge kernel (a multiple of 64: whole slices)
 constexpr int kDotChunk = 16; // basis vectors per multi-dot pass
 constexpr int kOrthoBlocks = 1024; // grid of the FP32 orthogonalisation kernels (512 and 2048 measured no better)
 

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <array>
 #include <utility>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -19,7 +20,7 @@ namespace mhdg
 {
 namespace
 {
-constexpr int kPairUnroll = 4; // 16-byte loads (two row entries each) issued together in the FP32 edge kernel
+constexpr int kPairUnroll = 3; // 16-byte row pairs loaded together by the FP32 edge kernel (rows of up to six entries in one group)
 struct F3 { float x, y, z; };
 __device__ __forceinline__ F3 f3(const float *p, long i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 __device__ __forceinline__ F3 fcross(const F3 &a, const F3 &b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
@@ -33,6 +34,15 @@ __device__ __forceinline__ float4 ldg4(const float *p, int32_t q)
 {
     const F4v v = ((GlobalF4)p)[q];
     return make_float4(v.x, v.y, v.z, v.w);
+}
+typedef int I4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) I4v *GlobalI4;
+// a row pair of the edge kernel, one 16-byte load, kept (mask -1) or made a (0, +0) pad (mask 0) with no branch: a
+// conditional load, or a select on its result, became a branch around the load and a wait at its end
+__device__ __forceinline__ int4 ld_pair(const int4 *p, int32_t q, int32_t mask)
+{
+    const I4v v = ((GlobalI4)p)[q];
+    return make_int4(v.x & mask, v.y & mask, v.z & mask, v.w & mask);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -97,14 +107,25 @@ __global__ void __launch_bounds__(256) k_d1_f32(DevMesh d, DevMesh32 s, const fl
 //   kOpSmooth:  x + omega D^-1 (b - A x)          (a damped-Jacobi sweep of A x = b, D^-1 = invdiag)
 //   kOpResid:   b - A x
 // and 0 on Dirichlet edges in every mode.
-template <int MODE>
-__device__ __forceinline__ float edge_out(const DevMesh32 &s, int32_t e, float y, const float *v, const float *pv, const float *b, float omega)
+// The per-edge inputs of the output.  The FP32 edge kernel loads them after its row sums: loading them ahead of the
+// rows took four more VGPRs and measured no faster.
+struct EdgeIn
 {
-    if (MODE == kOpShifted) return (pv[e] - v[e]) + y;
-    const float ax = pv[e] + y;
+    float me, pve, ve, be; // m_e, pv_e, v_e (kOpShifted), b_e (kOpSmooth, kOpResid)
+};
+template <int MODE>
+__device__ __forceinline__ EdgeIn edge_in(const DevMesh32 &s, int32_t e, const float *v, const float *pv, const float *b)
+{
+    return {s.m_e[e], pv[e], MODE == kOpShifted ? v[e] : 0.0f, MODE == kOpSmooth || MODE == kOpResid ? b[e] : 0.0f};
+}
+template <int MODE>
+__device__ __forceinline__ float edge_out(const DevMesh32 &s, int32_t e, float y, const EdgeIn &x, float omega)
+{
+    if (MODE == kOpShifted) return (x.pve - x.ve) + y;
+    const float ax = x.pve + y;
     if (MODE == kOpApply) return ax;
-    if (MODE == kOpSmooth) return pv[e] + omega * s.invdiag[e] * (b[e] - ax);
-    return b[e] - ax;
+    if (MODE == kOpSmooth) return x.pve + omega * s.invdiag[e] * (x.be - ax);
+    return x.be - ax;
 }
 
 // One high-valence edge (the polar axes) per block: terms in parallel, tree sums (order-free in FP32)
@@ -113,10 +134,10 @@ __device__ __forceinline__ void edge_apply_f32_heavy(const DevMesh &d, const Dev
                                                      const float *phi, const float *g, const float *v, const float *pv, const int8_t *dir,
                                                      float dt, float *out, const float *b, float omega)
 {
-    __shared__ float sh[2][4];
+    __shared__ float sh[2][kEdgeBlock / 64];
     float p0 = 0.0f, p1 = 0.0f;
     if (resistive)
-        for (int32_t k = d.e2f_ptr[e] + threadIdx.x; k < d.e2f_ptr[e + 1]; k += 256)
+        for (int32_t k = d.e2f_ptr[e] + threadIdx.x; k < d.e2f_ptr[e + 1]; k += kEdgeBlock)
         {
             const int32_t q = d.e2f[k], f = pid(q);
             p0 += fsg(q) * (s.star2[f] * phi[f]);
@@ -124,26 +145,30 @@ __device__ __forceinline__ void edge_apply_f32_heavy(const DevMesh &d, const Dev
     if (hall)
     {
         const F3 t = f3(s.evec, e);
-        for (int32_t k = d.ec_ptr[e] + threadIdx.x; k < d.ec_ptr[e + 1]; k += 256) p1 += s.ecw[k] * fdot(f3(g, d.ec_cell[k]), t);
+        for (int32_t k = d.ec_ptr[e] + threadIdx.x; k < d.ec_ptr[e + 1]; k += kEdgeBlock) p1 += s.ecw[k] * fdot(f3(g, d.ec_cell[k]), t);
     }
     for (int o = 32; o > 0; o >>= 1) { p0 += __shfl_down(p0, o, 64); p1 += __shfl_down(p1, o, 64); }
     if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = p0; sh[1][threadIdx.x >> 6] = p1; }
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        const float P0 = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3], P1 = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
-        out[e] = dir[e] ? 0.0f : edge_out<MODE>(s, e, dt * (s.m_e[e] * P0 + P1), v, pv, b, omega);
+        float P0 = 0.0f, P1 = 0.0f;
+        for (int w = 0; w < kEdgeBlock / 64; ++w) { P0 += sh[0][w]; P1 += sh[1][w]; }
+        if (dir[e]) { out[e] = 0.0f; return; }
+        const EdgeIn x = edge_in<MODE>(s, e, v, pv, b);
+        out[e] = edge_out<MODE>(s, e, dt * (x.me * P0 + P1), x, omega);
     }
 }
 
 // out = dir ? 0 : (A P - I) v = (pv - v) + dt F(Phi') with F = m_e sum_f s star2_f Phi'_f + sum_c w_ec g_c . t_e
 // (the shifted operator: same Krylov space as A P, and the O(1) identity part never enters the FP32 vector)
-// Each thread issues the (index, weight) loads of up to kEdgeUnroll entries together and then their gathers
-// together (the gather depends on the index, so a loop of one entry at a time keeps about one load in flight per
-// wave: 403 against 288 us per launch on the 8M shell); the sums still run in entry order.
+// The rows come in slices of 64 edges (DevMesh32::fpairs, cpairs): each pair load of a wavefront reads one contiguous
+// kilobyte (the per-edge CSR rows, 40 bytes apart, touched a cache line per lane), a row's pair loads are issued
+// before its gathers, padding slots are gathered and added as zero terms without a branch, and the sums run in
+// entry order.  Face row, then cell row: both rows' loads together took 74 VGPRs (six waves per SIMD) and ran slower.
 // Blocks from n_light on take one high-valence edge each (heavy[block - n_light]) in the same launch.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi,
+__global__ void __launch_bounds__(kEdgeBlock) k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi,
                                                         const float *g, const float *v, const float *pv, const int8_t *dir, float dt,
                                                         float *out, const int32_t *heavy, int32_t n_light, const float *b, float omega)
 {
@@ -152,64 +177,57 @@ __global__ void __launch_bounds__(256) k_edge_apply_f32(DevMesh d, DevMesh32 s, 
         edge_apply_f32_heavy<MODE>(d, s, resistive, hall, heavy[blockIdx.x - n_light], phi, g, v, pv, dir, dt, out, b, omega);
         return;
     }
-    const int32_t e = xcd_block(uint32_t(n_light)) * int32_t(blockDim.x) + int32_t(threadIdx.x);
+    const int32_t e = xcd_block(uint32_t(n_light)) * kEdgeBlock + int32_t(threadIdx.x);
     if (e >= d.ne) return;
-    if (dir[e]) { out[e] = 0.0f; return; }
-    const int32_t f0 = d.e2f_ptr[e], f1 = d.e2f_ptr[e + 1];
-    // the cell row's bounds and the edge vector load with the face row's bounds, not after the face row's sum
-    int32_t c0 = 0, c1 = 0;
-    F3 t{0.0f, 0.0f, 0.0f};
-    if (hall) { c0 = d.ec_ptr[e]; c1 = d.ec_ptr[e + 1]; t = f3(s.evec, e); }
-    if (f1 - f0 > kLightMax) return; // one of the heavy blocks
-    float p0 = 0.0f, p1 = 0.0f;
-    // Row entries are read two per 16-byte load from the even entry at or before the row start (the pair arrays carry
-    // one padding entry at the end), half the load instructions of one 8-byte load per entry; entries outside the row
-    // are skipped, and the sums still run in entry order.
-    if (resistive)
-        for (int32_t k = f0 & ~1; k < f1; k += 2 * kPairUnroll)
-        {
-            int4 q[kPairUnroll];
-            float x[2 * kPairUnroll];
-#pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u)
-                if (k + 2 * u < f1) q[u] = reinterpret_cast<const int4 *>(s.e2fp)[(k >> 1) + u];
-#pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u)
-            {
-                if (k + 2 * u >= f0 && k + 2 * u < f1) x[2 * u] = phi[q[u].x];
-                if (k + 2 * u + 1 >= f0 && k + 2 * u + 1 < f1) x[2 * u + 1] = phi[q[u].z];
-            }
-#pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u)
-            {
-                if (k + 2 * u >= f0 && k + 2 * u < f1) p0 += __int_as_float(q[u].y) * x[2 * u];
-                if (k + 2 * u + 1 >= f0 && k + 2 * u + 1 < f1) p0 += __int_as_float(q[u].w) * x[2 * u + 1];
-            }
-        }
-    if (hall)
+    const int8_t de = dir[e], he = s.eheavy[e]; // both loads ahead of one test
+    if (de | he)
     {
-        for (int32_t k = c0 & ~1; k < c1; k += 2 * kPairUnroll)
+        if (!he) out[e] = 0.0f; // a Dirichlet edge; a high-valence edge is one of the heavy blocks
+        return;
+    }
+    const int32_t sl = __builtin_amdgcn_readfirstlane(e >> 6), lane = e & 63; // a wavefront is one slice
+    const F3 t = hall ? f3(s.evec, e) : F3{0.0f, 0.0f, 0.0f};
+    float p0 = 0.0f, p1 = 0.0f;
+    const int32_t fa = resistive ? s.fslice[sl] : 0, fn = resistive ? (s.fslice[sl + 1] - fa) >> 6 : 0;
+    const int32_t ca = hall ? s.cslice[sl] : 0, cn = hall ? (s.cslice[sl + 1] - ca) >> 6 : 0;
+    if (resistive)
+    {
+        for (int32_t p = 0; p < fn; p += kPairUnroll)
         {
             int4 q[kPairUnroll];
-            F3 x[2 * kPairUnroll];
+            float xf[2 * kPairUnroll];
 #pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u)
-                if (k + 2 * u < c1) q[u] = reinterpret_cast<const int4 *>(s.ecp)[(k >> 1) + u];
+            for (int u = 0; u < kPairUnroll; ++u) q[u] = ld_pair(s.fpairs, fa + lane + 64 * (p + u), p + u < fn ? -1 : 0);
 #pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u)
-            {
-                if (k + 2 * u >= c0 && k + 2 * u < c1) x[2 * u] = f3(g, q[u].x);
-                if (k + 2 * u + 1 >= c0 && k + 2 * u + 1 < c1) x[2 * u + 1] = f3(g, q[u].z);
-            }
+            for (int u = 0; u < kPairUnroll; ++u) { xf[2 * u] = phi[q[u].x]; xf[2 * u + 1] = phi[q[u].z]; }
 #pragma unroll
             for (int u = 0; u < kPairUnroll; ++u)
             {
-                if (k + 2 * u >= c0 && k + 2 * u < c1) p1 += __int_as_float(q[u].y) * fdot(x[2 * u], t);
-                if (k + 2 * u + 1 >= c0 && k + 2 * u + 1 < c1) p1 += __int_as_float(q[u].w) * fdot(x[2 * u + 1], t);
+                p0 += __int_as_float(q[u].y) * xf[2 * u];
+                p0 += __int_as_float(q[u].w) * xf[2 * u + 1];
             }
         }
     }
-    out[e] = edge_out<MODE>(s, e, dt * (s.m_e[e] * p0 + p1), v, pv, b, omega);
+    if (hall)
+    {
+        for (int32_t p = 0; p < cn; p += kPairUnroll)
+        {
+            int4 q[kPairUnroll];
+            F3 xc[2 * kPairUnroll];
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u) q[u] = ld_pair(s.cpairs, ca + lane + 64 * (p + u), p + u < cn ? -1 : 0);
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u) { xc[2 * u] = f3(g, q[u].x); xc[2 * u + 1] = f3(g, q[u].z); }
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u)
+            {
+                p1 += __int_as_float(q[u].y) * fdot(xc[2 * u], t);
+                p1 += __int_as_float(q[u].w) * fdot(xc[2 * u + 1], t);
+            }
+        }
+    }
+    const EdgeIn x = edge_in<MODE>(s, e, v, pv, b);
+    out[e] = edge_out<MODE>(s, e, dt * (x.me * p0 + p1), x, omega);
 }
 template __global__ void k_edge_apply_f32<kOpShifted>(DevMesh, DevMesh32, int32_t, int32_t, const float *, const float *, const float *,
                                                       const float *, const int8_t *, float, float *, const int32_t *, int32_t, const float *,
@@ -682,19 +700,44 @@ __global__ void __launch_bounds__(256) k_update_mixed(const float *const *V, con
             delta[e] += s * static_cast<double>(invdiag[e]);
         }
 }
-__global__ void __launch_bounds__(256) k_pack_ec(const int32_t *cell, const float *w, int64_t n, int2 *out)
+// Slices of the edge kernel's rows (DevMesh32::fpairs, cpairs): pair slots per edge of every 64-edge slice (the
+// longest light row's), then each light edge's entries at their slots, and the high-valence flags
+__global__ void __launch_bounds__(256) k_slice_pairs(const int32_t *ptr, const int32_t *e2f_ptr, int32_t ne, int32_t n_slices, int32_t *np)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = make_int2(cell[i], __float_as_int(w[i]));
+    const int32_t e = gid();
+    int32_t p = 0;
+    if (e < ne && e2f_ptr[e + 1] - e2f_ptr[e] <= kLightMax) p = (ptr[e + 1] - ptr[e] + 1) / 2;
+    for (int o = 32; o > 0; o >>= 1) p = max(p, __shfl_xor(p, o, 64));
+    if ((threadIdx.x & 63) == 0 && (e >> 6) < n_slices) np[e >> 6] = p;
 }
-__global__ void __launch_bounds__(256) k_pack_e2f(const int32_t *e2f, const float *star2, int64_t n, int2 *out)
+// face rows (star2 != nullptr: entry (face, signed star2 of the face) from the signed e2f ids) or cell rows
+// ((cell, weight) from idx and w); slots past a row keep the zero fill (entry 0, weight +0)
+__global__ void __launch_bounds__(256) k_slice_fill(const int32_t *ptr, const int32_t *e2f_ptr, const int32_t *idx, const float *w,
+                                                    const float *star2, int32_t ne, const int32_t *slice, int32_t *pairs)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n)
+    const int32_t e = gid();
+    if (e >= ne || e2f_ptr[e + 1] - e2f_ptr[e] > kLightMax) return;
+    int32_t *base = pairs + 4 * (size_t(slice[e >> 6]) + (e & 63));
+    for (int32_t k = ptr[e], j = 0; k < ptr[e + 1]; ++k, ++j)
     {
-        const int32_t q = e2f[i], f = pid(q);
-        out[i] = make_int2(f, __float_as_int(fsg(q) * star2[f]));
+        int32_t *slot = base + 4 * 64 * size_t(j >> 1) + 2 * (j & 1);
+        if (star2)
+        {
+            const int32_t q = idx[k], f = pid(q);
+            slot[0] = f;
+            slot[1] = __float_as_int(fsg(q) * star2[f]);
+        }
+        else
+        {
+            slot[0] = idx[k];
+            slot[1] = __float_as_int(w[k]);
+        }
     }
+}
+__global__ void __launch_bounds__(256) k_heavy_flags(const int32_t *e2f_ptr, int32_t ne, int8_t *flag)
+{
+    const int32_t e = gid();
+    if (e < ne) flag[e] = e2f_ptr[e + 1] - e2f_ptr[e] > kLightMax;
 }
 __global__ void __launch_bounds__(256) k_f64_to_f32(const double *x, int64_t n, float *out)
 {
@@ -760,12 +803,40 @@ void mhdg_ctx::mixed_build()
     M.s.recon = f32(d.recon, 3 * nc2f);
     M.s.ecw = f32(d.ecw, nec);
     {
-        int2 *ecp = static_cast<int2 *>(alloc((nec + 1) * sizeof(int2))); // + 1: the last pair load of the last row
-        hipLaunchKernelGGL(k_pack_ec, dim3(nb(nec)), dim3(256), 0, stream, d.ec_cell, M.s.ecw, int64_t(nec), ecp);
-        M.s.ecp = ecp;
-        int2 *e2fp = static_cast<int2 *>(alloc((e2f_nnz + 1) * sizeof(int2)));
-        hipLaunchKernelGGL(k_pack_e2f, dim3(nb(e2f_nnz)), dim3(256), 0, stream, d.e2f, M.s.star2, int64_t(e2f_nnz), e2fp);
-        M.s.e2fp = e2fp;
+        // slice layout of the edge kernel's rows: slot counts on the device, their prefix sums here (set-up only)
+        const int32_t n_slices = int32_t((E + 63) / 64);
+        auto slices = [&](const int32_t *ptr, const int32_t *idx, const float *w, const float *star2, const int4 *&pairs,
+                          const int32_t *&slice) {
+            int32_t *off = static_cast<int32_t *>(alloc((size_t(n_slices) + 1) * sizeof(int32_t)));
+            hipLaunchKernelGGL(k_slice_pairs, dim3(nb(E)), dim3(256), 0, stream, ptr, d.e2f_ptr, int32_t(E), n_slices, off);
+            std::vector<int32_t> h(size_t(n_slices) + 1);
+            chk(hipMemcpyAsync(h.data(), off, size_t(n_slices) * sizeof(int32_t), hipMemcpyDeviceToHost, stream), "D2H");
+            chk(hipStreamSynchronize(stream), "sync");
+            int64_t acc = 0;
+            for (int32_t k = 0; k < n_slices; ++k)
+            {
+                const int64_t n = 64 * int64_t(h[size_t(k)]);
+                h[size_t(k)] = int32_t(acc);
+                acc += n;
+                if (acc > INT32_MAX) throw std::runtime_error("edge rows too large for 32-bit slice offsets");
+            }
+            h[size_t(n_slices)] = int32_t(acc);
+            chk(hipMemcpyAsync(off, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream), "H2D");
+            // + one group of zeroed slots: the kernel loads whole groups of pairs and drops those past a slice's count
+            const size_t slots = size_t(acc) + 64 * size_t(kPairUnroll);
+            int4 *p = static_cast<int4 *>(alloc(slots * sizeof(int4)));
+            chk(hipMemsetAsync(p, 0, slots * sizeof(int4), stream), "memset");
+            hipLaunchKernelGGL(k_slice_fill, dim3(nb(E)), dim3(256), 0, stream, ptr, d.e2f_ptr, idx, w, star2, int32_t(E), off,
+                               reinterpret_cast<int32_t *>(p));
+            chk(hipStreamSynchronize(stream), "sync"); // h is freed on return
+            pairs = p;
+            slice = off;
+        };
+        slices(d.e2f_ptr, d.e2f, nullptr, M.s.star2, M.s.fpairs, M.s.fslice);
+        slices(d.ec_ptr, d.ec_cell, M.s.ecw, nullptr, M.s.cpairs, M.s.cslice);
+        int8_t *eh = static_cast<int8_t *>(alloc(E));
+        hipLaunchKernelGGL(k_heavy_flags, dim3(nb(E)), dim3(256), 0, stream, d.e2f_ptr, int32_t(E), eh);
+        M.s.eheavy = eh;
     }
     M.s.evec = f32(d.evec, 3 * E);
 
@@ -881,8 +952,9 @@ int mhdg_ctx::solve_mixed(const double *Phi_n, const double *Eex, const int8_t *
                     LAUNCH("bcell_f32", k_bcell_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.phi, M.b);
                     LAUNCH("hall_g_f32", k_hall_g_f32_g<kCellLanes>, dim3(nb(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.b, M.g);
                 }
-                LAUNCH("edge_apply_f32", k_edge_apply_f32<kOpShifted>, dim3(nb(ne) + n_heavy), dim3(256), 0, stream, d, M.s, int32_t(op_res),
-                       int32_t(op_hall), M.phi, M.g, M.V[j], M.pv, dir, dtf, M.w, static_cast<const int32_t *>(heavy), int32_t(nb(ne)),
+                const int32_t n_light = int32_t((int64_t(ne) + kEdgeBlock - 1) / kEdgeBlock);
+                LAUNCH("edge_apply_f32", k_edge_apply_f32<kOpShifted>, dim3(n_light + n_heavy), dim3(kEdgeBlock), 0, stream, d, M.s,
+                       int32_t(op_res), int32_t(op_hall), M.phi, M.g, M.V[j], M.pv, dir, dtf, M.w, static_cast<const int32_t *>(heavy), n_light,
                        static_cast<const float *>(nullptr), 0.0f);
             }
             // classical Gram-Schmidt against V_0..V_j; the update also normalises V_{j+1} unless the DGKS test asks

@@ -256,21 +256,22 @@ void mhdg_ctx::op32(int mode, int hall_on, const float *pv, const float *v, cons
         LAUNCH("bcell_f32", k_bcell_f32_g<kCellLanes>, dim3(nbk(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.phi, M.b);
         LAUNCH("hall_g_f32", k_hall_g_f32_g<kCellLanes>, dim3(nbk(kCellLanes * int64_t(nc))), dim3(256), 0, stream, d, M.s, M.b, M.g);
     }
-    const dim3 grid(nbk(ne) + n_heavy);
+    const int32_t n_light = int32_t((int64_t(ne) + kEdgeBlock - 1) / kEdgeBlock);
+    const dim3 grid(n_light + n_heavy), block(kEdgeBlock);
     const int32_t *hv = static_cast<const int32_t *>(heavy);
     switch (mode)
     {
     case kOpApply:
-        LAUNCH("mg_apply_f32", k_edge_apply_f32<kOpApply>, grid, dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(hall_on), M.phi, M.g, v, pv,
-               dir, dtf, out, hv, int32_t(nbk(ne)), b, omega);
+        LAUNCH("mg_apply_f32", k_edge_apply_f32<kOpApply>, grid, block, 0, stream, d, M.s, int32_t(op_res), int32_t(hall_on), M.phi, M.g, v, pv,
+               dir, dtf, out, hv, n_light, b, omega);
         break;
     case kOpSmooth:
-        LAUNCH("mg_smooth_f32", k_edge_apply_f32<kOpSmooth>, grid, dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(hall_on), M.phi, M.g, v,
-               pv, dir, dtf, out, hv, int32_t(nbk(ne)), b, omega);
+        LAUNCH("mg_smooth_f32", k_edge_apply_f32<kOpSmooth>, grid, block, 0, stream, d, M.s, int32_t(op_res), int32_t(hall_on), M.phi, M.g, v,
+               pv, dir, dtf, out, hv, n_light, b, omega);
         break;
     case kOpResid:
-        LAUNCH("mg_resid_f32", k_edge_apply_f32<kOpResid>, grid, dim3(256), 0, stream, d, M.s, int32_t(op_res), int32_t(hall_on), M.phi, M.g, v,
-               pv, dir, dtf, out, hv, int32_t(nbk(ne)), b, omega);
+        LAUNCH("mg_resid_f32", k_edge_apply_f32<kOpResid>, grid, block, 0, stream, d, M.s, int32_t(op_res), int32_t(hall_on), M.phi, M.g, v,
+               pv, dir, dtf, out, hv, n_light, b, omega);
         break;
     default: throw std::runtime_error("op32: unknown mode");
     }

@@ -18,9 +18,10 @@
 // std::vectors owned by the drivers: each call copies its inputs to the device and its outputs back.
 //
 // Scope: one rank (the Exchanger is the identity, every entity is owned); the fluid calls serve any species
-// index below 10 with the caller's free-stream state (the ion slots of MarsSolver's ten-ion fluid).  A GhostFn is evaluated on the host once per boundary face and species into the ghost table
-// of the device call, with the first-order cell state as `inside` (exact for ghost functions that ignore the
-// inside state, which is the only use MarsSolver makes of it: MarsSolver.cpp:124-128, 148-167).
+// index below 10 with the caller's free-stream state (the ion slots of MarsSolver's ten-ion fluid).  A GhostFn is
+// evaluated on the host once per boundary face and species into the ghost table of the device call, with the inside
+// state the reference would hand it (the cell value for the gradients, the reconstructed face state for the fluxes),
+// so any GhostFn gives the reference's result, MarsSolver's (MarsSolver.cpp:124-128) included.
 // The implicit solve runs in FP64 (the reference iteration) unless MHDG_SOLVE_PRECISION=1 selects the
 // mixed-precision solve (2: mixed precision with the multigrid preconditioner; 3: FP64 with classical Gram-Schmidt).
 #include <cstdlib>
@@ -185,9 +186,12 @@ mhdg_ctx *context(const uns::CellComplex &cx)
     return context(r);
 }
 
-// GhostFn -> per-face ghost table (5 primitives per face; NaN-free copies of W for non-boundary faces)
+// GhostFn -> per-face ghost table (5 primitives per face; zero rows for faces without a ghost).  The GhostFn sees
+// the inside state the reference hands it: the cell value in the gradient call (grad == nullptr,
+// Reconstruction.cpp:95-105), the linear reconstruction at the face centre with its positivity fallback in the flux
+// call (FluxRHS.cpp:16-27, 57-65), restated here in the same evaluation order.
 std::vector<double> ghost_table(const uns::LocalMesh &lm, const uns::Geometry &geo, const mhd::BoundaryInfo &bc, const double *W,
-                                int species, const mhd::GhostFn &fn)
+                                const double *grad, int species, const mhd::GhostFn &fn)
 {
     const uns::CellComplex &cx = lm.mesh.cx;
     std::vector<double> t(5 * static_cast<size_t>(cx.n_faces()), 0.0);
@@ -202,7 +206,20 @@ std::vector<double> ghost_table(const uns::LocalMesh &lm, const uns::Geometry &g
         const double s = sg / geo.face_area_mag[f];
         const uns::Vec3 n(s * geo.face_area[f].x, s * geo.face_area[f].y, s * geo.face_area[f].z);
         const double *w = W + 5 * static_cast<size_t>(c);
-        const mhd::Prim g = fn(f, species, mhd::Prim{w[0], w[1], w[2], w[3], w[4]}, n);
+        mhd::Prim in{w[0], w[1], w[2], w[3], w[4]};
+        if (grad)
+        {
+            const double *gr = grad + 15 * static_cast<size_t>(c);
+            const uns::Vec3 dx = geo.face_center[f] - geo.cell_center[c];
+            mhd::Prim r;
+            r.rho = w[0] + gr[0] * dx.x + gr[1] * dx.y + gr[2] * dx.z;
+            r.u = w[1] + gr[3] * dx.x + gr[4] * dx.y + gr[5] * dx.z;
+            r.v = w[2] + gr[6] * dx.x + gr[7] * dx.y + gr[8] * dx.z;
+            r.w = w[3] + gr[9] * dx.x + gr[10] * dx.y + gr[11] * dx.z;
+            r.p = w[4] + gr[12] * dx.x + gr[13] * dx.y + gr[14] * dx.z;
+            if (r.rho > 0 && r.p > 0) in = r;
+        }
+        const mhd::Prim g = fn(f, species, in, n);
         double *q = &t[5 * static_cast<size_t>(f)];
         q[0] = g.rho; q[1] = g.u; q[2] = g.v; q[3] = g.w; q[4] = g.p;
     }
@@ -261,7 +278,7 @@ void Stencil::limited_gradients(const uns::LocalMesh &lm, const uns::Geometry &g
     ok(ctx, mhdg_set_recon_rel_limit(ctx, rel_limit_), "Stencil::set_rel_limit");
     const double f5[5] = {fs.rho, fs.u, fs.v, fs.w, fs.p};
     std::vector<double> gt;
-    if (ghost && *ghost) gt = ghost_table(lm, geo, bc, W, species, *ghost);
+    if (ghost && *ghost) gt = ghost_table(lm, geo, bc, W, nullptr, species, *ghost);
     ok(ctx, mhdg_limited_gradients(ctx, W, species, f5, limit ? 1 : 0, gt.empty() ? nullptr : gt.data(), grad), "limited_gradients");
 }
 
@@ -275,7 +292,7 @@ void fluid_flux_rhs(const uns::LocalMesh &lm, const uns::Geometry &geo, const Bo
     mhdg_ctx *ctx = context(r);
     const double f5[5] = {fs.rho, fs.u, fs.v, fs.w, fs.p};
     std::vector<double> gt;
-    if (ghost && *ghost) gt = ghost_table(lm, geo, bc, W, species, *ghost);
+    if (ghost && *ghost) gt = ghost_table(lm, geo, bc, W, grad, species, *ghost);
     ok(ctx, mhdg_flux_rhs(ctx, species, f5, W, grad, gt.empty() ? nullptr : gt.data(), rhs), "fluid_flux_rhs");
 }
 

@@ -243,6 +243,8 @@ class Reference:
                 "ref_flux_rhs": [_P, C.c_int, _P, _P, _P, _P],
                 "ref_limited_gradients_ghost": [_P, _P, C.c_int, _P, C.c_int, _P, _P, _P],
                 "ref_flux_rhs_ghost": [_P, C.c_int, _P, _P, _P, _P, _P, _P], "ref_faraday_edge_dt": [_P, _P, _P, _P],
+                "ref_limited_gradients_wall_ghost": [_P, _P, C.c_int, _P, C.c_int, _P, _P],
+                "ref_flux_rhs_wall_ghost": [_P, C.c_int, _P, _P, _P, _P, _P],
                 "ref_ideal_emf": [_P, _P, _P, _P, _P, _P, _D, _P], "ref_faraday": [_P, _P, _P, _D],
                 "ref_div_b": [_P, _P, _P, _P, _P], "ref_local_dt": [_P, _P, _P, _P],
                 "ref_add_sources": [_P, C.c_int, _P, _P, _P, _P, _P, _D, _P],
@@ -367,6 +369,20 @@ class Reference:
         self.lib().ref_flux_rhs_ghost(self._h, species, _ptr(_f(fs)), _ptr(_f(W)), _ptr(_f(grad)), _ptr(_f(ghost)), _ptr(out),
                                       C.byref(seen))
         return out, seen.value
+
+    def limited_gradients_wall_ghost(self, W, species, fs, wall, limit=True):
+        """Stencil::limited_gradients with a GhostFn of MarsSolver's form (MarsSolver.cpp:124-128): `wall` (5 per face)
+        on wall faces, ghost_state of the inside state the reference hands it on the other boundary faces."""
+        out = np.zeros(15 * self.nc)
+        self.lib().ref_limited_gradients_wall_ghost(self._h, _ptr(_f(W)), species, _ptr(_f(fs)), int(limit), _ptr(_f(wall)), _ptr(out))
+        return out
+
+    def flux_rhs_wall_ghost(self, species, fs, W, grad, wall):
+        """fluid_flux_rhs with the GhostFn of limited_gradients_wall_ghost (its outflow ghosts are the reconstructed
+        face states)."""
+        out = np.zeros(5 * self.nc)
+        self.lib().ref_flux_rhs_wall_ghost(self._h, species, _ptr(_f(fs)), _ptr(_f(W)), _ptr(_f(grad)), _ptr(_f(wall)), _ptr(out))
+        return out
 
     def ideal_emf(self, u, B_tot, B_ind, phi_tot, phi_ind, cuct=0.5, E=None):
         E = np.zeros(self.ne) if E is None else _f(E).copy()

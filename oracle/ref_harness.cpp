@@ -197,6 +197,34 @@ void ref_flux_rhs_ghost(ref_ctx *r, int species, const double *fs, const double 
     const GhostFn g = table_ghost(table, seen);
     fluid_flux_rhs(r->lm, r->geo, *r->bc, r->cfg, species, prim5(fs), W, grad, rhs, &g);
 }
+// The same two calls with a GhostFn of MarsSolver's form (MarsSolver.cpp:124-128): the cached per-face state on wall
+// faces, ghost_state(kind, species, inside, n, fs) on every other boundary face, so the ghost of an outflow face is the
+// inside state the reference hands the GhostFn (the cell value for the gradients, the reconstructed face state for the
+// fluxes).
+static GhostFn wall_table_ghost(const ref_ctx *r, const double *table, const Prim &fs)
+{
+    return [r, table, fs](Id f, int species, const Prim &in, const Vec3 &n) {
+        const FaceKind k = r->bc->face_kind[f];
+        if (k == FaceKind::Wall)
+        {
+            const double *g = table + 5 * static_cast<std::size_t>(f);
+            return Prim{g[0], g[1], g[2], g[3], g[4]};
+        }
+        return ghost_state(k, species, in, n, fs);
+    };
+}
+void ref_limited_gradients_wall_ghost(ref_ctx *r, const double *W, int species, const double *fs, int limit, const double *table,
+                                      double *grad)
+{
+    const GhostFn g = wall_table_ghost(r, table, prim5(fs));
+    r->st->limited_gradients(r->lm, r->geo, *r->bc, W, species, prim5(fs), limit != 0, grad, &g);
+}
+void ref_flux_rhs_wall_ghost(ref_ctx *r, int species, const double *fs, const double *W, const double *grad, const double *table,
+                             double *rhs)
+{
+    const GhostFn g = wall_table_ghost(r, table, prim5(fs));
+    fluid_flux_rhs(r->lm, r->geo, *r->bc, r->cfg, species, prim5(fs), W, grad, rhs, &g);
+}
 void ref_ideal_emf(ref_ctx *r, const double *u, const double *Bt, const double *Bi, const double *pt, const double *pi, double cuct, double *E)
 {
     IdealInputs in;

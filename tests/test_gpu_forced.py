@@ -119,3 +119,40 @@ def test_shim_ghostfn_table(small_explicit, species):
         assert np.array_equal(gh, go)
         rh, _ = hyb.flux_rhs(species, fs, W, go, ghost=table)
         assert np.array_equal(rh, orc.flux_rhs(species, fs, W, go, table))
+
+
+@pytest.mark.skipif(not __import__("oracle.oracle", fromlist=["x"]).have_ref_on_device(),
+                    reason="oracle/_ref/libmpcns_ref_gpu.so not built")
+@pytest.mark.parametrize("species", [0, 1])
+def test_shim_ghostfn_of_marssolver_form(small_explicit, species):
+    """A GhostFn of MarsSolver's form (MarsSolver.cpp:124-128: a cached state on wall faces, ghost_state of the inside
+    state on the others) through the shim on the device, against the compiled reference on the CPU, bit for bit.  The
+    reference hands the GhostFn the cell value in the gradients and the reconstructed face state in the fluxes
+    (FluxRHS.cpp:57-65), so the ghost of an outflow face differs between the two calls; a table built from the cell
+    values alone gives other fluxes."""
+    from mpcns_uns_amd import workloads
+    from oracle.oracle import reference_for_mesh
+
+    w = small_explicit
+    m = w["mesh"]
+    _, orc = _pair(w)
+    orc.prepare()
+    W = orc.derived("W_H" if species == 0 else "W_Na")
+    fs = np.array(w["cfg"].fs[species][:])
+    wall = np.abs(np.random.default_rng(5 + species).normal(1.0, 0.2, 5 * m.n_faces))
+    kind = m.array("face_kind")
+    assert np.count_nonzero(kind == 3) > 0  # outflow faces, whose ghost is the inside state
+    ref = reference_for_mesh(m, w["cfg"], dipole_m=workloads.DIPOLE_M)
+    hyb = reference_for_mesh(m, w["cfg"], dipole_m=workloads.DIPOLE_M, on_device=True)
+    for limit in (True, False):
+        gr = ref.limited_gradients_wall_ghost(W, species, fs, wall, limit)
+        assert np.array_equal(hyb.limited_gradients_wall_ghost(W, species, fs, wall, limit), gr)
+        rr = ref.flux_rhs_wall_ghost(species, fs, W, gr, wall)
+        assert np.array_equal(hyb.flux_rhs_wall_ghost(species, fs, W, gr, wall), rr)
+        # the same GhostFn evaluated with the cell value as the inside state of every boundary face
+        ptr, idx = m.array("f2c.ptr"), m.array("f2c.idx")
+        first = np.zeros(5 * m.n_faces)
+        for f in np.nonzero((kind != 0) & (kind != 4))[0]:
+            c = idx[ptr[f]]
+            first[5 * f:5 * f + 5] = wall[5 * f:5 * f + 5] if kind[f] == 1 else (fs if kind[f] == 2 else W[5 * c:5 * c + 5])
+        assert not np.array_equal(ref.flux_rhs(species, fs, W, gr, ghost=first)[0], rr)

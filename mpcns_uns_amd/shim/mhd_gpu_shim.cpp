@@ -317,15 +317,29 @@ std::vector<char> active_faces(const uns::CellComplex &cx)
     for (Id f = 0; f < cx.n_faces(); ++f) a[f] = cx.f2c.count(f) > 0;
     return a;
 }
-void faraday_update(const uns::CellComplex &cx, const std::vector<char> &, const double *E, double *B, double dt)
+// The device updates every face; the faces the caller's mask leaves inactive keep their values, as in the reference
+// (the drivers pass active_faces, under which no face with a cell is inactive on one rank).
+namespace
 {
-    mhdg_ctx *ctx = context(cx);
-    ok(ctx, mhdg_faraday(ctx, E, B, dt), "faraday_update");
+template <class F> void masked_update(const uns::CellComplex &cx, const std::vector<char> &active, double *B, F &&update)
+{
+    if (active.size() != static_cast<size_t>(cx.n_faces())) throw std::runtime_error("faraday_update: active mask of the wrong size");
+    std::vector<std::pair<Id, double>> keep;
+    for (Id f = 0; f < cx.n_faces(); ++f)
+        if (!active[f]) keep.emplace_back(f, B[f]);
+    update();
+    for (const auto &[f, v] : keep) B[f] = v;
 }
-void faraday_update_edge_dt(const uns::CellComplex &cx, const std::vector<char> &, const double *E, const double *dt_edge, double *B)
+} // namespace
+void faraday_update(const uns::CellComplex &cx, const std::vector<char> &active, const double *E, double *B, double dt)
 {
     mhdg_ctx *ctx = context(cx);
-    ok(ctx, mhdg_faraday_edge_dt(ctx, E, dt_edge, B), "faraday_update_edge_dt");
+    masked_update(cx, active, B, [&] { ok(ctx, mhdg_faraday(ctx, E, B, dt), "faraday_update"); });
+}
+void faraday_update_edge_dt(const uns::CellComplex &cx, const std::vector<char> &active, const double *E, const double *dt_edge, double *B)
+{
+    mhdg_ctx *ctx = context(cx);
+    masked_update(cx, active, B, [&] { ok(ctx, mhdg_faraday_edge_dt(ctx, E, dt_edge, B), "faraday_update_edge_dt"); });
 }
 
 // DivB.cpp:8-24

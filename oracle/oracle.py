@@ -243,7 +243,7 @@ class Reference:
                 "ref_flux_rhs": [_P, C.c_int, _P, _P, _P, _P],
                 "ref_limited_gradients_ghost": [_P, _P, C.c_int, _P, C.c_int, _P, _P, _P],
                 "ref_flux_rhs_ghost": [_P, C.c_int, _P, _P, _P, _P, _P, _P], "ref_faraday_edge_dt": [_P, _P, _P, _P],
-                "ref_limited_gradients_wall_ghost": [_P, _P, C.c_int, _P, C.c_int, _P, _P],
+                "ref_limited_gradients_wall_ghost": [_P, _P, C.c_int, _P, C.c_int, _P, _P], "ref_faraday_masked": [_P, _P, _P, _D, _P],
                 "ref_flux_rhs_wall_ghost": [_P, C.c_int, _P, _P, _P, _P, _P],
                 "ref_ideal_emf": [_P, _P, _P, _P, _P, _P, _D, _P], "ref_faraday": [_P, _P, _P, _D],
                 "ref_div_b": [_P, _P, _P, _P, _P], "ref_local_dt": [_P, _P, _P, _P],
@@ -369,6 +369,12 @@ class Reference:
         self.lib().ref_flux_rhs_ghost(self._h, species, _ptr(_f(fs)), _ptr(_f(W)), _ptr(_f(grad)), _ptr(_f(ghost)), _ptr(out),
                                       C.byref(seen))
         return out, seen.value
+
+    def faraday_masked(self, E, Phi, dt, mask):
+        """faraday_update with a caller-given face mask (nonzero = active): inactive faces keep their values."""
+        out = _f(Phi).copy()
+        self.lib().ref_faraday_masked(self._h, _ptr(_f(E)), _ptr(out), dt, _ptr(np.ascontiguousarray(mask, dtype=np.int8)))
+        return out
 
     def limited_gradients_wall_ghost(self, W, species, fs, wall, limit=True):
         """Stencil::limited_gradients with a GhostFn of MarsSolver's form (MarsSolver.cpp:124-128): `wall` (5 per face)

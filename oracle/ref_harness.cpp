@@ -234,6 +234,12 @@ void ref_ideal_emf(ref_ctx *r, const double *u, const double *Bt, const double *
     add_ideal_emf(r->lm, r->geo, in, p, E);
 }
 void ref_faraday(ref_ctx *r, const double *E, double *Phi, double dt) { faraday_update(r->lm.mesh.cx, r->active, E, Phi, dt); }
+// faraday_update with the caller's face mask (nonzero = active) in place of active_faces
+void ref_faraday_masked(ref_ctx *r, const double *E, double *Phi, double dt, const int8_t *mask)
+{
+    const std::vector<char> a(mask, mask + r->lm.mesh.cx.n_faces());
+    faraday_update(r->lm.mesh.cx, a, E, Phi, dt);
+}
 void ref_faraday_edge_dt(ref_ctx *r, const double *E, const double *dt_edge, double *Phi)
 {
     faraday_update_edge_dt(r->lm.mesh.cx, r->active, E, dt_edge, Phi);

@@ -156,3 +156,23 @@ def test_shim_ghostfn_of_marssolver_form(small_explicit, species):
             c = idx[ptr[f]]
             first[5 * f:5 * f + 5] = wall[5 * f:5 * f + 5] if kind[f] == 1 else (fs if kind[f] == 2 else W[5 * c:5 * c + 5])
         assert not np.array_equal(ref.flux_rhs(species, fs, W, gr, ghost=first)[0], rr)
+
+
+@pytest.mark.skipif(not __import__("oracle.oracle", fromlist=["x"]).have_ref_on_device(),
+                    reason="oracle/_ref/libmpcns_ref_gpu.so not built")
+def test_shim_faraday_keeps_inactive_faces(small_explicit):
+    """faraday_update (Faraday.cpp:15-25) with a caller's mask other than active_faces: through the shim on the device
+    the inactive faces keep their values and the active ones are updated, bit for bit as the compiled reference."""
+    from mpcns_uns_amd import workloads
+    from oracle.oracle import reference_for_mesh
+
+    w = small_explicit
+    m = w["mesh"]
+    rng = np.random.default_rng(11)
+    E, Phi = rng.normal(size=m.n_edges), rng.normal(size=m.n_faces)
+    mask = (rng.random(m.n_faces) < 0.5).astype(np.int8)
+    ref = reference_for_mesh(m, w["cfg"], dipole_m=workloads.DIPOLE_M)
+    hyb = reference_for_mesh(m, w["cfg"], dipole_m=workloads.DIPOLE_M, on_device=True)
+    r, h = ref.faraday_masked(E, Phi, 0.3, mask), hyb.faraday_masked(E, Phi, 0.3, mask)
+    assert np.array_equal(h, r)
+    assert np.array_equal(r[mask == 0], Phi[mask == 0]) and not np.array_equal(r[mask == 1], Phi[mask == 1])

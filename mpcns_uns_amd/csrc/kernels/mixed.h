@@ -15,15 +15,18 @@ struct DevMesh32
     float *aB0;               // per cell alpha_c B0_c of the frozen state (3)
     const float *aV;          // per (cell, face) entry: s A_f / V_c (3)
     const int32_t *nbr;       // per (cell, face) entry: the cell across the face, -1 on the boundary
-    // The edge kernel's rows in slices of 64 edges (one wavefront): entries 2p and 2p+1 of the row of edge 64 s + i as
-    // one 16-byte pair (index, bits of the FP32 weight, index, bits of the FP32 weight) at pair slot
-    // slice[s] + 64 p + i, so each pair load of a wavefront reads one contiguous kilobyte.  A slice holds as many
-    // pairs per edge as its longest row needs; slots past a row, and the rows of high-valence edges (the heavy
-    // blocks' edges), hold (0, +0), so the kernel gathers and adds them without a branch (a zero term leaves the
-    // value of every sum as it was).
-    const int4 *fpairs;       // edge-face rows: (face, signed star2 of the face)
-    const int4 *cpairs;       // edge-cell rows: (cell, edge-cell weight)
-    const int32_t *fslice, *cslice; // first pair slot of every slice (n_slices + 1 entries)
+    // The edge kernel's rows in slices of 64 edges (one wavefront).  Entries 2p and 2p+1 of the row of edge 64 s + i
+    // form pair p of lane i; a slice holds as many pairs per edge as its longest row needs, the pairs of one p side by
+    // side, so each pair load of a wavefront reads one contiguous range.  A slice whose entries' ids span less than
+    // 2^16 is compact: 12-byte pairs (the two ids as 16-bit offsets from the slice's base id in one word, then the
+    // two FP32 weights); any other slice is wide: 16-byte pairs (id, weight, id, weight).  Slots past a row, and the
+    // rows of high-valence edges (the heavy blocks' edges), hold offset or id 0 with weight +0, so the kernel gathers
+    // and adds them without a branch (a zero term leaves the value of every sum as it was).
+    // Descriptor of slice s: {first word, pairs per edge, base id, wide}.
+    const int4 *fdesc;        // edge-face rows, entries (face, signed star2 of the face): slice descriptors
+    const int32_t *fwords;    //   and the slices' words
+    const int4 *cdesc;        // edge-cell rows, entries (cell, edge-cell weight)
+    const int32_t *cwords;
     const int8_t *eheavy;     // 1 on the edges of valence above kLightMax (the edge kernel's heavy blocks)
     int32_t cdeg, fdeg;       // faces per cell / edges per face when uniform (tets: 4, 3), else 0 (ptr arrays)
 };

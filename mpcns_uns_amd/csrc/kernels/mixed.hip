@@ -20,7 +20,7 @@ namespace mhdg
 {
 namespace
 {
-constexpr int kPairUnroll = 3; // 16-byte row pairs loaded together by the FP32 edge kernel (rows of up to six entries in one group)
+constexpr int kPairUnroll = 3; // row pairs loaded together by the FP32 edge kernel (rows of up to six entries in one group)
 struct F3 { float x, y, z; };
 __device__ __forceinline__ F3 f3(const float *p, long i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 __device__ __forceinline__ F3 fcross(const F3 &a, const F3 &b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
@@ -37,12 +37,21 @@ __device__ __forceinline__ float4 ldg4(const float *p, int32_t q)
 }
 typedef int I4v __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) I4v *GlobalI4;
-// a row pair of the edge kernel, one 16-byte load, kept (mask -1) or made a (0, +0) pad (mask 0) with no branch: a
-// conditional load, or a select on its result, became a branch around the load and a wait at its end
-__device__ __forceinline__ int4 ld_pair(const int4 *p, int32_t q, int32_t mask)
+typedef const __attribute__((address_space(1))) int32_t *GlobalI32;
+// Row pairs of the edge kernel, kept (mask -1) or made a zero pad (mask 0) with no branch: a conditional load, or a
+// select on its result, became a branch around the load and a wait at its end.  Pairs past a slice's count read the
+// following slots (the word arrays end in one zeroed group of slots).
+// wide pair: one 16-byte load at word w (a multiple of 4)
+__device__ __forceinline__ int4 ld_wide(const int32_t *words, int32_t w, int32_t mask)
 {
-    const I4v v = ((GlobalI4)p)[q];
+    const I4v v = ((GlobalI4)words)[w >> 2];
     return make_int4(v.x & mask, v.y & mask, v.z & mask, v.w & mask);
+}
+// compact pair: three words from w (one 12-byte load): x = the two 16-bit offsets, y and z = the weights
+__device__ __forceinline__ int3 ld_compact(const int32_t *words, int32_t w, int32_t mask)
+{
+    const GlobalI32 p = (GlobalI32)words + w;
+    return make_int3(p[0] & mask, p[1] & mask, p[2] & mask);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -160,9 +169,82 @@ __device__ __forceinline__ void edge_apply_f32_heavy(const DevMesh &d, const Dev
     }
 }
 
+// Sum of one edge's row in its slice (descriptor dsc): KIND 0 the face row, sum_f w_f x_f; KIND 1 the cell row,
+// sum_c w_c g_c . t.  A group of the row's pairs is loaded, then gathered, then summed in entry order.
+template <int KIND>
+__device__ __forceinline__ float row_sum(const int4 &dsc, const int32_t *words, int32_t lane, const float *x, const F3 &t)
+{
+    float acc = 0.0f;
+    const int32_t np = dsc.y;
+    if (dsc.w == 0)
+    {
+        for (int32_t p = 0; p < np; p += kPairUnroll)
+        {
+            int3 q[kPairUnroll];
+            float xs[KIND == 0 ? 2 * kPairUnroll : 1];
+            F3 gs[KIND == 1 ? 2 * kPairUnroll : 1];
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u) q[u] = ld_compact(words, dsc.x + 3 * (64 * (p + u) + lane), p + u < np ? -1 : 0);
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u)
+            {
+                const int32_t i0 = dsc.z + (q[u].x & 0xffff), i1 = dsc.z + int32_t(uint32_t(q[u].x) >> 16);
+                if constexpr (KIND == 0) { xs[2 * u] = x[i0]; xs[2 * u + 1] = x[i1]; }
+                else { gs[2 * u] = f3(x, i0); gs[2 * u + 1] = f3(x, i1); }
+            }
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u)
+            {
+                if constexpr (KIND == 0)
+                {
+                    acc += __int_as_float(q[u].y) * xs[2 * u];
+                    acc += __int_as_float(q[u].z) * xs[2 * u + 1];
+                }
+                else
+                {
+                    acc += __int_as_float(q[u].y) * fdot(gs[2 * u], t);
+                    acc += __int_as_float(q[u].z) * fdot(gs[2 * u + 1], t);
+                }
+            }
+        }
+    }
+    else
+    {
+        for (int32_t p = 0; p < np; p += kPairUnroll)
+        {
+            int4 q[kPairUnroll];
+            float xs[KIND == 0 ? 2 * kPairUnroll : 1];
+            F3 gs[KIND == 1 ? 2 * kPairUnroll : 1];
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u) q[u] = ld_wide(words, dsc.x + 4 * (64 * (p + u) + lane), p + u < np ? -1 : 0);
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u)
+            {
+                if constexpr (KIND == 0) { xs[2 * u] = x[q[u].x]; xs[2 * u + 1] = x[q[u].z]; }
+                else { gs[2 * u] = f3(x, q[u].x); gs[2 * u + 1] = f3(x, q[u].z); }
+            }
+#pragma unroll
+            for (int u = 0; u < kPairUnroll; ++u)
+            {
+                if constexpr (KIND == 0)
+                {
+                    acc += __int_as_float(q[u].y) * xs[2 * u];
+                    acc += __int_as_float(q[u].w) * xs[2 * u + 1];
+                }
+                else
+                {
+                    acc += __int_as_float(q[u].y) * fdot(gs[2 * u], t);
+                    acc += __int_as_float(q[u].w) * fdot(gs[2 * u + 1], t);
+                }
+            }
+        }
+    }
+    return acc;
+}
+
 // out = dir ? 0 : (A P - I) v = (pv - v) + dt F(Phi') with F = m_e sum_f s star2_f Phi'_f + sum_c w_ec g_c . t_e
 // (the shifted operator: same Krylov space as A P, and the O(1) identity part never enters the FP32 vector)
-// The rows come in slices of 64 edges (DevMesh32::fpairs, cpairs): each pair load of a wavefront reads one contiguous
+// The rows come in slices of 64 edges (DevMesh32::fdesc/fwords, cdesc/cwords): each pair load of a wavefront reads one contiguous
 // kilobyte (the per-edge CSR rows, 40 bytes apart, touched a cache line per lane), a row's pair loads are issued
 // before its gathers, padding slots are gathered and added as zero terms without a branch, and the sums run in
 // entry order.  Face row, then cell row: both rows' loads together took 74 VGPRs (six waves per SIMD) and ran slower.
@@ -187,45 +269,8 @@ __global__ void __launch_bounds__(kEdgeBlock) k_edge_apply_f32(DevMesh d, DevMes
     }
     const int32_t sl = __builtin_amdgcn_readfirstlane(e >> 6), lane = e & 63; // a wavefront is one slice
     const F3 t = hall ? f3(s.evec, e) : F3{0.0f, 0.0f, 0.0f};
-    float p0 = 0.0f, p1 = 0.0f;
-    const int32_t fa = resistive ? s.fslice[sl] : 0, fn = resistive ? (s.fslice[sl + 1] - fa) >> 6 : 0;
-    const int32_t ca = hall ? s.cslice[sl] : 0, cn = hall ? (s.cslice[sl + 1] - ca) >> 6 : 0;
-    if (resistive)
-    {
-        for (int32_t p = 0; p < fn; p += kPairUnroll)
-        {
-            int4 q[kPairUnroll];
-            float xf[2 * kPairUnroll];
-#pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u) q[u] = ld_pair(s.fpairs, fa + lane + 64 * (p + u), p + u < fn ? -1 : 0);
-#pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u) { xf[2 * u] = phi[q[u].x]; xf[2 * u + 1] = phi[q[u].z]; }
-#pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u)
-            {
-                p0 += __int_as_float(q[u].y) * xf[2 * u];
-                p0 += __int_as_float(q[u].w) * xf[2 * u + 1];
-            }
-        }
-    }
-    if (hall)
-    {
-        for (int32_t p = 0; p < cn; p += kPairUnroll)
-        {
-            int4 q[kPairUnroll];
-            F3 xc[2 * kPairUnroll];
-#pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u) q[u] = ld_pair(s.cpairs, ca + lane + 64 * (p + u), p + u < cn ? -1 : 0);
-#pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u) { xc[2 * u] = f3(g, q[u].x); xc[2 * u + 1] = f3(g, q[u].z); }
-#pragma unroll
-            for (int u = 0; u < kPairUnroll; ++u)
-            {
-                p1 += __int_as_float(q[u].y) * fdot(xc[2 * u], t);
-                p1 += __int_as_float(q[u].w) * fdot(xc[2 * u + 1], t);
-            }
-        }
-    }
+    const float p0 = resistive ? row_sum<0>(s.fdesc[sl], s.fwords, lane, phi, t) : 0.0f;
+    const float p1 = hall ? row_sum<1>(s.cdesc[sl], s.cwords, lane, g, t) : 0.0f;
     const EdgeIn x = edge_in<MODE>(s, e, v, pv, b);
     out[e] = edge_out<MODE>(s, e, dt * (x.me * p0 + p1), x, omega);
 }
@@ -700,37 +745,68 @@ __global__ void __launch_bounds__(256) k_update_mixed(const float *const *V, con
             delta[e] += s * static_cast<double>(invdiag[e]);
         }
 }
-// Slices of the edge kernel's rows (DevMesh32::fpairs, cpairs): pair slots per edge of every 64-edge slice (the
-// longest light row's), then each light edge's entries at their slots, and the high-valence flags
-__global__ void __launch_bounds__(256) k_slice_pairs(const int32_t *ptr, const int32_t *e2f_ptr, int32_t ne, int32_t n_slices, int32_t *np)
+// Slices of the edge kernel's rows (DevMesh32::fdesc/fwords, cdesc/cwords).  Per 64-edge slice: the pairs per
+// edge its longest light row needs and the smallest and largest id in its light rows ({pairs, min, max, 0}; a slice
+// with none has max -1)
+__global__ void __launch_bounds__(256) k_slice_scan(const int32_t *ptr, const int32_t *idx, int32_t signed_ids, const int32_t *e2f_ptr,
+                                                    int32_t ne, int32_t n_slices, int4 *out)
 {
     const int32_t e = gid();
-    int32_t p = 0;
-    if (e < ne && e2f_ptr[e + 1] - e2f_ptr[e] <= kLightMax) p = (ptr[e + 1] - ptr[e] + 1) / 2;
-    for (int o = 32; o > 0; o >>= 1) p = max(p, __shfl_xor(p, o, 64));
-    if ((threadIdx.x & 63) == 0 && (e >> 6) < n_slices) np[e >> 6] = p;
+    int32_t p = 0, lo = INT32_MAX, hi = -1;
+    if (e < ne && e2f_ptr[e + 1] - e2f_ptr[e] <= kLightMax)
+    {
+        p = (ptr[e + 1] - ptr[e] + 1) / 2;
+        for (int32_t k = ptr[e]; k < ptr[e + 1]; ++k)
+        {
+            const int32_t i = signed_ids ? pid(idx[k]) : idx[k];
+            lo = min(lo, i);
+            hi = max(hi, i);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        p = max(p, __shfl_xor(p, o, 64));
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && (e >> 6) < n_slices) out[e >> 6] = make_int4(p, lo, hi, 0);
 }
-// face rows (star2 != nullptr: entry (face, signed star2 of the face) from the signed e2f ids) or cell rows
-// ((cell, weight) from idx and w); slots past a row keep the zero fill (entry 0, weight +0)
+// each light edge's entries at their slots: face rows (star2 != nullptr: entry (face, signed star2 of the face) from the
+// signed e2f ids) or cell rows ((cell, weight) from idx and w); slots past a row keep the zero fill
 __global__ void __launch_bounds__(256) k_slice_fill(const int32_t *ptr, const int32_t *e2f_ptr, const int32_t *idx, const float *w,
-                                                    const float *star2, int32_t ne, const int32_t *slice, int32_t *pairs)
+                                                    const float *star2, int32_t ne, const int4 *desc, int32_t *words)
 {
     const int32_t e = gid();
     if (e >= ne || e2f_ptr[e + 1] - e2f_ptr[e] > kLightMax) return;
-    int32_t *base = pairs + 4 * (size_t(slice[e >> 6]) + (e & 63));
+    const int4 dsc = desc[e >> 6];
+    const int32_t lane = e & 63;
     for (int32_t k = ptr[e], j = 0; k < ptr[e + 1]; ++k, ++j)
     {
-        int32_t *slot = base + 4 * 64 * size_t(j >> 1) + 2 * (j & 1);
+        int32_t id, wb;
         if (star2)
         {
-            const int32_t q = idx[k], f = pid(q);
-            slot[0] = f;
-            slot[1] = __float_as_int(fsg(q) * star2[f]);
+            const int32_t q = idx[k];
+            id = pid(q);
+            wb = __float_as_int(fsg(q) * star2[id]);
         }
         else
         {
-            slot[0] = idx[k];
-            slot[1] = __float_as_int(w[k]);
+            id = idx[k];
+            wb = __float_as_int(w[k]);
+        }
+        const size_t slot = 64 * size_t(j >> 1) + size_t(lane);
+        if (dsc.w)
+        {
+            int32_t *q = words + size_t(dsc.x) + 4 * slot + 2 * (j & 1);
+            q[0] = id;
+            q[1] = wb;
+        }
+        else
+        {
+            int32_t *q = words + size_t(dsc.x) + 3 * slot;
+            // the offset fits 16 bits (the host chose this slice compact)
+            q[0] = int32_t(uint32_t(q[0]) | (uint32_t(id - dsc.z) << (16 * (j & 1))));
+            q[1 + (j & 1)] = wb;
         }
     }
 }
@@ -803,37 +879,40 @@ void mhdg_ctx::mixed_build()
     M.s.recon = f32(d.recon, 3 * nc2f);
     M.s.ecw = f32(d.ecw, nec);
     {
-        // slice layout of the edge kernel's rows: slot counts on the device, their prefix sums here (set-up only)
+        // slice layout of the edge kernel's rows: per-slice scans on the device, the descriptors here (set-up only).
+        // MHDG_EDGE_ROWS_WIDE=1 keeps every slice wide (the same sums; tests compare the two layouts).
         const int32_t n_slices = int32_t((E + 63) / 64);
-        auto slices = [&](const int32_t *ptr, const int32_t *idx, const float *w, const float *star2, const int4 *&pairs,
-                          const int32_t *&slice) {
-            int32_t *off = static_cast<int32_t *>(alloc((size_t(n_slices) + 1) * sizeof(int32_t)));
-            hipLaunchKernelGGL(k_slice_pairs, dim3(nb(E)), dim3(256), 0, stream, ptr, d.e2f_ptr, int32_t(E), n_slices, off);
-            std::vector<int32_t> h(size_t(n_slices) + 1);
-            chk(hipMemcpyAsync(h.data(), off, size_t(n_slices) * sizeof(int32_t), hipMemcpyDeviceToHost, stream), "D2H");
+        const char *wide_env = std::getenv("MHDG_EDGE_ROWS_WIDE");
+        const bool force_wide = wide_env && wide_env[0] == '1';
+        auto slices = [&](const int32_t *ptr, const int32_t *idx, int32_t signed_ids, const float *w, const float *star2,
+                          const int4 *&desc_out, const int32_t *&words_out) {
+            int4 *desc = static_cast<int4 *>(alloc(size_t(std::max(n_slices, 1)) * sizeof(int4)));
+            hipLaunchKernelGGL(k_slice_scan, dim3(nb(E)), dim3(256), 0, stream, ptr, idx, signed_ids, d.e2f_ptr, int32_t(E), n_slices, desc);
+            std::vector<int4> h(static_cast<size_t>(n_slices));
+            chk(hipMemcpyAsync(h.data(), desc, h.size() * sizeof(int4), hipMemcpyDeviceToHost, stream), "D2H");
             chk(hipStreamSynchronize(stream), "sync");
             int64_t acc = 0;
-            for (int32_t k = 0; k < n_slices; ++k)
+            for (int4 &q : h)
             {
-                const int64_t n = 64 * int64_t(h[size_t(k)]);
-                h[size_t(k)] = int32_t(acc);
-                acc += n;
-                if (acc > INT32_MAX) throw std::runtime_error("edge rows too large for 32-bit slice offsets");
+                const int32_t np = q.x, lo = q.y, hi = q.z;
+                const bool wide = force_wide || (hi >= 0 && int64_t(hi) - lo > 0xffff);
+                if (wide) acc = (acc + 3) & ~int64_t(3); // a wide slice's 16-byte loads are aligned
+                q = make_int4(int32_t(acc), np, wide || hi < 0 ? 0 : lo, wide ? 1 : 0);
+                acc += int64_t(np) * 64 * (wide ? 4 : 3);
+                if (acc > INT32_MAX / 2) throw std::runtime_error("edge rows too large for 32-bit slice offsets");
             }
-            h[size_t(n_slices)] = int32_t(acc);
-            chk(hipMemcpyAsync(off, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream), "H2D");
+            chk(hipMemcpyAsync(desc, h.data(), h.size() * sizeof(int4), hipMemcpyHostToDevice, stream), "H2D");
             // + one group of zeroed slots: the kernel loads whole groups of pairs and drops those past a slice's count
-            const size_t slots = size_t(acc) + 64 * size_t(kPairUnroll);
-            int4 *p = static_cast<int4 *>(alloc(slots * sizeof(int4)));
-            chk(hipMemsetAsync(p, 0, slots * sizeof(int4), stream), "memset");
-            hipLaunchKernelGGL(k_slice_fill, dim3(nb(E)), dim3(256), 0, stream, ptr, d.e2f_ptr, idx, w, star2, int32_t(E), off,
-                               reinterpret_cast<int32_t *>(p));
+            const size_t nw = size_t(acc) + 64 * 4 * size_t(kPairUnroll);
+            int32_t *words = static_cast<int32_t *>(alloc(nw * sizeof(int32_t)));
+            chk(hipMemsetAsync(words, 0, nw * sizeof(int32_t), stream), "memset");
+            hipLaunchKernelGGL(k_slice_fill, dim3(nb(E)), dim3(256), 0, stream, ptr, d.e2f_ptr, idx, w, star2, int32_t(E), desc, words);
             chk(hipStreamSynchronize(stream), "sync"); // h is freed on return
-            pairs = p;
-            slice = off;
+            desc_out = desc;
+            words_out = words;
         };
-        slices(d.e2f_ptr, d.e2f, nullptr, M.s.star2, M.s.fpairs, M.s.fslice);
-        slices(d.ec_ptr, d.ec_cell, M.s.ecw, nullptr, M.s.cpairs, M.s.cslice);
+        slices(d.e2f_ptr, d.e2f, 1, nullptr, M.s.star2, M.s.fdesc, M.s.fwords);
+        slices(d.ec_ptr, d.ec_cell, 0, M.s.ecw, nullptr, M.s.cdesc, M.s.cwords);
         int8_t *eh = static_cast<int8_t *>(alloc(E));
         hipLaunchKernelGGL(k_heavy_flags, dim3(nb(E)), dim3(256), 0, stream, d.e2f_ptr, int32_t(E), eh);
         M.s.eheavy = eh;

@@ -166,6 +166,28 @@ def test_mixed_solve_matches_fp64(pair):
     assert maxrel(E32, E64) < 1e-6
 
 
+def test_edge_rows_compact_and_wide_agree(pair, monkeypatch):
+    """The FP32 edge kernel's row slices in their compact form (16-bit offsets from a slice's base id, 12-byte pairs)
+    and in their wide form (32-bit ids, 16-byte pairs; MHDG_EDGE_ROWS_WIDE=1) give the same mixed solve, and the same
+    multigrid-preconditioned solve (whose V-cycle runs the same kernel), bit for bit."""
+    from mpcns_uns_amd import Device
+
+    _, _, w, f = pair
+    rng = np.random.default_rng(9)
+    ne_eff = np.sqrt((f["W_H"][0::5] + f["W_Na"][0::5] / 23.0) ** 2 + 1e-4)
+    dirichlet = (w["mesh"].array("edge_mode") != 0).astype(np.int8)
+    E_expl = rng.normal(size=w["mesh"].n_edges) * 1e-3
+    runs = []
+    for wide in ("0", "1"):
+        monkeypatch.setenv("MHDG_EDGE_ROWS_WIDE", wide)
+        d = Device(w["mesh"], w["cfg"])
+        d.ind_set_frozen(f["B_tot"], ne_eff)
+        runs.append([d.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, precision=p) for p in (1, 2)])
+    for (Ea, ia, ra), (Eb, ib, rb) in zip(*runs):
+        assert ia == ib and ra == rb and ra < 1e-8
+        assert np.array_equal(Ea, Eb)
+
+
 def test_step_mixed_vs_fp64(small_hall):
     from mpcns_uns_amd import Device
 

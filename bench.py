@@ -28,7 +28,7 @@ PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 # what the same counters read for the library's streaming kernels over 2 GiB arrays (x2.00 and x1.00); the command
 # and the per-kernel table are in profiles/hall_step_shell_8m_hbm_counters.md.  Counter runs cannot share a process
 # with this measurement.
-PMC_TRAFFIC = {("shell_8m_mixed", "lat", "edge_apply_f32"): 1486461889}
+PMC_TRAFFIC = {("shell_8m_mixed", "lat", "edge_apply_f32"): 1239424502}
 
 
 def step_compulsory_bytes(its, restart, ns, mixed, vcycle=False):
@@ -77,9 +77,9 @@ def kernel_bytes(mesh, name, precision_bytes=8, ns=2):
         # alpha B0 (3C), b (3C), c2f ptr + idx, f2c, area, volume, g (3C) write
         "hall_g_f32": 12 * C + 4 * (C + 1) + 4 * nc2f + 8 * F + 12 * F + 4 * C + 12 * C + 12 * C,
         # dir, the two rows in compact CSR form (row offsets + (index, weight) entries: e2f with signed star2, edge-cell
-        # with weights), phi, edge vec, g, m_e, pv, v, out.  The kernel streams the rows in 64-edge slices padded to
-        # each slice's longest row plus a valence flag per edge (1.360 GB at 8M against these 1.269 GB); the count
-        # stays the layout-independent CSR size.
+        # with weights), phi, edge vec, g, m_e, pv, v, out.  The kernel streams the rows in 64-edge slices of 12-byte
+        # pairs with 16-bit offsets, padded to each slice's longest row, plus a valence flag per edge (1.135 GB at
+        # 8M against these 1.269 GB); the count stays the CSR size with 32-bit ids.
         "edge_apply_f32": E + 4 * (E + 1) + 8 * nf2e + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 4,
         # the same gather with the multigrid outputs: A x (x, out), a Jacobi sweep or a residual (x, b, invdiag, out)
         "mg_apply_f32": E + 4 * (E + 1) + 8 * nf2e + 4 * F + 12 * E + 4 * (E + 1) + 8 * nec + 12 * C + 4 * E * 3,

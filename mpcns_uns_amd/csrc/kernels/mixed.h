@@ -28,6 +28,11 @@ struct DevMesh32
     const int4 *cdesc;        // edge-cell rows, entries (cell, edge-cell weight)
     const int32_t *cwords;
     const int8_t *eheavy;     // 1 on the edges of valence above kLightMax (the edge kernel's heavy blocks)
+    // Tetrahedra in the four-lane cell-vector kernel (one wavefront = 16 cells): the face of each (cell, face) entry as
+    // a 16-bit offset from its wavefront's smallest face id; null when some wavefront's faces span 2^16 or more, and
+    // the kernel reads c2f instead.
+    const uint16_t *c2f16;
+    const int32_t *c2fbase;
     int32_t cdeg, fdeg;       // faces per cell / edges per face when uniform (tets: 4, 3), else 0 (ptr arrays)
 };
 

@@ -302,7 +302,14 @@ template <int G> __global__ void __launch_bounds__(256) k_bcell_f32_g(DevMesh d,
     const int32_t c = t / G, lane = t % G;
     const bool ok = c < d.nc;
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    if (ok)
+    if (ok && s.c2f16 && s.cdeg == G)
+    {
+        // one entry per lane, the face from its 16-bit offset (G == 4: a wavefront is 16 cells)
+        const int32_t k = c * G + lane;
+        const float v = phi[s.c2fbase[t >> 6] + s.c2f16[k]];
+        bx = s.recon[3 * size_t(k)] * v; by = s.recon[3 * size_t(k) + 1] * v; bz = s.recon[3 * size_t(k) + 2] * v;
+    }
+    else if (ok)
     {
         int32_t k0, k1;
         rows(s.cdeg, d.c2f_ptr, c, k0, k1);
@@ -329,6 +336,8 @@ template <int G> __global__ void __launch_bounds__(256) k_hall_g_f32_g(DevMesh d
     if (ok && s.cdeg == G)
     {
         // one entry per lane (tetrahedra with four lanes): no row bounds to wait for
+        // (a 16-bit neighbour offset from the cell measured 6.31 against 6.06 ms per 8M step: its decode sits between
+        // the load and the neighbour gather)
         const int32_t k = c * G + lane, nb = s.nbr[k];
         const F3 av = f3(s.aV, k), bc = f3(b, c);
         F3 bf = bc;
@@ -810,6 +819,29 @@ __global__ void __launch_bounds__(256) k_slice_fill(const int32_t *ptr, const in
         }
     }
 }
+// 16-bit face offsets of the four-lane FP32 cell-vector kernel on tetrahedra (DevMesh32::c2f16, c2fbase): per
+// wavefront of 16 cells the smallest face id and whether its faces span less than 2^16
+__global__ void __launch_bounds__(256) k_cell16_scan(DevMesh d, int32_t nc, int32_t n_groups, int32_t *base, int8_t *fits)
+{
+    const int32_t t = gid(), c = t >> 2;
+    int32_t lo = INT32_MAX, hi = -1;
+    if (c < nc) lo = hi = pid(d.c2f[t]);
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && (t >> 6) < n_groups)
+    {
+        base[t >> 6] = hi < 0 ? 0 : lo;
+        fits[t >> 6] = int8_t(hi < 0 || int64_t(hi) - lo <= 0xffff);
+    }
+}
+__global__ void __launch_bounds__(256) k_cell16_fill(DevMesh d, int32_t nc, const int32_t *base, uint16_t *c2f16)
+{
+    const int32_t t = gid(), c = t >> 2;
+    if (c < nc) c2f16[t] = uint16_t(pid(d.c2f[t]) - base[t >> 6]);
+}
 __global__ void __launch_bounds__(256) k_heavy_flags(const int32_t *e2f_ptr, int32_t ne, int8_t *flag)
 {
     const int32_t e = gid();
@@ -880,9 +912,9 @@ void mhdg_ctx::mixed_build()
     M.s.ecw = f32(d.ecw, nec);
     {
         // slice layout of the edge kernel's rows: per-slice scans on the device, the descriptors here (set-up only).
-        // MHDG_EDGE_ROWS_WIDE=1 keeps every slice wide (the same sums; tests compare the two layouts).
+        // MHDG_WIDE_IDS=1 keeps every slice wide and the cell kernels on 32-bit ids (the same sums; tests compare).
         const int32_t n_slices = int32_t((E + 63) / 64);
-        const char *wide_env = std::getenv("MHDG_EDGE_ROWS_WIDE");
+        const char *wide_env = std::getenv("MHDG_WIDE_IDS");
         const bool force_wide = wide_env && wide_env[0] == '1';
         auto slices = [&](const int32_t *ptr, const int32_t *idx, int32_t signed_ids, const float *w, const float *star2,
                           const int4 *&desc_out, const int32_t *&words_out) {
@@ -924,6 +956,25 @@ void mhdg_ctx::mixed_build()
         int32_t *nbr = static_cast<int32_t *>(alloc(nc2f * sizeof(int32_t)));
         hipLaunchKernelGGL(k_tables_f32, dim3(nb(nc)), dim3(256), 0, stream, d, aV, nbr);
         M.s.aV = aV; M.s.nbr = nbr;
+        // 16-bit face offsets for tetrahedra when every wavefront's faces fit (set-up only; MHDG_WIDE_IDS=1 keeps 32-bit ids)
+        const char *wide_env = std::getenv("MHDG_WIDE_IDS");
+        M.s.c2f16 = nullptr; M.s.c2fbase = nullptr;
+        if (cdeg == kCellLanes && !(wide_env && wide_env[0] == '1'))
+        {
+            const int32_t n_groups = int32_t((int64_t(nc) * kCellLanes + 63) / 64);
+            int32_t *base = static_cast<int32_t *>(alloc(size_t(n_groups) * sizeof(int32_t)));
+            int8_t *fits = static_cast<int8_t *>(alloc(size_t(n_groups)));
+            hipLaunchKernelGGL(k_cell16_scan, dim3(nb(int64_t(nc) * kCellLanes)), dim3(256), 0, stream, d, int32_t(nc), n_groups, base, fits);
+            std::vector<int8_t> hf(static_cast<size_t>(n_groups));
+            chk(hipMemcpyAsync(hf.data(), fits, hf.size(), hipMemcpyDeviceToHost, stream), "D2H");
+            chk(hipStreamSynchronize(stream), "sync");
+            if (std::all_of(hf.begin(), hf.end(), [](int8_t v) { return v != 0; }))
+            {
+                uint16_t *c16 = static_cast<uint16_t *>(alloc(nc2f * sizeof(uint16_t)));
+                hipLaunchKernelGGL(k_cell16_fill, dim3(nb(int64_t(nc) * kCellLanes)), dim3(256), 0, stream, d, int32_t(nc), base, c16);
+                M.s.c2f16 = c16; M.s.c2fbase = base;
+            }
+        }
         M.s.cdeg = cdeg; M.s.fdeg = fdeg;
     }
     M.pv = static_cast<float *>(alloc(E * sizeof(float)));

@@ -166,10 +166,10 @@ def test_mixed_solve_matches_fp64(pair):
     assert maxrel(E32, E64) < 1e-6
 
 
-def test_edge_rows_compact_and_wide_agree(pair, monkeypatch):
-    """The FP32 edge kernel's row slices in their compact form (16-bit offsets from a slice's base id, 12-byte pairs)
-    and in their wide form (32-bit ids, 16-byte pairs; MHDG_EDGE_ROWS_WIDE=1) give the same mixed solve, and the same
-    multigrid-preconditioned solve (whose V-cycle runs the same kernel), bit for bit."""
+def test_16_bit_and_32_bit_ids_agree(pair, monkeypatch):
+    """The FP32 operator's 16-bit ids (the edge kernel's compact row slices: offsets from a slice's base id in 12-byte
+    pairs; the cell kernels' face and neighbour offsets) and its 32-bit ids (MHDG_WIDE_IDS=1) give the same mixed solve,
+    and the same multigrid-preconditioned solve (whose V-cycle runs the same kernels), bit for bit."""
     from mpcns_uns_amd import Device
 
     _, _, w, f = pair
@@ -179,7 +179,7 @@ def test_edge_rows_compact_and_wide_agree(pair, monkeypatch):
     E_expl = rng.normal(size=w["mesh"].n_edges) * 1e-3
     runs = []
     for wide in ("0", "1"):
-        monkeypatch.setenv("MHDG_EDGE_ROWS_WIDE", wide)
+        monkeypatch.setenv("MHDG_WIDE_IDS", wide)
         d = Device(w["mesh"], w["cfg"])
         d.ind_set_frozen(f["B_tot"], ne_eff)
         runs.append([d.ind_solve(w["state"][2], E_expl, dirichlet, 2e-3, precision=p) for p in (1, 2)])

@@ -28,7 +28,7 @@ PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 # what the same counters read for the library's streaming kernels over 2 GiB arrays (x2.00 and x1.00); the command
 # and the per-kernel table are in profiles/hall_step_shell_8m_hbm_counters.md.  Counter runs cannot share a process
 # with this measurement.
-PMC_TRAFFIC = {("shell_8m_mixed", "lat", "edge_apply_f32"): 1239424502}
+PMC_TRAFFIC = {("shell_8m_mixed", "lat", "edge_apply_f32"): 1239373251}
 
 
 def step_compulsory_bytes(its, restart, ns, mixed, vcycle=False):

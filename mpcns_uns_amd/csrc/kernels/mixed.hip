@@ -244,10 +244,12 @@ __device__ __forceinline__ float row_sum(const int4 &dsc, const int32_t *words, 
 
 // out = dir ? 0 : (A P - I) v = (pv - v) + dt F(Phi') with F = m_e sum_f s star2_f Phi'_f + sum_c w_ec g_c . t_e
 // (the shifted operator: same Krylov space as A P, and the O(1) identity part never enters the FP32 vector)
-// The rows come in slices of 64 edges (DevMesh32::fdesc/fwords, cdesc/cwords): each pair load of a wavefront reads one contiguous
-// kilobyte (the per-edge CSR rows, 40 bytes apart, touched a cache line per lane), a row's pair loads are issued
-// before its gathers, padding slots are gathered and added as zero terms without a branch, and the sums run in
-// entry order.  Face row, then cell row: both rows' loads together took 74 VGPRs (six waves per SIMD) and ran slower.
+// The rows come in slices of 64 edges (DevMesh32::fdesc/fwords, cdesc/cwords): each pair load of a wavefront reads one
+// contiguous range, 768 bytes of 12-byte pairs with 16-bit offsets in a compact slice (the per-edge CSR rows, 40 bytes
+// apart, touched a cache line per lane), a row's pair loads are issued before its gathers, padding slots are gathered
+// and added as zero terms without a branch, and the sums run in entry order.  The kernel's time follows its L2
+// requests, so the compact pairs (a quarter fewer row bytes) took it from 278 to 240 us per launch on the 8M shell.
+// Face row, then cell row: both rows' loads together took 74 VGPRs (six waves per SIMD) and ran slower.
 // Blocks from n_light on take one high-valence edge each (heavy[block - n_light]) in the same launch.
 template <int MODE>
 __global__ void __launch_bounds__(kEdgeBlock) k_edge_apply_f32(DevMesh d, DevMesh32 s, int32_t resistive, int32_t hall, const float *phi,

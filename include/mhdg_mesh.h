@@ -90,8 +90,8 @@ int mhdg_mesh_seeded_state(const mhdg_mesh *m, const mhdg_config *cfg, double no
  * mesh: the stored incidences, numbering and orientation are used as they are, geometry is recomputed, and
  * the stored topology hash is verified.  A part file of a several-rank partition is refused. */
 int mhdg_mesh_read_unsbin(const char *path, mhdg_mesh **out);
-/* The n part files part_0 .. part_{n-1}.unsb of an n-rank partition (uns::io::write_local_mesh of each
- * uns::Distributor::build(r), src/uns/parallel/Distributor.cpp:107-160, written by the reference's preprocessing
+/* The n part files part_00000.unsb .. of an n-rank partition (uns::io::write_local_mesh of each
+ * uns::Distributor::build(r), src/uns/parallel/Distributor.cpp:115-221, written by the reference's preprocessing
  * pipeline, Preprocess/Pipeline.cpp:133-156) merged into the one global complex they were cut from, in its global
  * numbering: every entity's row comes from the file of its owner (the first n_owned entities of each dimension), its
  * column ids mapped through that file's global ids, signs as stored; each file's topology hash and every ghost copy are

@@ -78,7 +78,7 @@ uint64_t topology_hash(const Complex &cx);
 void write_complex(const std::string &path, const Complex &cx, const std::vector<double> &xyz, const std::vector<std::string> &tag_names);
 // Reads a complex file written for one rank (nranks = 1); verifies the stored topology hash.
 void read_complex(const std::string &path, Complex &cx, std::vector<double> &xyz, std::vector<std::string> &tag_names);
-// The n part files of an n-rank partition (uns::Distributor; src/uns/parallel/Distributor.cpp:107-160) merged into
+// The n part files of an n-rank partition (uns::Distributor; src/uns/parallel/Distributor.cpp:115-221) merged into
 // the global complex they were cut from, in its global numbering: each entity's row from its owner's file, column
 // ids through that file's global ids; each file's topology hash and every ghost copy are checked.
 void read_partition(const std::vector<std::string> &paths, Complex &cx, std::vector<double> &xyz, std::vector<std::string> &tag_names);
